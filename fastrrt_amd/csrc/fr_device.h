@@ -1,0 +1,129 @@
+/*
+ * fr_device.h — device-side plan descriptors for the gfx950 planning loop.
+ *
+ * One "plan" is one FastRRT search: a node grid of W x H cells kept in HBM
+ * as structure-of-arrays planes (type, parent, heading, cost, child count),
+ * the goal-connection planes computed once per plan, the per-plan obstacle
+ * distance plane the footprint test reads, and a small control block.  The
+ * same kernels advance one plan (the FastRRT handle: one plan per launch
+ * sequence, host-synchronised per loop() call) or hundreds of plans of
+ * uneven size at once (the batch engine: persistent grid-stride kernels over
+ * a flattened (plan, tile) table, captured as a hipGraph).
+ */
+#ifndef FR_DEVICE_H
+#define FR_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fr_math.h"
+
+#define FR_TILE 1024          /* cells per tile of the flattened tile table */
+#define FR_BLOCK 256          /* threads per block of the tile kernels */
+#define FR_DIST_CAP 63        /* cap of the Chebyshev obstacle-distance plane */
+#define FR_BIG_COST 99999999999LL
+
+enum FrStatus {
+    FR_IDLE = 0,
+    FR_SEARCHING = 1,
+    FR_FOUND = 2,     /* goal reached; path extraction pending or done */
+    FR_FAILED = 3,    /* iteration cap / deadline */
+};
+
+/* Parameters and plane pointers of one plan slot (read-only to kernels). */
+struct FrPlanDev {
+    int W, H, N;
+    int pad, PW, PH;          /* padded distance plane (W + 2 pad) x (H + 2 pad) */
+    int mx, mz;               /* half footprint (min_dist / 2, truncated) */
+    int lbx, lbz, ubx, ubz;   /* ego box, pixels inside are ignored */
+    float og_x, og_y;         /* map-frame origin in grid pixels */
+    float v, max_path, dist_tol, herr_tol;
+    float max_curv_loop;      /* curvature bound of the per-loop goal connection */
+    float max_curv_goal;      /* curvature bound of the per-plan precompute */
+    double max_steer_rad, lr;
+    int sx, sz, gx, gz;
+    float sh, gh;
+    int region_side, dens_w, dens_h;
+    int smart;
+    int max_iter;
+    uint64_t seed, stream;
+    /* node grid */
+    uint8_t *type;
+    int32_t *derive, *px, *pz;
+    float *heading, *cost;
+    float *intrinsic;         /* may be null: all-zero intrinsic costs */
+    /* goal connection */
+    uint8_t *gc_flag;
+    float *gc_heading, *gc_cost;
+    /* scratch planes */
+    unsigned long long *claim;
+    uint8_t *mark;
+    uint8_t *dist;            /* padded Chebyshev distance to the nearest obstacle pixel */
+    uint8_t *dist_tmp;
+    uint8_t *trav;            /* bit0: feasible for all headings; bit1: safe-zone bit 0x100 */
+    uint32_t *region;
+    /* input frame (float3 per pixel) and class-cost table */
+    const float *frame;
+    const float *class_costs;
+    int n_classes;
+    /* path buffers */
+    float4 *path;
+    float4 *pd;
+    int path_cap;
+};
+
+/* Mutable per-plan control block. */
+struct FrCtl {
+    int status;
+    int iter;
+    int last_temp;
+    int goal_reached;
+    int collision;
+    int n_node_pre;           /* NODE cells at expansion start */
+    int n_conv;               /* NODE -> COLLISION conversions this iteration */
+    int n_temp;               /* TEMP cells after erase */
+    int n_all;                /* non-NULL cells after erase */
+    int reseeded;             /* the start was re-added this iteration */
+    int pending_start;        /* the start cell must be goal-checked next iteration */
+    int frontier, force;
+    unsigned epoch;
+    int n_expansions;         /* rollouts queued this iteration */
+    int n_fresh;              /* nodes goal-checked this iteration */
+    long long best_lcost;     /* goal direct connection: min floor(100 * total) */
+    unsigned long long best_node_key;
+    int found;                /* direct connection added this iteration */
+    /* path extraction / optimisation */
+    long long bn_cost;
+    int bn_pos;
+    int path_len;
+    int opt_rounds;
+    int opt_changed;
+    float opt_gain_f;
+    int opt_best_i;
+    unsigned long long opt_key;
+    int full_check;           /* next connect pass must scan all NODE cells */
+    int pad0;
+};
+
+/* One queued expansion (one rollout). */
+struct FrExp {
+    int slot;
+    int src;
+    int end;                  /* -1: rejected */
+    float heading;
+    float cost;
+    int pad0;
+};
+
+/* One node to goal-check / connect this iteration. */
+struct FrFresh {
+    int slot;
+    int pos;
+    long long lcost;          /* node-local best, FR_BIG_COST if none */
+    int widx;                 /* window index of the node-local best */
+    int c_pad;
+    float c_cost;             /* node -> window-cell cost */
+    float total;              /* c_cost + window cell's goal cost */
+};
+
+#endif

@@ -1,0 +1,408 @@
+/*
+ * fr_engine.cpp — slab allocation and the launch engine of the planning loop.
+ */
+#include "fr_engine.h"
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+
+static thread_local std::string g_fr_error;
+
+void fr_set_error(const std::string &msg) { g_fr_error = msg; }
+const char *fr_get_error() { return g_fr_error.c_str(); }
+
+/* ------------------------------------------------------------------------ */
+/* event profiler                                                           */
+/* ------------------------------------------------------------------------ */
+
+namespace {
+struct ProfState {
+    std::mutex mu;
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> done[FR_PROF_N];
+    hipEvent_t open_ev[FR_PROF_N] = {};
+    std::vector<hipEvent_t> pool;
+    long launches[FR_PROF_N] = {};
+    double total_ms[FR_PROF_N] = {};
+};
+ProfState &prof()
+{
+    static ProfState p;
+    return p;
+}
+hipEvent_t prof_event(ProfState &p)
+{
+    if (!p.pool.empty()) {
+        hipEvent_t e = p.pool.back();
+        p.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    FR_CHECK(hipEventCreate(&e));
+    return e;
+}
+void prof_drain(ProfState &p, int k)
+{
+    for (auto &pr : p.done[k]) {
+        FR_CHECK(hipEventSynchronize(pr.second));
+        float ms = 0;
+        FR_CHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+        p.launches[k] += 1;
+        p.total_ms[k] += ms;
+        p.pool.push_back(pr.first);
+        p.pool.push_back(pr.second);
+    }
+    p.done[k].clear();
+}
+} // namespace
+
+void fr_prof_enable(bool on)
+{
+    std::lock_guard<std::mutex> g(prof().mu);
+    prof().on = on;
+}
+bool fr_prof_on() { return prof().on; }
+void fr_prof_begin(int k, hipStream_t st)
+{
+    ProfState &p = prof();
+    if (!p.on)
+        return;
+    std::lock_guard<std::mutex> g(p.mu);
+    hipEvent_t e = prof_event(p);
+    FR_CHECK(hipEventRecord(e, st));
+    p.open_ev[k] = e;
+}
+void fr_prof_end(int k, hipStream_t st)
+{
+    ProfState &p = prof();
+    if (!p.on)
+        return;
+    std::lock_guard<std::mutex> g(p.mu);
+    if (!p.open_ev[k])
+        return;
+    hipEvent_t e = prof_event(p);
+    FR_CHECK(hipEventRecord(e, st));
+    p.done[k].push_back({p.open_ev[k], e});
+    p.open_ev[k] = nullptr;
+    if (p.done[k].size() > 4096)
+        prof_drain(p, k);
+}
+void fr_prof_read(int k, long *launches, double *total_ms)
+{
+    ProfState &p = prof();
+    std::lock_guard<std::mutex> g(p.mu);
+    prof_drain(p, k);
+    *launches = p.launches[k];
+    *total_ms = p.total_ms[k];
+}
+void fr_prof_reset()
+{
+    ProfState &p = prof();
+    std::lock_guard<std::mutex> g(p.mu);
+    for (int k = 0; k < FR_PROF_N; ++k) {
+        prof_drain(p, k);
+        p.launches[k] = 0;
+        p.total_ms[k] = 0;
+    }
+}
+
+static size_t fr_align(size_t v) { return (v + 255) & ~(size_t)255; }
+
+/* ------------------------------------------------------------------------ */
+/* FrSlab                                                                   */
+/* ------------------------------------------------------------------------ */
+
+FrSlab::~FrSlab() { release(); }
+
+void FrSlab::release()
+{
+    if (base_)
+        (void)hipFree(base_);
+    if (dist_base_)
+        (void)hipFree(dist_base_);
+    base_ = nullptr;
+    dist_base_ = nullptr;
+    bytes_ = 0;
+}
+
+/* The padded distance planes depend on the footprint; they are reallocated
+ * alone so that the node grid survives a footprint change, as the reference's
+ * grid survives set_plan_data. */
+void FrSlab::set_footprint(int mx_, int mz_, hipStream_t st)
+{
+    if (dist_base_ && mx_ == mx && mz_ == mz)
+        return;
+    if (dist_base_)
+        (void)hipFree(dist_base_);
+    dist_base_ = nullptr;
+    mx = mx_;
+    mz = mz_;
+    const double hyp = std::sqrt((double)mx * mx + (double)mz * mz);
+    pad = (int)std::ceil(hyp) + 2;
+    PW = W + 2 * pad;
+    PH = H + 2 * pad;
+    const size_t PWH = (size_t)PW * PH;
+    FR_CHECK(hipMalloc(&dist_base_, 2 * fr_align(PWH)));
+    dist = (uint8_t *)dist_base_;
+    dist_tmp = (uint8_t *)dist_base_ + fr_align(PWH);
+    FR_CHECK(hipMemsetAsync(dist_base_, 0, 2 * fr_align(PWH), st));
+}
+
+void FrSlab::allocate(int W_, int H_, int mx_, int mz_, int path_cap_, bool with_intrinsic_, hipStream_t st)
+{
+    release();
+    W = W_;
+    H = H_;
+    mx = mx_;
+    mz = mz_;
+    with_intrinsic = with_intrinsic_;
+    path_cap = path_cap_ > 0 ? path_cap_ : W * H + 1;
+    const size_t N = (size_t)W * H;
+    const int dens = (W / 16 + 1) * (H / 16 + 1); /* fits every region side >= 16 */
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        size_t o = off;
+        off += fr_align(b);
+        return o;
+    };
+    const size_t o_type = take(N);
+    const size_t o_derive = take(4 * N);
+    const size_t o_px = take(4 * N);
+    const size_t o_pz = take(4 * N);
+    const size_t o_heading = take(4 * N);
+    const size_t o_cost = take(4 * N);
+    const size_t o_intr = with_intrinsic ? take(4 * N) : 0;
+    const size_t o_direct = with_intrinsic ? take(4 * N) : 0;
+    const size_t o_gcf = take(N);
+    const size_t o_gch = take(4 * N);
+    const size_t o_gcc = take(4 * N);
+    const size_t o_mark = take(N);
+    const size_t o_trav = take(N);
+    const size_t o_region = take(4 * (size_t)2 * dens);
+    const size_t o_path = take(16 * (size_t)path_cap);
+    const size_t o_pd = take(16 * (size_t)path_cap);
+    const size_t zero_end = off; /* everything above starts at zero */
+    const size_t o_claim = take(8 * N);
+    bytes_ = off;
+    FR_CHECK(hipMalloc(&base_, bytes_));
+    char *b = (char *)base_;
+    type = (uint8_t *)(b + o_type);
+    derive = (int32_t *)(b + o_derive);
+    px = (int32_t *)(b + o_px);
+    pz = (int32_t *)(b + o_pz);
+    heading = (float *)(b + o_heading);
+    cost = (float *)(b + o_cost);
+    intrinsic = with_intrinsic ? (float *)(b + o_intr) : nullptr;
+    direct = with_intrinsic ? (float *)(b + o_direct) : nullptr;
+    gc_flag = (uint8_t *)(b + o_gcf);
+    gc_heading = (float *)(b + o_gch);
+    gc_cost = (float *)(b + o_gcc);
+    mark = (uint8_t *)(b + o_mark);
+    trav = (uint8_t *)(b + o_trav);
+    region = (uint32_t *)(b + o_region);
+    path = (float4 *)(b + o_path);
+    pd = (float4 *)(b + o_pd);
+    claim = (unsigned long long *)(b + o_claim);
+    FR_CHECK(hipMemsetAsync(base_, 0, zero_end, st));
+    FR_CHECK(hipMemsetAsync(claim, 0xFF, 8 * N, st));
+    mx = -1;
+    mz = -1;
+    set_footprint(mx_, mz_, st);
+}
+
+FrFrame::~FrFrame()
+{
+    if (d_xyz)
+        (void)hipFree(d_xyz);
+    if (d_costs)
+        (void)hipFree(d_costs);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FrEngine                                                                 */
+/* ------------------------------------------------------------------------ */
+
+FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
+    : max_slots_(max_slots), queue_cap_(queue_cap), st_(stream), own_stream_(false)
+{
+    if (!st_) {
+        FR_CHECK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+        own_stream_ = true;
+    }
+    h_plans_.assign(max_slots, FrPlanDev{});
+    const long max_tiles = queue_cap / FR_TILE + max_slots + 1;
+    FR_CHECK(hipMalloc(&L_.plans, sizeof(FrPlanDev) * max_slots));
+    FR_CHECK(hipMalloc(&L_.ctl, sizeof(FrCtl) * max_slots));
+    FR_CHECK(hipMalloc(&L_.slots, sizeof(int) * max_slots));
+    FR_CHECK(hipMalloc(&d_setup_slots_, sizeof(int) * max_slots));
+    FR_CHECK(hipMalloc(&L_.tiles, sizeof(int2) * max_tiles));
+    FR_CHECK(hipMalloc(&L_.n_tiles, sizeof(int)));
+    FR_CHECK(hipMalloc(&L_.counters, sizeof(int) * FR_CNT_N));
+    FR_CHECK(hipMalloc(&L_.exp_q, sizeof(FrExp) * (queue_cap + 1)));
+    FR_CHECK(hipMalloc(&L_.fresh_q, sizeof(FrFresh) * (2 * queue_cap + max_slots + 1)));
+    FR_CHECK(hipMalloc(&d_scratch_, sizeof(int) * 64));
+    FR_CHECK(hipMemsetAsync(L_.ctl, 0, sizeof(FrCtl) * max_slots, st_));
+    FR_CHECK(hipMemsetAsync(L_.n_tiles, 0, sizeof(int), st_));
+    FR_CHECK(hipMemsetAsync(L_.counters, 0, sizeof(int) * FR_CNT_N, st_));
+    L_.grid_tiles = (int)std::min<long>(max_tiles, 2048);
+    L_.grid_items = (int)std::max<long>(1, std::min<long>((queue_cap + FR_BLOCK - 1) / FR_BLOCK, 2048));
+    L_.grid_slots = 1;
+}
+
+FrEngine::~FrEngine()
+{
+    (void)hipStreamSynchronize(st_);
+    (void)hipFree(L_.plans);
+    (void)hipFree(L_.ctl);
+    (void)hipFree(L_.slots);
+    (void)hipFree(d_setup_slots_);
+    (void)hipFree(L_.tiles);
+    (void)hipFree(L_.n_tiles);
+    (void)hipFree(L_.counters);
+    (void)hipFree(L_.exp_q);
+    (void)hipFree(L_.fresh_q);
+    (void)hipFree(d_scratch_);
+    if (own_stream_)
+        (void)hipStreamDestroy(st_);
+}
+
+void FrEngine::bind(int slot, FrSlab &s, const FrPlanParams &p, const FrFrameView &frame)
+{
+    FrPlanDev d{};
+    d.W = s.W;
+    d.H = s.H;
+    d.N = s.W * s.H;
+    d.pad = s.pad;
+    d.PW = s.PW;
+    d.PH = s.PH;
+    d.mx = s.mx;
+    d.mz = s.mz;
+    d.lbx = p.lbx;
+    d.lbz = p.lbz;
+    d.ubx = p.ubx;
+    d.ubz = p.ubz;
+    d.og_x = p.og_x;
+    d.og_y = p.og_y;
+    d.v = p.v;
+    d.max_path = p.max_path;
+    d.dist_tol = p.dist_tol;
+    d.herr_tol = p.herr_tol;
+    d.max_curv_loop = p.max_curv_loop;
+    d.max_curv_goal = p.max_curv_goal;
+    d.max_steer_rad = p.max_steer_rad;
+    d.lr = p.lr;
+    d.sx = p.sx;
+    d.sz = p.sz;
+    d.gx = p.gx;
+    d.gz = p.gz;
+    d.sh = p.sh;
+    d.gh = p.gh;
+    d.region_side = p.region_side;
+    d.dens_w = s.W / p.region_side + 1;
+    d.dens_h = s.H / p.region_side + 1;
+    d.smart = p.smart;
+    d.max_iter = p.max_iter;
+    d.seed = p.seed;
+    d.stream = p.stream;
+    d.type = s.type;
+    d.derive = s.derive;
+    d.px = s.px;
+    d.pz = s.pz;
+    d.heading = s.heading;
+    d.cost = s.cost;
+    d.intrinsic = s.intrinsic;
+    d.gc_flag = s.gc_flag;
+    d.gc_heading = s.gc_heading;
+    d.gc_cost = s.gc_cost;
+    d.claim = s.claim;
+    d.mark = s.mark;
+    d.dist = s.dist;
+    d.dist_tmp = s.dist_tmp;
+    d.trav = s.trav;
+    d.region = s.region;
+    d.frame = frame.xyz;
+    d.class_costs = frame.costs;
+    d.n_classes = frame.n_classes;
+    d.path = s.path;
+    d.pd = s.pd;
+    d.path_cap = s.path_cap;
+    h_plans_[slot] = d;
+    upload_plan(slot);
+}
+
+void FrEngine::upload_plan(int slot)
+{
+    FR_CHECK(hipMemcpyAsync(L_.plans + slot, &h_plans_[slot], sizeof(FrPlanDev), hipMemcpyHostToDevice, st_));
+}
+
+void FrEngine::reset_ctl(int slot, int status)
+{
+    FrCtl c{};
+    c.status = status;
+    c.pending_start = 1;
+    write_ctl(slot, c);
+}
+
+void FrEngine::read_ctl(int slot, FrCtl *out)
+{
+    FR_CHECK(hipMemcpyAsync(out, L_.ctl + slot, sizeof(FrCtl), hipMemcpyDeviceToHost, st_));
+    FR_CHECK(hipStreamSynchronize(st_));
+}
+
+void FrEngine::write_ctl(int slot, const FrCtl &c)
+{
+    /* the copy source must outlive the async copy: stage through a member */
+    static thread_local FrCtl staged;
+    staged = c;
+    FR_CHECK(hipMemcpyAsync(L_.ctl + slot, &staged, sizeof(FrCtl), hipMemcpyHostToDevice, st_));
+    FR_CHECK(hipStreamSynchronize(st_));
+}
+
+void FrEngine::set_active(const std::vector<int> &slots)
+{
+    active_ = slots;
+    h_tiles_.clear();
+    for (int s : slots) {
+        const int N = h_plans_[s].N;
+        const int nt = (N + FR_TILE - 1) / FR_TILE;
+        for (int t = 0; t < nt; ++t)
+            h_tiles_.push_back(make_int2(s, t));
+    }
+    const int nt = (int)h_tiles_.size();
+    if (!slots.empty())
+        FR_CHECK(hipMemcpyAsync(L_.slots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, st_));
+    if (nt)
+        FR_CHECK(hipMemcpyAsync(L_.tiles, h_tiles_.data(), sizeof(int2) * nt, hipMemcpyHostToDevice, st_));
+    static thread_local int staged_nt;
+    staged_nt = nt;
+    FR_CHECK(hipMemcpyAsync(L_.n_tiles, &staged_nt, sizeof(int), hipMemcpyHostToDevice, st_));
+    FR_CHECK(hipStreamSynchronize(st_));
+}
+
+void FrEngine::setup(const std::vector<int> &slots, bool goal_pass)
+{
+    if (slots.empty())
+        return;
+    int max_N = 0, max_PH = 0;
+    long max_PWH = 0;
+    for (int s : slots) {
+        const FrPlanDev &d = h_plans_[s];
+        max_N = std::max(max_N, d.N);
+        max_PH = std::max(max_PH, d.PH);
+        max_PWH = std::max(max_PWH, (long)d.PW * d.PH);
+    }
+    FR_CHECK(hipMemcpyAsync(d_setup_slots_, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, st_));
+    fr_launch_setup(L_, d_setup_slots_, (int)slots.size(), max_N, max_PH, max_PWH, goal_pass, st_);
+    FR_CHECK(hipGetLastError());
+    FR_CHECK(hipStreamSynchronize(st_));
+}
+
+void FrEngine::iteration()
+{
+    fr_launch_iteration(L_, (int)active_.size(), st_);
+    FR_CHECK(hipGetLastError());
+}
+
+void FrEngine::sync() { FR_CHECK(hipStreamSynchronize(st_)); }

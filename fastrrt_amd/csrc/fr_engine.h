@@ -1,0 +1,195 @@
+/*
+ * fr_engine.h — host-side engine over the planning kernels.
+ *
+ * FrEngine owns the device tables (plan descriptors, control blocks, the
+ * flattened tile table and the work queues) of up to `max_slots` plans and
+ * one HIP stream.  FrSlab owns the SoA planes of one plan.
+ */
+#ifndef FR_ENGINE_H
+#define FR_ENGINE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "fr_device.h"
+
+enum { FR_CNT_EXP = 0, FR_CNT_FRESH = 1, FR_CNT_N = 8 };
+
+struct FrLaunch {
+    FrPlanDev *plans;
+    FrCtl *ctl;
+    int *slots;
+    int2 *tiles;
+    int *n_tiles;
+    int *counters;
+    FrExp *exp_q;
+    FrFresh *fresh_q;
+    int grid_tiles, grid_items, grid_slots;
+};
+
+struct FrBestNodeArgs {
+    int x, z;
+    float heading, radius, max_herr;
+};
+
+/* kernel launchers (fr_kernels.hip) */
+void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, int max_PH, long max_PWH,
+                     bool goal_pass, hipStream_t st);
+void fr_launch_iteration(const FrLaunch &L, int n_slots, hipStream_t st);
+void fr_launch_clear(const FrLaunch &L, int slot, int N, hipStream_t st);
+void fr_launch_count(const FrLaunch &L, int slot, int N, int *d_out, hipStream_t st);
+void fr_launch_goal_scan(const FrLaunch &L, int slot, int N, int gx, int gz, float gh, float tol, float herr,
+                         hipStream_t st);
+void fr_launch_best_node(const FrLaunch &L, const int *d_slots, int n, int max_N, const FrBestNodeArgs *ov,
+                         hipStream_t st);
+void fr_launch_walk(const FrLaunch &L, const int *d_slots, int n, hipStream_t st);
+void fr_launch_opt_round(const FrLaunch &L, const int *d_slots, int n, int max_path, hipStream_t st);
+void fr_launch_graph_add(const FrLaunch &L, int slot, int x, int z, float h, int px, int pz, float cost, int type,
+                         int count_parent, hipStream_t st);
+void fr_launch_set_collision(const FrLaunch &L, int slot, int x, int z, int npx, int npz, float h, float cost,
+                             hipStream_t st);
+void fr_launch_derivate(const FrLaunch &L, int slot, float steer, float path_size, float v, int x, int z,
+                        int *d_out, hipStream_t st);
+void fr_launch_accept(const FrLaunch &L, int slot, int N, hipStream_t st);
+void fr_launch_solve_collisions(const FrLaunch &L, int slot, int N, int nodes, hipStream_t st);
+void fr_launch_region_count(const FrLaunch &L, int slot, int N, uint32_t *d_out, hipStream_t st);
+
+/* Optional per-kernel timing with HIP events recorded on the launching
+ * stream around the kernels named below. */
+enum FrProfKernel { FR_PROF_GOAL_CONNECTION = 0, FR_PROF_EXPAND = 1, FR_PROF_CONNECT = 2, FR_PROF_ITERATION = 3,
+                    FR_PROF_N = 4 };
+void fr_prof_enable(bool on);
+bool fr_prof_on();
+void fr_prof_begin(int k, hipStream_t st);
+void fr_prof_end(int k, hipStream_t st);
+/* synchronises the recorded events; returns launches and total ms */
+void fr_prof_read(int k, long *launches, double *total_ms);
+void fr_prof_reset();
+
+/* error plumbing: every HIP call of the library goes through FR_CHECK */
+void fr_set_error(const std::string &msg);
+const char *fr_get_error();
+#define FR_CHECK(expr)                                                                                   \
+    do {                                                                                                 \
+        hipError_t _e = (expr);                                                                          \
+        if (_e != hipSuccess) {                                                                          \
+            fr_set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                             \
+            throw std::runtime_error(std::string("fastrrt: ") + #expr + ": " + hipGetErrorString(_e));   \
+        }                                                                                                \
+    } while (0)
+
+/* Parameters one plan is created with. */
+struct FrPlanParams {
+    int W = 0, H = 0;
+    int min_dx = 0, min_dz = 0;   /* full min-distance (halved internally) */
+    int lbx = -1, lbz = -1, ubx = -1, ubz = -1;
+    float og_x = 0, og_y = 0;
+    float v = 1.0f, max_path = 40.0f, dist_tol = 20.0f, herr_tol = 0.0f;
+    float max_curv_loop = 0.0f, max_curv_goal = 0.8f;
+    double max_steer_rad = 0, lr = 0;
+    int sx = 0, sz = 0, gx = 0, gz = 0;
+    float sh = 0, gh = 0;
+    int region_side = 128;
+    int smart = 0;
+    int max_iter = 0;
+    uint64_t seed = 0, stream = 0;
+    int path_cap = 0;             /* 0: W*H + 1 */
+    bool with_intrinsic = false;
+};
+
+/* Device planes of one plan (one allocation, sub-allocated). */
+class FrSlab {
+  public:
+    FrSlab() = default;
+    ~FrSlab();
+    FrSlab(const FrSlab &) = delete;
+    FrSlab &operator=(const FrSlab &) = delete;
+    void allocate(int W, int H, int mx, int mz, int path_cap, bool with_intrinsic, hipStream_t st);
+    void release();
+    void set_footprint(int mx, int mz, hipStream_t st);
+    size_t bytes() const { return bytes_; }
+    bool allocated() const { return base_ != nullptr; }
+    int W = 0, H = 0, pad = 0, PW = 0, PH = 0, mx = -1, mz = -1, path_cap = 0;
+    bool with_intrinsic = false;
+    uint8_t *type = nullptr;
+    int32_t *derive = nullptr, *px = nullptr, *pz = nullptr;
+    float *heading = nullptr, *cost = nullptr, *intrinsic = nullptr, *direct = nullptr;
+    uint8_t *gc_flag = nullptr;
+    float *gc_heading = nullptr, *gc_cost = nullptr;
+    unsigned long long *claim = nullptr;
+    uint8_t *mark = nullptr, *dist = nullptr, *dist_tmp = nullptr, *trav = nullptr;
+    uint32_t *region = nullptr;
+    float4 *path = nullptr, *pd = nullptr;
+
+  private:
+    void *base_ = nullptr;
+    void *dist_base_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+/* Device pointers a plan reads its obstacles from. */
+struct FrFrameView {
+    const float *xyz = nullptr;   /* W*H float3 */
+    const float *costs = nullptr; /* class-cost table */
+    int n_classes = 0;
+};
+
+/* A device-resident frame: float3 pixels + class-cost table. */
+struct FrFrame {
+    int W = 0, H = 0, n_classes = 0;
+    float *d_xyz = nullptr;
+    float *d_costs = nullptr;
+    std::vector<float> h_costs;
+    ~FrFrame();
+};
+
+class FrEngine {
+  public:
+    FrEngine(int max_slots, long queue_cap, hipStream_t stream = nullptr);
+    ~FrEngine();
+    FrEngine(const FrEngine &) = delete;
+    FrEngine &operator=(const FrEngine &) = delete;
+
+    hipStream_t stream() const { return st_; }
+    const FrLaunch &launch() const { return L_; }
+    int max_slots() const { return max_slots_; }
+
+    /* describe plan `slot` (slab + params + frame) and upload its descriptor */
+    void bind(int slot, FrSlab &slab, const FrPlanParams &p, const FrFrameView &frame);
+    FrPlanDev &host_plan(int slot) { return h_plans_[slot]; }
+    void upload_plan(int slot);
+
+    /* control blocks */
+    void reset_ctl(int slot, int status);
+    void read_ctl(int slot, FrCtl *out);           /* synchronous */
+    void write_ctl(int slot, const FrCtl &c);
+
+    /* the set of plans the iteration kernels walk */
+    void set_active(const std::vector<int> &slots);
+    int n_active() const { return (int)active_.size(); }
+    const int *d_active() const { return L_.slots; }
+
+    void setup(const std::vector<int> &slots, bool goal_pass = true); /* per-plan precompute */
+    void iteration();                              /* one loop() for every active plan */
+    void sync();
+
+    long queue_cap() const { return queue_cap_; }
+    int *d_scratch_int() { return d_scratch_; }
+
+  private:
+    int max_slots_;
+    long queue_cap_;
+    hipStream_t st_;
+    bool own_stream_;
+    FrLaunch L_{};
+    std::vector<FrPlanDev> h_plans_;
+    std::vector<int> active_;
+    std::vector<int2> h_tiles_;
+    int *d_setup_slots_ = nullptr;
+    int *d_scratch_ = nullptr;
+};
+
+#endif

@@ -1,0 +1,1473 @@
+/*
+ * fr_kernels.hip — gfx950 kernels of the FastRRT planning loop.
+ *
+ * Work decomposition (one kernel per reference phase, all persistent and
+ * grid-stride so that one launch covers any number of plans of uneven size):
+ *   - cell kernels walk a flattened table of (plan slot, tile) pairs, 1024
+ *     cells per tile, 4 cells per lane;
+ *   - rollout and goal-connection kernels walk compacted queues built by the
+ *     cell kernels (block-level prefix sums + one atomic per block), so the
+ *     expensive per-node work scales with the number of nodes, not W*H;
+ *   - per-plan decisions (frontier mode, collision solve, reseed, the goal
+ *     connection adds) run on the device from each plan's control block, so
+ *     a loop iteration needs no host round trip.
+ *
+ * Results are deterministic and independent of how plans are batched: each
+ * expansion draws from Philox keyed by (seed, stream, iteration, cell),
+ * every read of the grid in a phase sees the state before that phase, and
+ * all races of the reference are resolved by ordered keys (the lowest source
+ * cell claims an end cell; argmins break ties on the lowest cell).
+ *
+ * The footprint test (reference src/cuda/path_feasible.cu:18-78) samples the
+ * same (2mz+1)(2mx+1) rotated lattice with the same FP32 arithmetic, but
+ * looks each sample up in a padded Chebyshev distance-to-obstacle plane: a
+ * sample at distance d certifies the next d-2 samples of its row, which are
+ * skipped.  The boolean is identical to sampling every point.
+ */
+#include <hip/hip_runtime.h>
+
+#include "fr_device.h"
+#include "fr_engine.h"
+
+/* ------------------------------------------------------------------------ */
+/* device helpers                                                           */
+/* ------------------------------------------------------------------------ */
+
+__device__ __forceinline__ int fr_host_trunc_d(double v)
+{
+    /* x86 host conversion: NaN / overflow -> INT_MIN (the reference runs path
+     * optimisation on the host, src/cuda/graph_optimization.cu:92-124) */
+    if (v != v || v >= 2147483648.0 || v <= -2147483649.0)
+        return (-2147483647 - 1);
+    return (int)v;
+}
+
+__device__ __forceinline__ unsigned long long fr_lcost_key(long long lc)
+{
+    return (unsigned long long)(lc + (1LL << 62));
+}
+
+/* path_feasible.cu:13-78, sample-identical, with distance-plane skipping. */
+__device__ bool dev_feasible(const FrPlanDev &P, int x, int z, float h)
+{
+    const int pos = z * P.W + x;
+    if (P.trav[pos] & 1)
+        return true;
+    float s, c;
+    fr_sincosf(h, &s, &c);
+    const uint8_t *D = P.dist;
+    const int PW = P.PW, pad = P.pad;
+    if (c != c || s != s) /* every sample truncates to pixel (0, 0) */
+        return D[pad * PW + pad] != 0;
+    const float fxc = (float)x, fzc = (float)z;
+    const int mx = P.mx, mz = P.mz;
+    for (int i = -mz; i <= mz; ++i) {
+        const float is = (float)i * s;
+        const float ic = (float)i * c;
+        int j = -mx;
+        while (j <= mx) {
+            const float fj = (float)j;
+            const float fx = fj * c - is + fxc;
+            const float fz = fj * s + ic + fzc;
+            const int xl = (int)fx;
+            const int zl = (int)fz;
+            const int d = D[(zl + pad) * PW + (xl + pad)];
+            if (d == 0)
+                return false;
+            j += (d > 2) ? (d - 1) : 1;
+        }
+    }
+    return true;
+}
+
+/* kinematics.cu:123-231.  Returns 0 when rejected. */
+__device__ int dev_rollout(const FrPlanDev &P, int sx, int sz, float steer, float path_size, int *ex, int *ez,
+                           float *eh, float *ec)
+{
+    const int W = P.W, H = P.H;
+    float x = P.og_y - (float)sz;
+    float y = (float)sx - P.og_x;
+    if ((double)steer > P.max_steer_rad)
+        steer = (float)P.max_steer_rad;
+    else if ((double)steer < -P.max_steer_rad)
+        steer = (float)(-P.max_steer_rad);
+    const float t = fr_tanf(steer);
+    const float dt = (float)0.1;
+    const float ds = P.v * dt;
+    const float beta = fr_atanf(t / 2.0f);
+    const float hif = (float)((double)(ds * fr_cosf(beta) * t) / (2.0 * P.lr));
+    const int max_size = fr_trunc_f(path_size) + 1;
+    int size = 0;
+    int last_x = sx, last_z = sz;
+    const int start_pos = sz * W + sx;
+    float heading = P.heading[start_pos];
+    float node_cost = P.cost[start_pos];
+    int loop_count = 0;
+    const int loop_max = fr_trunc_f((float)(100 * max_size) / dt);
+    const double ogx = (double)P.og_x, ogy = (double)P.og_y;
+    while (size < max_size) {
+        if (loop_count++ > loop_max)
+            break;
+        float sn, cs;
+        fr_sincosf(heading + beta, &sn, &cs);
+        x += ds * cs;
+        y += ds * sn;
+        heading += hif;
+        const int cx = fr_trunc_d(ogx + (double)y);
+        const int cz = fr_trunc_d(ogy - (double)x);
+        if (cx == last_x && cz == last_z)
+            continue;
+        if (cx < 0 || cx >= W)
+            break;
+        if (cz < 0 || cz >= H)
+            break;
+        size += 1;
+        node_cost += (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
+        if (!dev_feasible(P, cx, cz, heading))
+            return 0;
+        last_x = cx;
+        last_z = cz;
+    }
+    if (size < 1)
+        return 0;
+    *ex = last_x;
+    *ez = last_z;
+    *eh = heading;
+    *ec = node_cost;
+    return 1;
+}
+
+/* graph_direct_goal_connection.cu:16-167.  The curve is walked twice: the
+ * first pass evaluates the curvature bound and the always-sampled centre
+ * pixel of every point (both cheap); only curves that survive get footprint
+ * tests.  The result is the same as the reference's single pass because any
+ * failing point makes the whole connection fail. */
+__device__ float dev_check_direct(const FrPlanDev &P, float max_curv, int x, int z, float lh, int gx, int gz,
+                                  float gh, bool safe_zone, bool host_nan)
+{
+    const int W = P.W, H = P.H;
+    const int pos = z * W + x;
+    const int dxi = gx - x, dzi = gz - z;
+    const float distance = sqrtf((float)(dxi * dxi + dzi * dzi));
+    const int num_points = fr_trunc_f(distance);
+    if (num_points <= 0)
+        return -1;
+    const float a1 = (float)((double)lh - FR_HALF_PI_D);
+    const float a2 = (float)((double)gh - FR_HALF_PI_D);
+    float s1, c1, s2, c2;
+    fr_sincosf(a1, &s1, &c1);
+    fr_sincosf(a2, &s2, &c2);
+    const double t1x = (double)(distance * c1), t1y = (double)(distance * s1);
+    const double t2x = (double)(distance * c2), t2y = (double)(distance * s2);
+    const double dx0 = (double)x, dz0 = (double)z, dgx = (double)gx, dgz = (double)gz;
+    const bool skip_fp = safe_zone && ((P.trav[pos] & 2) != 0);
+    const double denom = (double)(num_points - 1);
+    float node_cost = P.cost[pos];
+    int last_x = -1, last_z = -1;
+    for (int pass = 0; pass < 2; ++pass) {
+        last_x = -1;
+        last_z = -1;
+        for (int i = 0; i < num_points; ++i) {
+            const double t = (0.0 + (double)i) / denom;
+            const double t2 = t * t, t3 = t2 * t;
+            const double h00 = 2 * t3 - 3 * t2 + 1;
+            const double h10 = t3 - 2 * t2 + t;
+            const double h01 = -2 * t3 + 3 * t2;
+            const double h11 = t3 - t2;
+            const double px = h00 * dx0 + h10 * t1x + h01 * dgx + h11 * t2x;
+            const double pz = h00 * dz0 + h10 * t1y + h01 * dgz + h11 * t2y;
+            if (px < 0 || px >= W)
+                continue;
+            if (pz < 0 || pz >= H)
+                continue;
+            const int cx = host_nan ? fr_host_trunc_d(px) : fr_trunc_d(px);
+            const int cz = host_nan ? fr_host_trunc_d(pz) : fr_trunc_d(pz);
+            if (cx == last_x && cz == last_z)
+                continue;
+            if (cx < 0 || cx >= W)
+                continue;
+            if (cz < 0 || cz >= H)
+                continue;
+            if (pass == 1)
+                node_cost += (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
+            const double t00 = 6 * t2 - 6 * t;
+            const double t10 = 3 * t2 - 4 * t + 1;
+            const double t01 = -6 * t2 + 6 * t;
+            const double t11 = 3 * t2 - 2 * t;
+            const double ddx = t00 * dx0 + t10 * t1x + t01 * dgx + t11 * t2x;
+            const double ddz = t00 * dz0 + t10 * t1y + t01 * dgz + t11 * t2y;
+            last_x = cx;
+            last_z = cz;
+            if (pass == 0) {
+                if (max_curv > 0) {
+                    const double d00 = 12 * t - 6;
+                    const double d10 = 6 * t - 4;
+                    const double d01 = -12 * t + 6;
+                    const double d11 = 6 * t - 2;
+                    const double dd2x = d00 * dx0 + d10 * t1x + d01 * dgx + d11 * t2x;
+                    const double dd2z = d00 * dz0 + d10 * t1y + d01 * dgz + d11 * t2y;
+                    const double v = ddx * ddx + ddz * ddz;
+                    const double cr = ddx * dd2z - ddz * dd2x;
+                    const float k = (float)((cr < 0 ? -cr : cr) / (v * __builtin_sqrt(v)));
+                    if (k > max_curv)
+                        return -1;
+                }
+                if (!skip_fp && !(P.trav[cz * W + cx] & 1) && P.dist[(cz + P.pad) * P.PW + cx + P.pad] == 0)
+                    return -1;
+            } else if (!skip_fp) {
+                const float heading = (float)((double)fr_atan2f((float)ddz, (float)ddx) + FR_HALF_PI_D);
+                if (!dev_feasible(P, cx, cz, heading))
+                    return -1;
+            }
+        }
+    }
+    if (last_x != gx && last_z != gz)
+        return -1;
+    return node_cost;
+}
+
+/* graph.cu:219-234 */
+__device__ __forceinline__ double dev_compute_heading(int x1, int z1, int x2, int z2)
+{
+    const double dz = (double)(z2 - z1), dx = (double)(x2 - x1);
+    if (dx == 0 && dz == 0)
+        return 0;
+    double v1;
+    if (dz != 0)
+        v1 = (double)fr_atan2f((float)(-dz), (float)dx);
+    else
+        v1 = (double)fr_atan2f(0.0f, (float)dx);
+    return FR_HALF_PI_D - v1;
+}
+
+/* graph.cu:364-374 executed on the device */
+__device__ void dev_add(const FrPlanDev &P, int x, int z, float h, int ppx, int ppz, float cost)
+{
+    if (x < 0 || x >= P.W || z < 0 || z >= P.H)
+        return;
+    const int pos = z * P.W + x;
+    if (ppx != -1 && ppz != -1) {
+        const long pp = (long)ppz * P.W + ppx;
+        if (pp >= 0 && pp < P.N)
+            P.derive[pp]++;
+    }
+    P.type[pos] = FR_TYPE_NODE;
+    P.px[pos] = ppx;
+    P.pz[pos] = ppz;
+    P.heading[pos] = h;
+    P.cost[pos] = cost;
+}
+
+__device__ __forceinline__ bool dev_goal_hit(const FrPlanDev &P, int x, int z, float h)
+{
+    const double dx = (double)(P.gx - x), dz = (double)(P.gz - z);
+    if (__builtin_sqrt(dx * dx + dz * dz) > (double)P.dist_tol)
+        return false;
+    return fabsf(h - P.gh) <= P.herr_tol;
+}
+
+/* Block-wide exclusive scan of one int per thread; returns the block total. */
+__device__ int block_exclusive_scan(int v, int *out_prefix)
+{
+    __shared__ int s_wave[FR_BLOCK / 64];
+    __shared__ int s_total;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        int n = __shfl_up(incl, off, 64);
+        if (lane >= off)
+            incl += n;
+    }
+    if (lane == 63)
+        s_wave[wid] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int w = 0; w < FR_BLOCK / 64; ++w) {
+            int t = s_wave[w];
+            s_wave[w] = acc;
+            acc += t;
+        }
+        s_total = acc;
+    }
+    __syncthreads();
+    *out_prefix = s_wave[wid] + incl - v;
+    int total = s_total;
+    __syncthreads();
+    return total;
+}
+
+/* ------------------------------------------------------------------------ */
+/* per-plan setup                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* Frame -> traversability bits and the raw obstacle mask.  An obstacle is a
+ * pixel whose class cost is negative (path_feasible.cu:472-475); pixels in
+ * the ego box are skipped by the reference (:469-470) and count as free; a
+ * class id outside the table counts as an obstacle. */
+__global__ void k_setup_prep(const FrPlanDev *plans, const int *slots)
+{
+    const FrPlanDev &P = plans[slots[blockIdx.y]];
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+        const int z = pos / P.W, x = pos - z * P.W;
+        const float *f = P.frame + 3 * (long)pos;
+        const int trav = fr_trunc_f(f[2]);
+        P.trav[pos] = (uint8_t)((trav & 1) | ((trav & 0x100) ? 2 : 0));
+        const int cls = fr_trunc_f(f[0]);
+        int obst = (cls < 0 || cls >= P.n_classes) ? 1 : (P.class_costs[cls] < 0);
+        if (x >= P.lbx && x <= P.ubx && z >= P.ubz && z <= P.lbz)
+            obst = 0;
+        P.mark[pos] = (uint8_t)obst;
+    }
+}
+
+/* Row pass of the padded Chebyshev distance plane: horizontal distance to the
+ * nearest obstacle pixel of the same row (capped). */
+__global__ void k_setup_rowdist(const FrPlanDev *plans, const int *slots)
+{
+    const FrPlanDev &P = plans[slots[blockIdx.y]];
+    const int zp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (zp >= P.PH)
+        return;
+    const int z = zp - P.pad;
+    uint8_t *row = P.dist_tmp + (long)zp * P.PW;
+    int d = FR_DIST_CAP;
+    for (int xp = 0; xp < P.PW; ++xp) {
+        const int x = xp - P.pad;
+        const bool ob = (z >= 0 && z < P.H && x >= 0 && x < P.W) && P.mark[z * P.W + x];
+        d = ob ? 0 : (d < FR_DIST_CAP ? d + 1 : FR_DIST_CAP);
+        row[xp] = (uint8_t)d;
+    }
+    d = FR_DIST_CAP;
+    for (int xp = P.PW - 1; xp >= 0; --xp) {
+        const int cur = row[xp];
+        d = (cur == 0) ? 0 : (d < FR_DIST_CAP ? d + 1 : FR_DIST_CAP);
+        if (d < cur)
+            row[xp] = (uint8_t)d;
+    }
+}
+
+/* Column pass: D(x, z) = min_dz max(|dz|, rowdist(x, z + dz)), capped. */
+__global__ void k_setup_coldist(const FrPlanDev *plans, const int *slots)
+{
+    const FrPlanDev &P = plans[slots[blockIdx.y]];
+    const long total = (long)P.PW * P.PH;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const int zp = (int)(i / P.PW), xp = (int)(i - (long)zp * P.PW);
+        int best = P.dist_tmp[i];
+        for (int dz = 1; dz < best; ++dz) {
+            if (zp - dz >= 0) {
+                int v = P.dist_tmp[(long)(zp - dz) * P.PW + xp];
+                v = v > dz ? v : dz;
+                if (v < best)
+                    best = v;
+            }
+            if (zp + dz < P.PH) {
+                int v = P.dist_tmp[(long)(zp + dz) * P.PW + xp];
+                v = v > dz ? v : dz;
+                if (v < best)
+                    best = v;
+            }
+        }
+        P.dist[i] = (uint8_t)best;
+    }
+}
+
+/* graph_direct_goal_connection.cu:169-198: per cell, Hermite to the goal. */
+__global__ void k_setup_goal_connection(const FrPlanDev *plans, const int *slots)
+{
+    const FrPlanDev &P = plans[slots[blockIdx.y]];
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+        const int z = pos / P.W, x = pos - z * P.W;
+        P.gc_flag[pos] = 0;
+        const float lh = (float)dev_compute_heading(x, z, P.gx, P.gz);
+        const float c = dev_check_direct(P, P.max_curv_goal, x, z, lh, P.gx, P.gz, P.gh, false, false);
+        if (c < 0)
+            continue;
+        P.gc_flag[pos] = 1;
+        P.gc_heading[pos] = lh;
+        P.gc_cost[pos] = c;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* loop iteration                                                           */
+/* ------------------------------------------------------------------------ */
+
+__global__ void k_begin(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n_slots, int *counters)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        counters[FR_CNT_EXP] = 0;
+        counters[FR_CNT_FRESH] = 0;
+    }
+    if (i >= n_slots)
+        return;
+    const int s = slots[i];
+    FrCtl &C = ctl[s];
+    const FrPlanDev &P = plans[s];
+    if (C.status != FR_SEARCHING)
+        return;
+    C.frontier = C.last_temp >= 100;
+    C.force = C.last_temp == 0;
+    C.collision = 0;
+    C.n_node_pre = 0;
+    C.n_conv = 0;
+    C.n_temp = 0;
+    C.n_all = 0;
+    C.reseeded = 0;
+    C.found = 0;
+    C.n_expansions = 0;
+    C.n_fresh = 0;
+    C.best_lcost = FR_BIG_COST;
+    C.best_node_key = ~0ull;
+    C.epoch += 1;
+    if (P.smart) {
+        const int nr = P.dens_w * P.dens_h;
+        uint32_t *wr = P.region + ((C.iter + 1) & 1) * nr;
+        for (int r = 0; r < nr; ++r)
+            wr[r] = 0;
+    }
+}
+
+/* Gate of __CUDA_random_node_expansion (graph_expansion.cu:157-164) and of
+ * __CUDA_smart_node_expansion (graph_expansion_search_zones.cu:461-473):
+ * queue one rollout per eligible NODE cell. */
+__global__ void k_select(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, const int *n_tiles_p,
+                         FrExp *q, int *counters)
+{
+    const int n_tiles = *n_tiles_p;
+    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const int2 tl = tiles[t];
+        const FrPlanDev &P = plans[tl.x];
+        FrCtl &C = ctl[tl.x];
+        if (C.status != FR_SEARCHING)
+            continue;
+        const int base = tl.y * FR_TILE;
+        int mine[4];
+        int cnt = 0, nodes = 0;
+        const int nr = P.dens_w * P.dens_h;
+        const uint32_t *rd = P.region + (C.iter & 1) * nr;
+        for (int k = 0; k < 4; ++k) {
+            const int pos = base + k * FR_BLOCK + threadIdx.x;
+            mine[k] = -1;
+            if (pos >= P.N || P.type[pos] != FR_TYPE_NODE)
+                continue;
+            nodes++;
+            bool ok;
+            if (P.smart) {
+                if (C.force)
+                    ok = true;
+                else if (C.frontier)
+                    ok = P.derive[pos] == 0;
+                else {
+                    const int z = pos / P.W, x = pos - z * P.W;
+                    const int dp = (z / P.region_side) * P.dens_w + x / P.region_side;
+                    ok = (double)rd[dp] <= 0.5 * (double)P.region_side;
+                }
+            } else {
+                ok = !(C.frontier && P.derive[pos] > 0);
+            }
+            if (ok) {
+                mine[k] = pos;
+                cnt++;
+            }
+        }
+        int prefix;
+        const int total = block_exclusive_scan(cnt, &prefix);
+        int nodes_prefix;
+        const int node_total = block_exclusive_scan(nodes, &nodes_prefix);
+        __shared__ int s_base;
+        if (threadIdx.x == 0) {
+            s_base = total ? atomicAdd(&counters[FR_CNT_EXP], total) : 0;
+            if (node_total)
+                atomicAdd(&C.n_node_pre, node_total);
+            if (total)
+                atomicAdd(&C.n_expansions, total);
+        }
+        __syncthreads();
+        int w = s_base + prefix;
+        for (int k = 0; k < 4; ++k)
+            if (mine[k] >= 0) {
+                FrExp e;
+                e.slot = tl.x;
+                e.src = mine[k];
+                e.end = -1;
+                e.heading = 0;
+                e.cost = 0;
+                e.pad0 = 0;
+                q[w++] = e;
+            }
+        __syncthreads();
+    }
+}
+
+/* expand_node (graph_expansion.cu:103-145) split in two: rollout + claim here,
+ * the commit in k_commit, so every rollout reads the pre-expansion grid. */
+__global__ void k_expand(const FrPlanDev *plans, FrCtl *ctl, FrExp *q, const int *counters)
+{
+    const int n = counters[FR_CNT_EXP];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        FrExp &e = q[i];
+        const FrPlanDev &P = plans[e.slot];
+        const FrCtl &C = ctl[e.slot];
+        const int src = e.src;
+        const int z = src / P.W, x = src - z * P.W;
+        float us, ul;
+        fr_expansion_draws(P.seed, P.stream, (uint32_t)C.iter, (uint32_t)src, &us, &ul);
+        const float steer = (2.0f * us - 1.0f) * (float)P.max_steer_rad;
+        float psize = P.max_path * ul;
+        if (psize < 5.0f)
+            psize = 5.0f;
+        int ex, ez;
+        float eh, ec;
+        if (!dev_rollout(P, x, z, steer, psize, &ex, &ez, &eh, &ec))
+            continue;
+        if ((float)ex == P.og_x && (float)ez == P.og_y)
+            continue;
+        const int epos = ez * P.W + ex;
+        if (epos == src)
+            continue;
+        const int st = P.type[epos];
+        if (st != FR_TYPE_NULL && st != FR_TYPE_NODE)
+            continue;
+        e.end = epos;
+        e.heading = eh;
+        e.cost = ec;
+        const unsigned long long key = ((unsigned long long)(0xFFFFFFFFu - C.epoch) << 32) | (unsigned)src;
+        atomicMin(&P.claim[epos], key);
+    }
+}
+
+__global__ void k_commit(const FrPlanDev *plans, FrCtl *ctl, const FrExp *q, const int *counters)
+{
+    const int n = counters[FR_CNT_EXP];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const FrExp e = q[i];
+        if (e.end < 0)
+            continue;
+        const FrPlanDev &P = plans[e.slot];
+        FrCtl &C = ctl[e.slot];
+        const unsigned long long key = ((unsigned long long)(0xFFFFFFFFu - C.epoch) << 32) | (unsigned)e.src;
+        if (P.claim[e.end] != key)
+            continue;
+        const int st = P.type[e.end];
+        const int sz = e.src / P.W, sx = e.src - sz * P.W;
+        P.px[e.end] = sx;
+        P.pz[e.end] = sz;
+        P.heading[e.end] = e.heading;
+        P.cost[e.end] = e.cost;
+        if (st == FR_TYPE_NULL) {
+            P.type[e.end] = FR_TYPE_TEMP;
+            P.derive[e.src] += 1;
+        } else {
+            P.type[e.end] = FR_TYPE_COLLISION;
+            P.derive[e.src] -= 1;
+            C.collision = 1;
+            atomicAdd(&C.n_conv, 1);
+        }
+    }
+}
+
+/* __CUDA_count_nodes_in_density_region (graph_expansion_search_zones.cu:42-66) */
+__global__ void k_density(const FrPlanDev *plans, const FrCtl *ctl, const int2 *tiles, const int *n_tiles_p)
+{
+    const int n_tiles = *n_tiles_p;
+    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const int2 tl = tiles[t];
+        const FrPlanDev &P = plans[tl.x];
+        const FrCtl &C = ctl[tl.x];
+        if (C.status != FR_SEARCHING || !P.smart)
+            continue;
+        const int nr = P.dens_w * P.dens_h;
+        uint32_t *wr = P.region + ((C.iter + 1) & 1) * nr;
+        const int base = tl.y * FR_TILE;
+        int cur = -1, acc = 0;
+        for (int k = 0; k < 4; ++k) {
+            const int pos = base + threadIdx.x * 4 + k;
+            if (pos >= P.N)
+                break;
+            const int ty = P.type[pos];
+            if (ty == FR_TYPE_NULL || ty == FR_TYPE_PROCESSING)
+                continue;
+            const int z = pos / P.W, x = pos - z * P.W;
+            const int r = (z / P.region_side) * P.dens_w + x / P.region_side;
+            if (r != cur) {
+                if (acc)
+                    atomicAdd(&wr[cur], (uint32_t)acc);
+                cur = r;
+                acc = 0;
+            }
+            acc++;
+        }
+        if (acc)
+            atomicAdd(&wr[cur], (uint32_t)acc);
+    }
+}
+
+__device__ __forceinline__ bool dev_solve_runs(const FrCtl &C)
+{
+    return C.collision && (C.n_node_pre - C.n_conv) > 2;
+}
+
+/* __CUDA_solveGraphCollision_erase_trees (graph_collision.cu:33-83), deciding
+ * on the pre-erase grid: marks only. */
+__global__ void k_erase_mark(const FrPlanDev *plans, const FrCtl *ctl, const int2 *tiles, const int *n_tiles_p)
+{
+    const int n_tiles = *n_tiles_p;
+    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const int2 tl = tiles[t];
+        const FrPlanDev &P = plans[tl.x];
+        const FrCtl &C = ctl[tl.x];
+        if (C.status != FR_SEARCHING || !dev_solve_runs(C))
+            continue;
+        const int nodes = C.n_node_pre - C.n_conv;
+        const int base = tl.y * FR_TILE;
+        for (int k = 0; k < 4; ++k) {
+            const int pos = base + k * FR_BLOCK + threadIdx.x;
+            if (pos >= P.N)
+                break;
+            uint8_t m = 0;
+            if (P.type[pos] != FR_TYPE_NULL) {
+                long curr = pos;
+                int i = nodes + 4;
+                while (i-- > 0) {
+                    const int ppx = P.px[curr], ppz = P.pz[curr];
+                    if (ppx == -1 && ppz == -1)
+                        break;
+                    const long next = (long)ppz * P.W + ppx;
+                    const int tn = (next >= 0 && next < P.N) ? P.type[next] : FR_TYPE_NULL;
+                    if (tn == FR_TYPE_COLLISION || tn == FR_TYPE_NULL) {
+                        m = 1;
+                        break;
+                    }
+                    curr = next;
+                }
+            }
+            P.mark[pos] = m;
+        }
+    }
+}
+
+/* Apply the erase marks and __CUDA_solveGraphCollision_set_nodes
+ * (graph_collision.cu:85-104, first half: COLLISION -> NODE, count 0).  The
+ * converted nodes are queued: they need their parents' counts incremented
+ * and a goal test. */
+__global__ void k_erase_apply(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, const int *n_tiles_p,
+                              FrFresh *fq, int *counters)
+{
+    const int n_tiles = *n_tiles_p;
+    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const int2 tl = tiles[t];
+        const FrPlanDev &P = plans[tl.x];
+        FrCtl &C = ctl[tl.x];
+        if (C.status != FR_SEARCHING || !dev_solve_runs(C))
+            continue;
+        const int base = tl.y * FR_TILE;
+        int mine[4];
+        int cnt = 0;
+        for (int k = 0; k < 4; ++k) {
+            const int pos = base + k * FR_BLOCK + threadIdx.x;
+            mine[k] = -1;
+            if (pos >= P.N)
+                continue;
+            if (P.mark[pos]) {
+                P.type[pos] = FR_TYPE_NULL;
+            } else if (P.type[pos] == FR_TYPE_COLLISION) {
+                P.type[pos] = FR_TYPE_NODE;
+                P.derive[pos] = 0;
+                mine[k] = pos;
+                cnt++;
+            }
+        }
+        int prefix;
+        const int total = block_exclusive_scan(cnt, &prefix);
+        __shared__ int s_base;
+        if (threadIdx.x == 0)
+            s_base = total ? atomicAdd(&counters[FR_CNT_FRESH], total) : 0;
+        __syncthreads();
+        int w = s_base + prefix;
+        for (int k = 0; k < 4; ++k)
+            if (mine[k] >= 0) {
+                FrFresh f;
+                f.slot = tl.x;
+                f.pos = mine[k];
+                f.lcost = FR_BIG_COST;
+                f.widx = -1;
+                f.c_pad = 1; /* converted from COLLISION */
+                f.c_cost = 0;
+                f.total = 0;
+                fq[w++] = f;
+            }
+        __syncthreads();
+    }
+}
+
+/* second half of set_nodes: the parents of converted nodes gain a child */
+__global__ void k_coll_parents(const FrPlanDev *plans, const FrFresh *fq, const int *counters)
+{
+    const int n = counters[FR_CNT_FRESH];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const FrFresh f = fq[i];
+        if (f.c_pad != 1)
+            continue;
+        const FrPlanDev &P = plans[f.slot];
+        const long pp = (long)P.pz[f.pos] * P.W + P.px[f.pos];
+        if (pp >= 0 && pp < P.N)
+            atomicAdd(&P.derive[pp], 1);
+    }
+}
+
+/* count(TEMP), countAll() and __CUDA_accept_derived_nodes
+ * (fastrrt.cpp:130-142, graph_expansion.cu:36-65).  Accepted nodes, the
+ * re-added start and (after external edits) every node are queued for the
+ * goal test and the goal connection search. */
+__global__ void k_accept(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, const int *n_tiles_p,
+                         FrFresh *fq, int *counters)
+{
+    const int n_tiles = *n_tiles_p;
+    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const int2 tl = tiles[t];
+        const FrPlanDev &P = plans[tl.x];
+        FrCtl &C = ctl[tl.x];
+        if (C.status != FR_SEARCHING)
+            continue;
+        const int base = tl.y * FR_TILE;
+        const int start_pos = (P.sx >= 0 && P.sx < P.W && P.sz >= 0 && P.sz < P.H) ? P.sz * P.W + P.sx : -1;
+        int mine[4];
+        int cnt = 0, temps = 0, alls = 0;
+        for (int k = 0; k < 4; ++k) {
+            const int pos = base + k * FR_BLOCK + threadIdx.x;
+            mine[k] = -1;
+            if (pos >= P.N)
+                continue;
+            const int ty = P.type[pos];
+            if (ty != FR_TYPE_NULL)
+                alls++;
+            if (ty == FR_TYPE_TEMP) {
+                temps++;
+                P.type[pos] = FR_TYPE_NODE;
+                mine[k] = pos;
+                cnt++;
+            } else if (ty == FR_TYPE_NODE && (C.full_check || (C.pending_start && pos == start_pos))) {
+                mine[k] = pos;
+                cnt++;
+            }
+        }
+        int prefix, tp, ap;
+        const int total = block_exclusive_scan(cnt, &prefix);
+        const int ttot = block_exclusive_scan(temps, &tp);
+        const int atot = block_exclusive_scan(alls, &ap);
+        __shared__ int s_base;
+        if (threadIdx.x == 0) {
+            s_base = total ? atomicAdd(&counters[FR_CNT_FRESH], total) : 0;
+            if (ttot)
+                atomicAdd(&C.n_temp, ttot);
+            if (atot)
+                atomicAdd(&C.n_all, atot);
+        }
+        __syncthreads();
+        int w = s_base + prefix;
+        for (int k = 0; k < 4; ++k)
+            if (mine[k] >= 0) {
+                FrFresh f;
+                f.slot = tl.x;
+                f.pos = mine[k];
+                f.lcost = FR_BIG_COST;
+                f.widx = -1;
+                f.c_pad = 0;
+                f.c_cost = 0;
+                f.total = 0;
+                fq[w++] = f;
+            }
+        __syncthreads();
+    }
+}
+
+/* __CUDA__check_goal_reached_with_direct_connection_cost
+ * (graph_check_goal_reached.cu:17-70) for the queued nodes, plus the goal
+ * test of __CUDA_KERNEL_checkGoalReached (graph_best_node.cu:182-222). */
+__global__ void k_connect(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, const int *counters)
+{
+    const int n = counters[FR_CNT_FRESH];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        FrFresh &f = fq[i];
+        const FrPlanDev &P = plans[f.slot];
+        FrCtl &C = ctl[f.slot];
+        const int pos = f.pos;
+        if (P.type[pos] != FR_TYPE_NODE)
+            continue;
+        const int z = pos / P.W, x = pos - z * P.W;
+        const float heading = P.heading[pos];
+        atomicAdd(&C.n_fresh, 1);
+        if (dev_goal_hit(P, x, z, heading))
+            C.goal_reached = 1;
+        const float r = P.dist_tol;
+        long long best = FR_BIG_COST;
+        int bq = -1;
+        float bc = 0, btot = 0;
+        for (int zp = fr_trunc_f((float)z - r); (float)zp < (float)z + r; zp++) {
+            if (zp < 0 || zp >= P.H)
+                continue;
+            for (int xp = fr_trunc_f((float)x - r); (float)xp < (float)x + r; xp++) {
+                if (xp < 0 || xp >= P.W)
+                    continue;
+                const int qq = zp * P.W + xp;
+                if (!P.gc_flag[qq])
+                    continue;
+                const float lih = P.gc_heading[qq];
+                const float c = dev_check_direct(P, P.max_curv_loop, x, z, heading, xp, zp, lih, true, false);
+                if (c < 0)
+                    continue;
+                const float total = c + P.gc_cost[qq];
+                const long long lc = (long long)floor((double)(100.0f * total));
+                if (lc <= best) {
+                    best = lc;
+                    bq = qq;
+                    bc = c;
+                    btot = total;
+                }
+            }
+        }
+        if (bq < 0)
+            continue;
+        f.lcost = best;
+        f.widx = bq;
+        f.c_cost = bc;
+        f.total = btot;
+        atomicMin((unsigned long long *)&C.best_lcost, fr_lcost_key(best));
+    }
+}
+
+__global__ void k_connect_pick(const FrPlanDev *plans, FrCtl *ctl, const FrFresh *fq, const int *counters)
+{
+    const int n = counters[FR_CNT_FRESH];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const FrFresh f = fq[i];
+        if (f.widx < 0)
+            continue;
+        FrCtl &C = ctl[f.slot];
+        if (fr_lcost_key(f.lcost) != (unsigned long long)C.best_lcost)
+            continue;
+        atomicMin(&C.best_node_key, ((unsigned long long)(unsigned)f.pos << 32) | (unsigned)i);
+    }
+}
+
+/* End of FastRRT::loop (fastrrt.cpp:130-180): reseed, goal-connection adds,
+ * goal test, iteration bookkeeping. */
+__global__ void k_finish(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n_slots, const FrFresh *fq)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_slots)
+        return;
+    const int s = slots[i];
+    FrCtl &C = ctl[s];
+    const FrPlanDev &P = plans[s];
+    if (C.status != FR_SEARCHING)
+        return;
+    C.last_temp = C.n_temp;
+    C.iter += 1;
+    if (C.n_temp == 0 && C.n_all == 0) {
+        dev_add(P, P.sx, P.sz, P.sh, -1, -1, 0.0f);
+        C.reseeded = 1;
+        C.pending_start = 1;
+        C.full_check = 0;
+    } else {
+        C.pending_start = 0;
+        C.full_check = 0;
+        if (C.best_node_key != ~0ull) {
+            const FrFresh f = fq[(unsigned)(C.best_node_key & 0xFFFFFFFFu)];
+            const int nz = f.pos / P.W, nx = f.pos - nz * P.W;
+            const int cz = f.widx / P.W, cx = f.widx - cz * P.W;
+            const float base = P.cost[f.pos];
+            const float ch = P.gc_heading[f.widx];
+            dev_add(P, cx, cz, ch, nx, nz, f.c_cost + base);
+            dev_add(P, P.gx, P.gz, P.gh, cx, cz, ch + base);
+            C.found = 1;
+            if (dev_goal_hit(P, cx, cz, ch))
+                C.goal_reached = 1;
+            if (P.gx >= 0 && P.gx < P.W && P.gz >= 0 && P.gz < P.H && dev_goal_hit(P, P.gx, P.gz, P.gh))
+                C.goal_reached = 1;
+            if (!C.goal_reached)
+                C.full_check = 1;
+        }
+    }
+    if (C.goal_reached)
+        C.status = FR_FOUND;
+    else if (P.max_iter > 0 && C.iter >= P.max_iter)
+        C.status = FR_FAILED;
+}
+
+/* ------------------------------------------------------------------------ */
+/* whole-grid helpers (graph API, path extraction, optimisation)           */
+/* ------------------------------------------------------------------------ */
+
+/* graph.cu:400-414: type <- NULL everywhere */
+__global__ void k_clear(const FrPlanDev *plans, int slot)
+{
+    const FrPlanDev &P = plans[slot];
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x)
+        P.type[pos] = FR_TYPE_NULL;
+}
+
+/* graph_list.cu:8-66: out[0..5] = count by type */
+__global__ void k_count_types(const FrPlanDev *plans, int slot, int *out)
+{
+    const FrPlanDev &P = plans[slot];
+    __shared__ int s_c[8];
+    if (threadIdx.x < 8)
+        s_c[threadIdx.x] = 0;
+    __syncthreads();
+    int c[6] = {0, 0, 0, 0, 0, 0};
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+        const int t = P.type[pos];
+        if (t < 6)
+            c[t]++;
+    }
+    for (int k = 0; k < 6; ++k)
+        if (c[k])
+            atomicAdd(&s_c[k], c[k]);
+    __syncthreads();
+    if (threadIdx.x < 6 && s_c[threadIdx.x])
+        atomicAdd(&out[threadIdx.x], s_c[threadIdx.x]);
+}
+
+/* graph_best_node.cu:182-222 over every NODE */
+__global__ void k_goal_scan(const FrPlanDev *plans, FrCtl *ctl, int slot, int gx, int gz, float gh, float tol,
+                            float herr)
+{
+    const FrPlanDev &P = plans[slot];
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+        if (P.type[pos] != FR_TYPE_NODE)
+            continue;
+        const int z = pos / P.W, x = pos - z * P.W;
+        const double dx = (double)(gx - x), dz = (double)(gz - z);
+        if (__builtin_sqrt(dx * dx + dz * dz) > (double)tol)
+            continue;
+        if (fabsf(P.heading[pos] - gh) <= herr)
+            ctl[slot].goal_reached = 1;
+    }
+}
+
+__global__ void k_best_node_init(FrCtl *ctl, const int *slots, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    FrCtl &C = ctl[slots[i]];
+    C.bn_cost = (long long)~0ull;
+    C.bn_pos = 0x7FFFFFFF;
+}
+
+/* __CUDA_KERNEL_findBestNodeWithHeading_bestCost (graph_best_node.cu:21-75).
+ * Ties: lowest cell index (second kernel). */
+__global__ void k_best_node_cost(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int tx_override,
+                                 int tz_override, float h_override, float r_override, float herr_override,
+                                 int use_override)
+{
+    const int s = slots[blockIdx.y];
+    const FrPlanDev &P = plans[s];
+    FrCtl &C = ctl[s];
+    const int tx = use_override ? tx_override : P.gx;
+    const int tz = use_override ? tz_override : P.gz;
+    const float th = use_override ? h_override : P.gh;
+    const float radius = use_override ? r_override : P.dist_tol;
+    const float herr = use_override ? herr_override : P.herr_tol;
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+        if (P.type[pos] != FR_TYPE_NODE)
+            continue;
+        const int z = pos / P.W, x = pos - z * P.W;
+        const int dx = tx - x, dz = tz - z;
+        const float dist = sqrtf((float)(dx * dx + dz * dz));
+        if (dist > radius)
+            continue;
+        const float h = P.heading[pos];
+        if (fabsf(h - th) > herr)
+            continue;
+        if (!dev_feasible(P, x, z, h))
+            continue;
+        const double cd = (double)(1.0f * dist) + (3.0 * FR_TO_DEG_D) * (double)h + (double)(1.0f * P.cost[pos]);
+        const long long cost = (long long)floor((double)(float)cd);
+        atomicMin((unsigned long long *)&C.bn_cost, fr_lcost_key(cost));
+    }
+}
+
+__global__ void k_best_node_pick(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int tx_override,
+                                 int tz_override, float h_override, float r_override, float herr_override,
+                                 int use_override)
+{
+    const int s = slots[blockIdx.y];
+    const FrPlanDev &P = plans[s];
+    FrCtl &C = ctl[s];
+    const unsigned long long best = (unsigned long long)C.bn_cost;
+    if (best >= fr_lcost_key(FR_BIG_COST))
+        return;
+    const int tx = use_override ? tx_override : P.gx;
+    const int tz = use_override ? tz_override : P.gz;
+    const float th = use_override ? h_override : P.gh;
+    const float radius = use_override ? r_override : P.dist_tol;
+    const float herr = use_override ? herr_override : P.herr_tol;
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+        if (P.type[pos] != FR_TYPE_NODE)
+            continue;
+        const int z = pos / P.W, x = pos - z * P.W;
+        const int dx = tx - x, dz = tz - z;
+        const float dist = sqrtf((float)(dx * dx + dz * dz));
+        if (dist > radius)
+            continue;
+        const float h = P.heading[pos];
+        if (fabsf(h - th) > herr)
+            continue;
+        const double cd = (double)(1.0f * dist) + (3.0 * FR_TO_DEG_D) * (double)h + (double)(1.0f * P.cost[pos]);
+        const long long cost = (long long)floor((double)(float)cd);
+        if (fr_lcost_key(cost) != best)
+            continue;
+        if (!dev_feasible(P, x, z, h))
+            continue;
+        atomicMin(&C.bn_pos, pos);
+    }
+}
+
+/* fastrrt.cpp:214-231: walk the parents of the best node, reversed into
+ * path[] as (x, z, heading, cost).  getParent (graph.cu:446-453) yields
+ * (-1, -1) for an out-of-bounds or NULL cell, so such a cell is still pushed
+ * and ends the walk.  One thread per plan; a path longer than the buffer is
+ * reported as empty, as the reference does after 10^6 steps. */
+__global__ void k_walk(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n_slots)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_slots)
+        return;
+    const int s = slots[i];
+    const FrPlanDev &P = plans[s];
+    FrCtl &C = ctl[s];
+    C.path_len = 0;
+    if ((unsigned long long)C.bn_cost >= fr_lcost_key(FR_BIG_COST) || C.bn_pos == 0x7FFFFFFF)
+        return;
+    int x = C.bn_pos % P.W, z = C.bn_pos / P.W;
+    int n = 0;
+    while (x != -1 && z != -1) {
+        if (n >= P.path_cap) {
+            C.path_len = 0;
+            return;
+        }
+        const bool inb = x >= 0 && x < P.W && z >= 0 && z < P.H;
+        const int pos = inb ? z * P.W + x : 0;
+        P.path[n++] = make_float4((float)x, (float)z, inb ? P.heading[pos] : 0.0f, inb ? P.cost[pos] : -1.0f);
+        if (!inb || P.type[pos] == FR_TYPE_NULL)
+            break;
+        x = P.px[pos];
+        z = P.pz[pos];
+    }
+    for (int a = 0, b = n - 1; a < b; ++a, --b) {
+        float4 t = P.path[a];
+        P.path[a] = P.path[b];
+        P.path[b] = t;
+    }
+    C.path_len = n;
+}
+
+/* graph_optimization.cu:25-56 for all (i, j >= i + 2): each i keeps the last
+ * improving j (atomicMax over j). pd[i].x holds that j (-1: none). */
+__global__ void k_opt_pairs(const FrPlanDev *plans, FrCtl *ctl, const int *slots)
+{
+    const int s = slots[blockIdx.y];
+    const FrPlanDev &P = plans[s];
+    const FrCtl &C = ctl[s];
+    const int n = C.path_len;
+    if (n < 3)
+        return;
+    const long npairs = (long)n * n;
+    const float max_curv = P.max_curv_loop;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < npairs; k += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(k / n), j = (int)(k - (long)i * n);
+        if (i >= n - 2 || j < i + 2)
+            continue;
+        const float4 a = P.path[i], b = P.path[j];
+        const float cur = b.w;
+        const float nc = dev_check_direct(P, max_curv, fr_trunc_f(a.x), fr_trunc_f(a.y), a.z, fr_trunc_f(b.x),
+                                          fr_trunc_f(b.y), b.z, true, true);
+        if (nc > 0 && nc < cur)
+            atomicMax((int *)&P.pd[i].x, j);
+    }
+}
+
+__global__ void k_opt_reset(const FrPlanDev *plans, FrCtl *ctl, const int *slots)
+{
+    const int s = slots[blockIdx.y];
+    const FrPlanDev &P = plans[s];
+    const int n = ctl[s].path_len;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int *pi = (int *)&P.pd[i].x;
+        *pi = -1;
+    }
+}
+
+/* graph_optimization.cu:126-183: the first i with the largest gain wins; the
+ * path is spliced, the grid cleared and the chain re-added.  One block per
+ * plan; lane 0 does the sequential splice/rebuild, the block clears. */
+__global__ void k_opt_apply(const FrPlanDev *plans, FrCtl *ctl, const int *slots)
+{
+    const int s = slots[blockIdx.x];
+    const FrPlanDev &P = plans[s];
+    FrCtl &C = ctl[s];
+    __shared__ int s_go;
+    __shared__ int s_n;
+    if (threadIdx.x == 0) {
+        int n = C.path_len;
+        C.opt_changed = 0;
+        int max_pos = -1;
+        float max_gain = -1, max_nc = 0;
+        for (int i = 0; i < n - 2; ++i) {
+            const int j = *(const int *)&P.pd[i].x;
+            if (j < 0)
+                continue;
+            const float4 a = P.path[i], b = P.path[j];
+            const float nc = dev_check_direct(P, P.max_curv_loop, fr_trunc_f(a.x), fr_trunc_f(a.y), a.z,
+                                              fr_trunc_f(b.x), fr_trunc_f(b.y), b.z, true, true);
+            const float gain = b.w - nc;
+            if (max_gain < gain) {
+                max_gain = gain;
+                max_pos = i;
+                max_nc = nc;
+            }
+        }
+        s_go = 0;
+        if (max_pos >= 0) {
+            const int next_pos = *(const int *)&P.pd[max_pos].x;
+            if (next_pos > max_pos + 1) {
+                P.path[next_pos].w = max_nc;
+                const int rm = next_pos - max_pos - 1;
+                for (int i = max_pos + 1; i + rm < n; ++i)
+                    P.path[i] = P.path[i + rm];
+                n -= rm;
+            }
+            C.path_len = n;
+            C.opt_changed = 1;
+            C.goal_reached = 0;
+            s_go = 1;
+        }
+        s_n = n;
+    }
+    __syncthreads();
+    if (!s_go)
+        return;
+    for (int pos = threadIdx.x; pos < P.N; pos += blockDim.x)
+        P.type[pos] = FR_TYPE_NULL;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int n = s_n;
+        int ppx = fr_trunc_f(P.path[0].x), ppz = fr_trunc_f(P.path[0].y);
+        dev_add(P, ppx, ppz, P.path[0].z, -1, -1, P.path[0].w);
+        for (int i = 1; i < n; ++i) {
+            const int cx = fr_trunc_f(P.path[i].x), cz = fr_trunc_f(P.path[i].y);
+            dev_add(P, cx, cz, P.path[i].z, ppx, ppz, P.path[i].w);
+            ppx = cx;
+            ppz = cz;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* launchers                                                                */
+/* ------------------------------------------------------------------------ */
+
+static int fr_grid_for(long work, int per_block, int cap)
+{
+    long b = (work + per_block - 1) / per_block;
+    if (b < 1)
+        b = 1;
+    if (b > cap)
+        b = cap;
+    return (int)b;
+}
+
+void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, int max_PH, long max_PWH,
+                     bool goal_pass, hipStream_t st)
+{
+    if (n <= 0)
+        return;
+    dim3 b(FR_BLOCK);
+    hipLaunchKernelGGL(k_setup_prep, dim3(fr_grid_for(max_N, FR_BLOCK, 1024), n), b, 0, st, L.plans, d_slots);
+    hipLaunchKernelGGL(k_setup_rowdist, dim3(fr_grid_for(max_PH, FR_BLOCK, 64), n), b, 0, st, L.plans, d_slots);
+    hipLaunchKernelGGL(k_setup_coldist, dim3(fr_grid_for(max_PWH, FR_BLOCK, 1024), n), b, 0, st, L.plans,
+                       d_slots);
+    if (goal_pass) {
+        fr_prof_begin(FR_PROF_GOAL_CONNECTION, st);
+        hipLaunchKernelGGL(k_setup_goal_connection, dim3(fr_grid_for(max_N, FR_BLOCK, 4096), n), b, 0, st, L.plans,
+                           d_slots);
+        fr_prof_end(FR_PROF_GOAL_CONNECTION, st);
+    }
+}
+
+void fr_launch_iteration(const FrLaunch &L, int n_slots, hipStream_t st)
+{
+    dim3 b(FR_BLOCK);
+    const int gs = L.grid_slots;
+    fr_prof_begin(FR_PROF_ITERATION, st);
+    hipLaunchKernelGGL(k_begin, dim3(fr_grid_for(n_slots, FR_BLOCK, 65535)), b, 0, st, L.plans, L.ctl, L.slots,
+                       n_slots, L.counters);
+    hipLaunchKernelGGL(k_select, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles, L.exp_q,
+                       L.counters);
+    fr_prof_begin(FR_PROF_EXPAND, st);
+    hipLaunchKernelGGL(k_expand, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.exp_q, L.counters);
+    fr_prof_end(FR_PROF_EXPAND, st);
+    hipLaunchKernelGGL(k_commit, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.exp_q, L.counters);
+    hipLaunchKernelGGL(k_density, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles);
+    hipLaunchKernelGGL(k_erase_mark, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles);
+    hipLaunchKernelGGL(k_erase_apply, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles,
+                       L.fresh_q, L.counters);
+    hipLaunchKernelGGL(k_coll_parents, dim3(L.grid_items), b, 0, st, L.plans, L.fresh_q, L.counters);
+    hipLaunchKernelGGL(k_accept, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles, L.fresh_q,
+                       L.counters);
+    fr_prof_begin(FR_PROF_CONNECT, st);
+    hipLaunchKernelGGL(k_connect, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);
+    fr_prof_end(FR_PROF_CONNECT, st);
+    hipLaunchKernelGGL(k_connect_pick, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);
+    hipLaunchKernelGGL(k_finish, dim3(fr_grid_for(n_slots, FR_BLOCK, 65535)), b, 0, st, L.plans, L.ctl, L.slots,
+                       n_slots, L.fresh_q);
+    fr_prof_end(FR_PROF_ITERATION, st);
+    (void)gs;
+}
+
+void fr_launch_clear(const FrLaunch &L, int slot, int N, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_clear, dim3(fr_grid_for(N, FR_BLOCK, 2048)), dim3(FR_BLOCK), 0, st, L.plans, slot);
+}
+
+void fr_launch_count(const FrLaunch &L, int slot, int N, int *d_out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_count_types, dim3(fr_grid_for(N, FR_BLOCK, 1024)), dim3(FR_BLOCK), 0, st, L.plans,
+                       slot, d_out);
+}
+
+void fr_launch_goal_scan(const FrLaunch &L, int slot, int N, int gx, int gz, float gh, float tol, float herr,
+                         hipStream_t st)
+{
+    hipLaunchKernelGGL(k_goal_scan, dim3(fr_grid_for(N, FR_BLOCK, 2048)), dim3(FR_BLOCK), 0, st, L.plans, L.ctl,
+                       slot, gx, gz, gh, tol, herr);
+}
+
+void fr_launch_best_node(const FrLaunch &L, const int *d_slots, int n, int max_N, const FrBestNodeArgs *ov,
+                         hipStream_t st)
+{
+    const int use = ov != nullptr;
+    const int tx = use ? ov->x : 0, tz = use ? ov->z : 0;
+    const float h = use ? ov->heading : 0, r = use ? ov->radius : 0, e = use ? ov->max_herr : 0;
+    dim3 g(fr_grid_for(max_N, FR_BLOCK, 1024), n);
+    hipLaunchKernelGGL(k_best_node_init, dim3(fr_grid_for(n, 64, 65535)), dim3(64), 0, st, L.ctl, d_slots, n);
+    hipLaunchKernelGGL(k_best_node_cost, g, dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots, tx, tz, h, r, e, use);
+    hipLaunchKernelGGL(k_best_node_pick, g, dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots, tx, tz, h, r, e, use);
+}
+
+void fr_launch_walk(const FrLaunch &L, const int *d_slots, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_walk, dim3(fr_grid_for(n, 64, 65535)), dim3(64), 0, st, L.plans, L.ctl, d_slots, n);
+}
+
+void fr_launch_opt_round(const FrLaunch &L, const int *d_slots, int n, int max_path, hipStream_t st)
+{
+    dim3 g1(fr_grid_for(max_path, FR_BLOCK, 64), n);
+    hipLaunchKernelGGL(k_opt_reset, g1, dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots);
+    dim3 g2(fr_grid_for((long)max_path * max_path, FR_BLOCK, 4096), n);
+    hipLaunchKernelGGL(k_opt_pairs, g2, dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots);
+    hipLaunchKernelGGL(k_opt_apply, dim3(n), dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots);
+}
+
+/* ------------------------------------------------------------------------ */
+/* single-plan graph operations (the CudaGraph API, python_cuda_graph_bind) */
+/* ------------------------------------------------------------------------ */
+
+/* graph.cu:364-381 (add / addTemporary); count_parent=0 for addTemporary */
+__global__ void k_op_add(const FrPlanDev *plans, int slot, int x, int z, float h, int ppx, int ppz, float cost,
+                         int type, int count_parent)
+{
+    const FrPlanDev &P = plans[slot];
+    if (x < 0 || x >= P.W || z < 0 || z >= P.H)
+        return;
+    const int pos = z * P.W + x;
+    if (count_parent && ppx != -1 && ppz != -1) {
+        const long pp = (long)ppz * P.W + ppx;
+        if (pp >= 0 && pp < P.N)
+            P.derive[pp]++;
+    }
+    P.type[pos] = (uint8_t)type;
+    P.px[pos] = ppx;
+    P.pz[pos] = ppz;
+    P.heading[pos] = h;
+    P.cost[pos] = cost;
+}
+
+/* graph.cu:553-563 */
+__global__ void k_op_set_collision(const FrPlanDev *plans, int slot, int x, int z, int npx, int npz, float h,
+                                   float cost)
+{
+    const FrPlanDev &P = plans[slot];
+    if (x < 0 || x >= P.W || z < 0 || z >= P.H)
+        return;
+    const int pos = z * P.W + x;
+    const long cpp = (long)P.pz[pos] * P.W + P.px[pos];
+    if (P.type[pos] != FR_TYPE_NODE)
+        return;
+    P.type[pos] = FR_TYPE_COLLISION;
+    P.px[pos] = npx;
+    P.pz[pos] = npz;
+    P.heading[pos] = h;
+    P.cost[pos] = cost;
+    if (cpp >= 0 && cpp < P.N)
+        P.derive[cpp]--;
+}
+
+/* CudaGraph::derivateNode -> expand_node, host sequential semantics
+ * (graph_expansion.cu:103-145,224-244).  out = end cell or (-1, -1). */
+__global__ void k_op_derivate(const FrPlanDev *plans, int slot, float steer, float path_size, float v, int x, int z,
+                              int *out)
+{
+    FrPlanDev P = plans[slot];
+    out[0] = -1;
+    out[1] = -1;
+    if (x < 0 || x >= P.W || z < 0 || z >= P.H)
+        return;
+    const int pos = z * P.W + x;
+    if (P.type[pos] != FR_TYPE_NODE)
+        return;
+    P.v = v;
+    int ex, ez;
+    float eh, ec;
+    if (!dev_rollout(P, x, z, steer, path_size, &ex, &ez, &eh, &ec))
+        return;
+    if ((float)ex == P.og_x && (float)ez == P.og_y)
+        return;
+    const int epos = ez * P.W + ex;
+    if (epos == pos)
+        return;
+    if (P.type[epos] == FR_TYPE_NULL) {
+        P.type[epos] = FR_TYPE_TEMP;
+        P.derive[pos]++;
+        P.px[epos] = x;
+        P.pz[epos] = z;
+        P.heading[epos] = eh;
+        P.cost[epos] = ec;
+    }
+    if (P.type[epos] == FR_TYPE_NODE) {
+        P.type[epos] = FR_TYPE_COLLISION;
+        P.px[epos] = x;
+        P.pz[epos] = z;
+        P.heading[epos] = eh;
+        P.cost[epos] = ec;
+        P.derive[pos]--;
+        out[2] = 1; /* node collision flag */
+    }
+    out[0] = ex;
+    out[1] = ez;
+}
+
+__global__ void k_op_accept(const FrPlanDev *plans, int slot)
+{
+    const FrPlanDev &P = plans[slot];
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x)
+        if (P.type[pos] == FR_TYPE_TEMP)
+            P.type[pos] = FR_TYPE_NODE;
+}
+
+__global__ void k_op_solve_mark(const FrPlanDev *plans, int slot, int nodes)
+{
+    const FrPlanDev &P = plans[slot];
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+        uint8_t m = 0;
+        if (P.type[pos] != FR_TYPE_NULL) {
+            long curr = pos;
+            int i = nodes + 4;
+            while (i-- > 0) {
+                const int ppx = P.px[curr], ppz = P.pz[curr];
+                if (ppx == -1 && ppz == -1)
+                    break;
+                const long next = (long)ppz * P.W + ppx;
+                const int tn = (next >= 0 && next < P.N) ? P.type[next] : FR_TYPE_NULL;
+                if (tn == FR_TYPE_COLLISION || tn == FR_TYPE_NULL) {
+                    m = 1;
+                    break;
+                }
+                curr = next;
+            }
+        }
+        P.mark[pos] = m;
+    }
+}
+
+__global__ void k_op_solve_apply(const FrPlanDev *plans, int slot)
+{
+    const FrPlanDev &P = plans[slot];
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+        if (P.mark[pos]) {
+            P.type[pos] = FR_TYPE_NULL;
+            P.mark[pos] = 0;
+        } else if (P.type[pos] == FR_TYPE_COLLISION) {
+            P.type[pos] = FR_TYPE_NODE;
+            P.derive[pos] = 0;
+            P.mark[pos] = 2;
+        }
+    }
+}
+
+__global__ void k_op_solve_parents(const FrPlanDev *plans, int slot)
+{
+    const FrPlanDev &P = plans[slot];
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+        if (P.mark[pos] != 2)
+            continue;
+        P.mark[pos] = 0;
+        const long pp = (long)P.pz[pos] * P.W + P.px[pos];
+        if (pp >= 0 && pp < P.N)
+            atomicAdd(&P.derive[pp], 1);
+    }
+}
+
+__global__ void k_op_region_count(const FrPlanDev *plans, int slot, uint32_t *out)
+{
+    const FrPlanDev &P = plans[slot];
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+        const int ty = P.type[pos];
+        if (ty == FR_TYPE_NULL || ty == FR_TYPE_PROCESSING)
+            continue;
+        const int z = pos / P.W, x = pos - z * P.W;
+        atomicAdd(&out[(z / P.region_side) * P.dens_w + x / P.region_side], 1u);
+    }
+}
+
+void fr_launch_graph_add(const FrLaunch &L, int slot, int x, int z, float h, int px, int pz, float cost, int type,
+                         int count_parent, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_op_add, dim3(1), dim3(1), 0, st, L.plans, slot, x, z, h, px, pz, cost, type, count_parent);
+}
+
+void fr_launch_set_collision(const FrLaunch &L, int slot, int x, int z, int npx, int npz, float h, float cost,
+                             hipStream_t st)
+{
+    hipLaunchKernelGGL(k_op_set_collision, dim3(1), dim3(1), 0, st, L.plans, slot, x, z, npx, npz, h, cost);
+}
+
+void fr_launch_derivate(const FrLaunch &L, int slot, float steer, float path_size, float v, int x, int z, int *d_out,
+                        hipStream_t st)
+{
+    hipLaunchKernelGGL(k_op_derivate, dim3(1), dim3(1), 0, st, L.plans, slot, steer, path_size, v, x, z, d_out);
+}
+
+void fr_launch_accept(const FrLaunch &L, int slot, int N, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_op_accept, dim3(fr_grid_for(N, FR_BLOCK, 2048)), dim3(FR_BLOCK), 0, st, L.plans, slot);
+}
+
+void fr_launch_solve_collisions(const FrLaunch &L, int slot, int N, int nodes, hipStream_t st)
+{
+    dim3 g(fr_grid_for(N, FR_BLOCK, 2048));
+    hipLaunchKernelGGL(k_op_solve_mark, g, dim3(FR_BLOCK), 0, st, L.plans, slot, nodes);
+    hipLaunchKernelGGL(k_op_solve_apply, g, dim3(FR_BLOCK), 0, st, L.plans, slot);
+    hipLaunchKernelGGL(k_op_solve_parents, g, dim3(FR_BLOCK), 0, st, L.plans, slot);
+}
+
+void fr_launch_region_count(const FrLaunch &L, int slot, int N, uint32_t *d_out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_op_region_count, dim3(fr_grid_for(N, FR_BLOCK, 2048)), dim3(FR_BLOCK), 0, st, L.plans,
+                       slot, d_out);
+}

@@ -1,0 +1,185 @@
+/*
+ * fastrrt.h — C ABI of libfastrrt.so (MI355X / gfx950 build).
+ *
+ * Drop-in for the reference's ctypes boundary: the FastRRT handle ABI of
+ * src/python_bind.cpp and the graph ABI of src/python_cuda_graph_bind.cpp
+ * keep their symbol names, argument order and meaning.  The one change is
+ * the frame: the reference takes a libdriveless SearchFrame*; here the frame
+ * is an opaque handle made by fastrrt_frame_create() from host float3 pixels
+ * (channel 0 = class id, 1 = distance to goal, 2 = traversability bits) and
+ * a class-cost table.  Like the reference, a plan borrows its frame: keep the
+ * frame alive until the search is done.
+ *
+ * Errors: the reference throws C++ exceptions through extern "C"
+ * (src/fastrrt.cpp:89, src/cuda/graph_list.cu:100).  Here every entry point
+ * catches, keeps the message for fastrrt_last_error(), and returns false /
+ * NULL / a no-op.  Thread safety: one handle per host thread, as in the
+ * reference; each handle owns its own HIP stream.
+ */
+#ifndef FASTRRT_C_ABI_H
+#define FASTRRT_C_ABI_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Frames (replace libdriveless SearchFrame*)                              */
+/* ---------------------------------------------------------------------- */
+
+/* New: the reference passes a SearchFrame* (python_bind.cpp:38, :263). */
+void *fastrrt_frame_create(int width, int height, const float *xyz3, const float *class_costs, int n_classes);
+void fastrrt_frame_destroy(void *frame);
+/* Re-upload the pixels of an existing frame (same size); returns false on error. */
+bool fastrrt_frame_update(void *frame, const float *xyz3);
+/* Message of the last failed call on this thread ("" if none). */
+const char *fastrrt_last_error(void);
+
+/* ---------------------------------------------------------------------- */
+/* FastRRT handle (src/python_bind.cpp)                                    */
+/* ---------------------------------------------------------------------- */
+
+/* python_bind.cpp:7-30.  class_costs is accepted and unused, as there. */
+void *fastrrt_initialize(int width, int height, float perception_width_m, float perception_height_m,
+                         float max_steering_angle_deg, float vehicle_length, int lower_bound_x, int lower_bound_z,
+                         int upper_bound_x, int upper_bound_z, float *class_costs, float max_curvature);
+/* python_bind.cpp:32-36 */
+void fastrrt_destroy(void *ptr);
+/* python_bind.cpp:38-64; frame = fastrrt_frame_create() handle */
+void set_plan_data(void *ptr, void *frame, int start_x, int start_z, float start_heading_rad, int goal_x,
+                   int goal_z, float goal_heading_rad, float velocity_m_s, int min_dist_x, int min_dist_z,
+                   int timeout_ms, float max_path_size, float dist_to_goal_tolerance,
+                   float heading_error_tolerance_rad);
+/* python_bind.cpp:66-70 */
+bool goal_reached(void *ptr);
+/* python_bind.cpp:72-76 */
+void search_init(void *ptr, bool copy_intrinsic_costs_from_frame);
+/* python_bind.cpp:77-81 */
+bool loop(void *ptr, bool smart_expansion);
+/* python_bind.cpp:83-87 */
+bool path_optimize(void *ptr);
+/* python_bind.cpp:89-107: [n, (x, z, type) x n] */
+int *export_graph_nodes(void *ptr);
+/* python_bind.cpp:109-112 (declared float* there; frees the int[] above) */
+void release_export_graph_nodes(float *ptr);
+/* python_bind.cpp:138-143: [n, (x, z, heading) x n] */
+float *get_planned_path(void *ptr);
+/* python_bind.cpp:145-150 */
+float *interpolate_planned_path(void *ptr);
+/* python_bind.cpp:152-155 */
+void release_planned_path_data(float *ptr);
+/* python_bind.cpp:157-171 */
+float *interpolate_planned_path_p(void *ptr, float *p, int size);
+/* python_bind.cpp:173-180 */
+float *ideal_curve(void *ptr, int goal_x, int goal_z, float goal_heading_rad);
+/* python_bind.cpp:182-186 */
+void compute_region_debug_performance(void *ptr);
+/* python_bind.cpp:188-192 */
+void save_current_graph_state(void *ptr, const char *filename);
+/* python_bind.cpp:194-198 */
+void load_graph_state(void *ptr, const char *filename);
+
+/* New: Philox key and stream of the next search_init (the reference seeds
+ * cuRAND from the wall clock, random_number_gen.cu:23-40). */
+void fastrrt_set_seed(void *ptr, uint64_t seed, uint64_t stream);
+/* New: loop() iterations since search_init. */
+int fastrrt_iterations(void *ptr);
+/* New: work of the last loop(): rollouts, nodes added, nodes goal-checked. */
+void fastrrt_iteration_stats(void *ptr, int *n_expansions, int *n_new, int *n_checked);
+/* New: binary counterpart of save_current_graph_state (graph.cu:496-520):
+ * copies W*H entries of each grid plane to the caller's buffers (any may be
+ * NULL); goal_* are the goal-connection planes of set_plan_data. */
+void fastrrt_export_state(void *ptr, int32_t *parent_x, int32_t *parent_z, int32_t *type, int32_t *child_count,
+                          float *heading, float *cost, uint8_t *goal_flag, float *goal_heading, float *goal_cost);
+
+/* New: HIP-event timing of the hot kernels, recorded on the stream each is
+ * launched on.  kernel: 0 = goal-connection precompute, 1 = expansion
+ * rollout, 2 = per-loop goal connection, 3 = one whole loop iteration. */
+void fastrrt_profile_enable(bool on);
+void fastrrt_profile_reset(void);
+bool fastrrt_profile_read(int kernel, long *launches, double *total_ms);
+
+/* ---------------------------------------------------------------------- */
+/* Graph handle (src/python_cuda_graph_bind.cpp)                           */
+/* ---------------------------------------------------------------------- */
+
+/* python_cuda_graph_bind.cpp:7-34; costs = [count, c0, c1, ...] */
+void *cudagraph_initialize(int width, int height, float perception_width_m, float perception_height_m,
+                           float max_steering_angle_deg, float vehicle_length, int min_distance_x,
+                           int min_distance_z, int lower_bound_x, int lower_bound_z, int upper_bound_x,
+                           int upper_bound_z, float *segmentation_class_cost, float max_curvature);
+/* :35-39 */
+void cudagraph_destroy(void *ptr);
+/* :55-73: W*H intrinsic costs */
+float *get_intrinsic_costs(void *ptr);
+/* :75-78 */
+void destroy_intrinsic_costs_ptr(float *ptr);
+/* :80-84 */
+void add(void *ptr, int x, int z, float heading_rad, int parent_x, int parent_z, float cost);
+/* :86-90 */
+void add_temporary(void *ptr, int x, int z, float heading_rad, int parent_x, int parent_z, float cost);
+/* :93-98 */
+void derivate_node(void *ptr, void *frame, float steering_angle, int path_size, float velocity, int x, int z);
+/* :100-104 */
+void accept_derived_nodes(void *ptr, int goal_x, int goal_z, float goal_heading);
+/* :105-109 */
+bool check_in_graph(void *ptr, int x, int z);
+/* :110-114 */
+float get_heading(void *ptr, int x, int z);
+/* :115-119 */
+void clear(void *ptr);
+/* :121-140: [n, (x, z, heading, type) x n] */
+float *list_all(void *ptr);
+/* :142-145 */
+void free_list_all(float *p);
+/* :155-159 */
+void solve_collisions(void *ptr);
+/* :161-167 */
+void process_direct_goal_connection(void *ptr, void *frame, int goal_x, int goal_z, float goal_heading_rad,
+                                    float max_curvature);
+/* :169-173 */
+bool is_directly_connected_to_goal(void *ptr, int x, int z);
+/* :175-179 */
+float direct_connection_to_goal_cost(void *ptr, int x, int z);
+/* :181-186 */
+float direct_connection_to_goal_heading(void *ptr, int x, int z);
+/* :188-191 */
+void dump_graph_to_file(void *ptr, char *filename);
+/* :193-197 */
+void read_from_dump_file(void *ptr, char *filename);
+/* :199-206 */
+int *get_parent(void *ptr, int x, int z);
+/* :208-210 */
+void free_parent_data(int *data_ptr);
+/* :212-215 */
+float get_cost(void *ptr, int x, int z);
+/* :217-220 */
+int count_all(void *ptr);
+
+/* New (the reference keeps these on the C++ class only): graph test hooks
+ * used by the reference's unit tests (unittests/src/test_graph_*.cpp). */
+void cudagraph_set_collision(void *ptr, int x, int z, int new_parent_x, int new_parent_z, float heading,
+                             float cost);
+int cudagraph_get_type(void *ptr, int x, int z);
+int cudagraph_get_child_count(void *ptr, int x, int z);
+int cudagraph_count(void *ptr, int type);
+bool cudagraph_check_consistent(void *ptr);
+/* findBestNode (graph_best_node.cu:134-178); out = (x, z) */
+void cudagraph_find_best_node(void *ptr, void *frame, float heading, float radius, int x, int z,
+                              float max_heading_error, int *out);
+bool cudagraph_check_goal_reached(void *ptr, int goal_x, int goal_z, float heading, float tolerance,
+                                  float max_heading_error);
+/* derivateNode returning the end cell (graph.h:177); out = (x, z) */
+void cudagraph_derivate_node(void *ptr, void *frame, float steering_angle, float path_size, float velocity, int x,
+                             int z, int *out);
+/* as fastrrt_export_state, for a graph handle */
+void cudagraph_export_state(void *ptr, int32_t *parent_x, int32_t *parent_z, int32_t *type, int32_t *child_count,
+                            float *heading, float *cost, uint8_t *goal_flag, float *goal_heading, float *goal_cost);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

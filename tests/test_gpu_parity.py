@@ -1,0 +1,152 @@
+"""Parity of the HIP planning loop against the CPU oracle (bit-exact).
+
+Every test drives libfastrrt.so through its C ABI (pyfastrrt layer) and the
+oracle on the same seeded scene, and compares the whole node grid (types,
+parents, child counts, FP32 headings and costs) after each loop() call, the
+goal-connection planes after set_plan_data, and the returned paths.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+from _parity import GpuPlanner, OraclePlanner, Params, diff_goal_planes, diff_states
+
+from fastrrt_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_scene(seed, size=128, coverage=0.10):
+    sc = scenes.scene_sized(seed, size, coverage, min_dist=(10, 16))
+    return sc
+
+
+SMALL = Params(min_dist=(10, 16), max_path=30.0, dist_tol=12.0)
+FULL = Params()
+
+
+def _run_side_by_side(sc, prm, seed, stream, max_iter, smart=False):
+    print(f"[parity] seed={seed} size={sc.width} cov={sc.coverage} smart={smart}", flush=True)
+    o = OraclePlanner(sc, prm, seed, stream)
+    g = GpuPlanner(sc, prm, seed, stream)
+    bad = diff_goal_planes(o.goal_planes(), g.goal_planes())
+    assert not bad, f"goal-connection planes differ: {bad}"
+    o.search_init()
+    g.search_init()
+    assert o.goal_reached() == g.goal_reached()
+    it = 0
+    while True:
+        go = (not o.goal_reached()) and o.loop(smart)
+        gg = (not g.goal_reached()) and g.loop(smart)
+        it += 1
+        bad = diff_states(o.state(), g.state())
+        assert not bad, f"iteration {it}: grid differs {bad}"
+        assert go == gg, f"iteration {it}: loop() returned {go} (oracle) vs {gg} (gpu)"
+        if not go or it >= max_iter:
+            break
+    return o, g, it
+
+
+@pytest.mark.parametrize("seed", [11, 12, 13, 14])
+def test_loop_parity_small(seed):
+    sc = _small_scene(seed)
+    o, g, it = _run_side_by_side(sc, SMALL, 1234 + seed, seed, max_iter=60)
+    assert o.goal_reached() == g.goal_reached()
+    po, pg = o.path(), g.path()
+    assert po.shape == pg.shape and (po.view(np.int32) == pg.view(np.int32)).all()
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_loop_parity_small_smart(seed):
+    sc = _small_scene(seed)
+    _run_side_by_side(sc, SMALL, 99 + seed, seed, max_iter=40, smart=True)
+
+
+@pytest.mark.parametrize("seed", [3, 7])
+def test_path_optimize_parity(seed):
+    sc = _small_scene(seed, coverage=0.05)
+    o, g, it = _run_side_by_side(sc, SMALL, 7 + seed, seed, max_iter=80)
+    if not o.goal_reached():
+        pytest.skip("scene not solved within the iteration cap")
+    for r in range(6):
+        ro, rg = o.path_optimize(), g.path_optimize()
+        assert ro == rg, f"optimize round {r}: {ro} vs {rg}"
+        bad = diff_states(o.state(), g.state())
+        assert not bad, f"optimize round {r}: grid differs {bad}"
+        if not ro:
+            break
+    po, pg = o.path(), g.path()
+    assert po.shape == pg.shape and (po.view(np.int32) == pg.view(np.int32)).all()
+    io, ig = o.interpolated(), g.path(interpolate=True)
+    assert io.shape == ig.shape and (io.view(np.int32) == ig.view(np.int32)).all()
+
+
+def test_loop_parity_256_full_footprint():
+    sc = scenes.scene_256(1, 0.10)
+    _run_side_by_side(sc, FULL, 4242, 1, max_iter=12)
+
+
+def test_known_answers_graph_api():
+    """unittests/src/test_graph_node_derivation.cpp, test_graph_best_node.cpp,
+    test_graph_solve_collisions.cpp through the graph ABI."""
+    from fastrrt_amd import pyfastrrt as P
+    frame = P.SearchFrame(O.empty_frame(256, 256), [0, 1, 2, 3, 4, -1])
+    g = P.CudaGraph(256, 256, 256, 256, 40, 5.412658773, 0, 0, -1, -1, -1, -1, np.array([0, 1, 2, 3, 4, -1]))
+    g.add(128, 128, P.angle(0.0), -1, -1, 0)
+    assert g.derivate_node_end(frame, 0.0, 49, 1.0, 128, 128) == (128, 78)
+    g.accept_derived_nodes()
+    assert g.check_in_graph(128, 78) and g.get_heading(128, 78).rad() == 0.0
+    # SingleCollisionSolve
+    g = P.CudaGraph(256, 256, 256, 256, 40, 5.412658773, 0, 0, -1, -1, -1, -1, np.array([0, 1, 2, 3, 4, -1]))
+    g.add(128, 128, P.angle(0), -1, -1, 0)
+    g.add(100, 100, P.angle(0), 128, 128, 0)
+    g.add(135, 100, P.angle(0), 128, 128, 0)
+    g.add(110, 70, P.angle(0), 135, 100, 0)
+    g.add(145, 70, P.angle(0), 135, 100, 0)
+    g.set_collision(110, 70, 100, 100, 0.0, 0.0)
+    g.solve_collisions()
+    assert (g.get_child_count(128, 128), g.get_child_count(100, 100), g.get_child_count(135, 100)) == (2, 1, 1)
+    assert g.check_graph_is_consistent()
+    assert g.get_parent(110, 70) == (100, 100) and g.get_type(110, 70) == 1
+    # TestBestNode
+    g = P.CudaGraph(256, 256, 34.641016151377535, 34.641016151377535, 40, 5.412658773, 0, 0, -1, -1, -1, -1,
+                    np.array([0, 1, 2, 3, 4, 5]))
+    g.add(128, 128, P.angle(0), -1, -1, 0)
+    for x in (128, 108, 158):
+        g.add(x, 3, P.angle(0), 128, 128, 10)
+    frame2 = P.SearchFrame(O.empty_frame(256, 256), [0, 1, 2, 3, 4, 5])
+    assert g.find_best_node(frame2, 0.0, 50.0, 128, 0, 1.0) == (128, 3)
+
+
+def test_curvy_derivation_matches_oracle():
+    """test_graph_node_derivation.cpp:193-224: 361 headings x 5 steering
+    angles, P = 15, from (100, 200); the HIP rollout and the oracle agree on
+    the end cell and FP32 heading bit for bit."""
+    from fastrrt_amd import pyfastrrt as P
+    frame = P.SearchFrame(O.empty_frame(256, 256), [0, 1, 2, 3, 4, -1])
+    g = P.CudaGraph(256, 256, 256, 256, 40, 5.412658773, 0, 0, -1, -1, -1, -1, np.array([0, 1, 2, 3, 4, -1]))
+    og = O.Graph(256, 256)
+    og.physical(256, 256, 40.0, 5.412658773)
+    og.costs([0, 1, 2, 3, 4, -1])
+    og.search(0, 0)
+    f = O.empty_frame(256, 256)
+    L = O.lib()
+    out = (O.ctypes.c_int * 2)()
+    mism = 0
+    for a in range(-180, 181):
+        h = np.float32(np.radians(a))
+        for i in range(-40, 41, 18):
+            s = np.float32(np.radians(i))
+            g.add(100, 200, P.angle(float(h)), -1, -1, 0)
+            end = g.derivate_node_end(frame, float(s), 15.0, 1.0, 100, 200)
+            og.add(100, 200, float(h), -1, -1, 0)
+            L.og_derivate_node(og.g, O.fptr(f), float(s), 15.0, 1.0, 100, 200, out)
+            if end != (out[0], out[1]):
+                mism += 1
+            elif end[0] >= 0:
+                gh = np.float32(g.get_heading(*end).rad())
+                oh = np.float32(og.heading_at(*end))
+                mism += int(gh.view(np.int32) != oh.view(np.int32))
+            g.clear()
+            L.og_clear(og.g)
+    assert mism == 0
