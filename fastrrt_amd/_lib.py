@@ -87,6 +87,37 @@ SIGNATURES = {
     "cudagraph_export_state": (None, [c_void_p, i32p, i32p, i32p, i32p, fp, fp, u8p, fp, fp]),
 }
 
+
+
+class PlanRequest(ctypes.Structure):
+    _fields_ = [
+        ("frame", c_void_p),
+        ("start_x", c_int), ("start_z", c_int), ("start_heading_rad", c_float),
+        ("goal_x", c_int), ("goal_z", c_int), ("goal_heading_rad", c_float),
+        ("velocity_m_s", c_float), ("min_dist_x", c_int), ("min_dist_z", c_int),
+        ("lower_bound_x", c_int), ("lower_bound_z", c_int), ("upper_bound_x", c_int), ("upper_bound_z", c_int),
+        ("max_steering_angle_deg", c_float), ("vehicle_length", c_float), ("max_curvature", c_float),
+        ("max_path_size", c_float), ("dist_to_goal_tolerance", c_float), ("heading_error_tolerance_rad", c_float),
+        ("max_iterations", c_int), ("optimize_rounds", c_int), ("smart", c_int), ("pad0", c_int),
+        ("deadline_ms", c_double), ("seed", c_u64), ("stream", c_u64), ("user_tag", c_i64),
+    ]
+
+
+class PlanResult(ctypes.Structure):
+    _fields_ = [("user_tag", c_i64), ("status", c_int), ("iterations", c_int), ("path_len", c_int),
+                ("pad0", c_int), ("submit_ms", c_double), ("done_ms", c_double)]
+
+
+SIGNATURES.update({
+    "fastrrt_batch_create": (c_void_p, [c_int, c_i64]),
+    "fastrrt_batch_destroy": (None, [c_void_p]),
+    "fastrrt_batch_submit": (c_int, [c_void_p, ctypes.POINTER(PlanRequest)]),
+    "fastrrt_batch_step": (c_int, [c_void_p, c_int]),
+    "fastrrt_batch_poll": (c_bool, [c_void_p, ctypes.POINTER(PlanResult), fp, c_int]),
+    "fastrrt_batch_drain": (None, [c_void_p]),
+    "fastrrt_batch_clock_ms": (c_double, [c_void_p]),
+})
+
 _lib = None
 
 

@@ -1,0 +1,114 @@
+"""Batched / served planning over the fastrrt_batch_* C ABI.
+
+A Batch holds up to `max_plans` concurrent plans (total grid cells bounded by
+`max_cells`, i.e. by HBM).  submit() queues a request; step() admits queued
+requests as slots free up, runs loop iterations of every running plan (one
+hipGraph replay each) and retires finished plans; poll() returns results.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from .pyfastrrt import SearchFrame
+
+
+class PlanSpec:
+    """One plan request (the parameters set_plan_data takes, plus batch-only
+    fields: iteration cap, optimisation rounds, deadline, RNG key)."""
+
+    def __init__(self, frame: SearchFrame, start, goal, min_dist=(22, 40), ego_lb=(-1, -1), ego_ub=(-1, -1),
+                 velocity=1.0, max_steering_deg=40.0, vehicle_length=5.412658773, max_curvature=-1.0,
+                 max_path_size=40.0, dist_to_goal_tolerance=20.0, heading_error_tolerance=math.radians(10),
+                 max_iterations=200, optimize_rounds=20, smart=False, deadline_ms=0.0, seed=0, stream=0, tag=0):
+        self.frame = frame
+        r = _lib.PlanRequest()
+        r.frame = frame.get_cuda_ptr()
+        r.start_x, r.start_z, r.start_heading_rad = int(start[0]), int(start[1]), float(start[2])
+        r.goal_x, r.goal_z, r.goal_heading_rad = int(goal[0]), int(goal[1]), float(goal[2])
+        r.velocity_m_s = velocity
+        r.min_dist_x, r.min_dist_z = int(min_dist[0]), int(min_dist[1])
+        r.lower_bound_x, r.lower_bound_z = int(ego_lb[0]), int(ego_lb[1])
+        r.upper_bound_x, r.upper_bound_z = int(ego_ub[0]), int(ego_ub[1])
+        r.max_steering_angle_deg = max_steering_deg
+        r.vehicle_length = vehicle_length
+        r.max_curvature = max_curvature
+        r.max_path_size = max_path_size
+        r.dist_to_goal_tolerance = dist_to_goal_tolerance
+        r.heading_error_tolerance_rad = float(np.float32(heading_error_tolerance))
+        r.max_iterations = int(max_iterations)
+        r.optimize_rounds = int(optimize_rounds)
+        r.smart = 1 if smart else 0
+        r.deadline_ms = float(deadline_ms)
+        r.seed = int(seed) & ((1 << 64) - 1)
+        r.stream = int(stream) & ((1 << 64) - 1)
+        r.user_tag = int(tag)
+        self.req = r
+
+
+class Result:
+    __slots__ = ("tag", "status", "iterations", "path", "submit_ms", "done_ms")
+
+    @property
+    def ok(self):
+        return self.status == 2
+
+    @property
+    def latency_ms(self):
+        return self.done_ms - self.submit_ms
+
+
+class Batch:
+    def __init__(self, max_plans=256, max_cells=256 * 256 * 256, path_cap=65536):
+        L = _lib.load()
+        self._ptr = L.fastrrt_batch_create(int(max_plans), int(max_cells))
+        if not self._ptr:
+            _lib.check("batch_create")
+        self._keep = {}
+        self._buf = np.zeros(3 * path_cap, np.float32)
+        self._cap = path_cap
+
+    def __del__(self):
+        try:
+            if getattr(self, "_ptr", None):
+                _lib.load().fastrrt_batch_destroy(self._ptr)
+                self._ptr = None
+        except Exception:
+            pass
+
+    def submit(self, spec: PlanSpec) -> int:
+        t = _lib.load().fastrrt_batch_submit(self._ptr, ctypes.byref(spec.req))
+        if t < 0:
+            _lib.check("batch_submit")
+        self._keep[spec.req.user_tag] = spec  # frames stay alive until the plan is polled
+        return t
+
+    def step(self, iterations=1) -> int:
+        n = _lib.load().fastrrt_batch_step(self._ptr, int(iterations))
+        if n < 0:
+            _lib.check("batch_step")
+        return n
+
+    def poll(self):
+        L = _lib.load()
+        res = _lib.PlanResult()
+        out = []
+        while L.fastrrt_batch_poll(self._ptr, ctypes.byref(res), self._buf.ctypes.data_as(_lib.fp), self._cap):
+            r = Result()
+            r.tag, r.status, r.iterations = res.user_tag, res.status, res.iterations
+            n = res.path_len
+            r.path = self._buf[:3 * n].reshape(n, 3).copy() if n > 0 else np.zeros((0, 3), np.float32)
+            r.submit_ms, r.done_ms = res.submit_ms, res.done_ms
+            self._keep.pop(r.tag, None)
+            out.append(r)
+        _lib.check("batch_poll")
+        return out
+
+    def drain(self):
+        _lib.load().fastrrt_batch_drain(self._ptr)
+        _lib.check("batch_drain")
+        return self.poll()
+
+    def clock_ms(self) -> float:
+        return _lib.load().fastrrt_batch_clock_ms(self._ptr)

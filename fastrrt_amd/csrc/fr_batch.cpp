@@ -1,0 +1,476 @@
+/*
+ * fr_batch.cpp — batched and served planning on one GPU.
+ *
+ * Many FastRRT plans of mixed grid size advance together: one loop
+ * iteration of every running plan is the fixed kernel sequence of
+ * fr_launch_iteration() over the engine's flattened (plan, tile) table,
+ * captured once as a hipGraph and replayed.  Requests are admitted as slots
+ * and the cell budget (HBM) free up, so plans that arrive while others run
+ * join at the next iteration (continuous batching); finished plans leave
+ * after their path_optimize rounds and path extraction, which also run
+ * batched.  Each plan draws its random numbers from Philox keyed by its own
+ * (seed, stream), so a plan's result does not depend on what else is in the
+ * batch.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <tuple>
+#include <vector>
+
+#include "../../include/fastrrt.h"
+#include "fr_engine.h"
+#include "fr_math.h"
+
+namespace {
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Pending {
+    fastrrt_plan_request req;
+    int ticket;
+    double t_submit;
+};
+
+struct Finished {
+    fastrrt_plan_result res;
+    std::vector<float> path; /* (x, z, heading) triples */
+};
+
+struct Slot {
+    bool busy = false;
+    int ticket = -1;
+    Pending p;
+    std::unique_ptr<FrSlab> slab;
+    long cells = 0;
+};
+
+} // namespace
+
+class FrBatch {
+  public:
+    FrBatch(int max_plans, long max_cells)
+        : eng_(max_plans, std::max<long>(max_cells, 1) + 1), max_cells_(max_cells), slots_(max_plans)
+    {
+        if (max_plans <= 0 || max_cells <= 0)
+            throw std::runtime_error("invalid batch capacity");
+        FR_CHECK(hipHostMalloc((void **)&h_ctl_, sizeof(FrCtl) * max_plans, hipHostMallocDefault));
+        FR_CHECK(hipMalloc(&d_list_, sizeof(int) * max_plans));
+        t0_ = now_ms();
+    }
+
+    ~FrBatch()
+    {
+        (void)hipStreamSynchronize(eng_.stream());
+        if (graph_exec_)
+            (void)hipGraphExecDestroy(graph_exec_);
+        if (graph_)
+            (void)hipGraphDestroy(graph_);
+        if (h_ctl_)
+            (void)hipHostFree(h_ctl_);
+        if (d_list_)
+            (void)hipFree(d_list_);
+    }
+
+    int submit(const fastrrt_plan_request &r)
+    {
+        const FrFrame *f = static_cast<const FrFrame *>(r.frame);
+        if (!f)
+            throw std::runtime_error("request without a frame");
+        if ((long)f->W * f->H > max_cells_)
+            throw std::runtime_error("request grid exceeds the batch cell budget");
+        Pending p;
+        p.req = r;
+        p.ticket = next_ticket_++;
+        p.t_submit = now_ms() - t0_;
+        queue_.push_back(p);
+        return p.ticket;
+    }
+
+    int in_flight() const { return (int)queue_.size() + running_; }
+
+    /* admit -> `iterations` loop iterations -> retire */
+    int step(int iterations)
+    {
+        admit();
+        if (running_ > 0) {
+            for (int k = 0; k < iterations; ++k)
+                launch_iteration();
+            poll_and_retire();
+        }
+        return in_flight();
+    }
+
+    bool poll(fastrrt_plan_result *res, float *path, int cap)
+    {
+        if (done_.empty())
+            return false;
+        Finished &f = done_.front();
+        *res = f.res;
+        const int n = (int)(f.path.size() / 3);
+        if (path && cap >= n) {
+            std::memcpy(path, f.path.data(), sizeof(float) * f.path.size());
+            res->path_len = n;
+        } else if (n > 0) {
+            res->path_len = -1;
+        }
+        done_.pop_front();
+        return true;
+    }
+
+    void drain()
+    {
+        while (in_flight() > 0)
+            step(4);
+    }
+
+    double clock_ms() const { return now_ms() - t0_; }
+
+  private:
+    static void fill_params(FrPlanParams &p, const fastrrt_plan_request &r, const FrFrame *f)
+    {
+        p.W = f->W;
+        p.H = f->H;
+        p.min_dx = r.min_dist_x;
+        p.min_dz = r.min_dist_z;
+        p.lbx = r.lower_bound_x;
+        p.lbz = r.lower_bound_z;
+        p.ubx = r.upper_bound_x;
+        p.ubz = r.upper_bound_z;
+        p.og_x = (float)(f->W / 2);
+        p.og_y = (float)(f->H / 2);
+        p.v = r.velocity_m_s;
+        p.max_path = r.max_path_size;
+        p.dist_tol = r.dist_to_goal_tolerance;
+        p.herr_tol = r.heading_error_tolerance_rad;
+        p.max_steer_rad = (double)r.max_steering_angle_deg * FR_TO_RAD_D;
+        p.lr = (double)(r.vehicle_length / 2.0f);
+        const float t = fr_tanf((float)p.max_steer_rad);
+        p.max_curv_loop = r.max_curvature < 0
+                              ? (float)((double)(2.0f * t) /
+                                        (0.5 * (double)r.vehicle_length * (double)sqrtf(4.0f + t)))
+                              : r.max_curvature;
+        p.max_curv_goal = 0.8f; /* fastrrt.cpp:79 */
+        p.sx = r.start_x;
+        p.sz = r.start_z;
+        p.sh = r.start_heading_rad;
+        p.gx = r.goal_x;
+        p.gz = r.goal_z;
+        p.gh = r.goal_heading_rad;
+        p.smart = r.smart ? 1 : 0;
+        p.max_iter = r.max_iterations > 0 ? r.max_iterations : 0;
+        p.seed = r.seed;
+        p.stream = r.stream;
+        p.region_side = 128;
+    }
+
+    std::unique_ptr<FrSlab> take_slab(int W, int H, int mx, int mz)
+    {
+        auto key = std::make_tuple(W, H, mx, mz);
+        auto it = pool_.find(key);
+        std::unique_ptr<FrSlab> s;
+        if (it != pool_.end() && !it->second.empty()) {
+            s = std::move(it->second.back());
+            it->second.pop_back();
+            s->allocate(W, H, mx, mz, path_cap(W, H), false, eng_.stream()); /* re-zero */
+        } else {
+            s.reset(new FrSlab());
+            s->allocate(W, H, mx, mz, path_cap(W, H), false, eng_.stream());
+        }
+        return s;
+    }
+
+    static int path_cap(int W, int H) { return std::min(W * H + 1, 1 << 16); }
+
+    void give_back(std::unique_ptr<FrSlab> s)
+    {
+        auto key = std::make_tuple(s->W, s->H, s->mx, s->mz);
+        auto &v = pool_[key];
+        if (v.size() < 8)
+            v.push_back(std::move(s));
+    }
+
+    void admit()
+    {
+        std::vector<int> admitted;
+        while (!queue_.empty()) {
+            int free_slot = -1;
+            for (int i = 0; i < (int)slots_.size(); ++i)
+                if (!slots_[i].busy) {
+                    free_slot = i;
+                    break;
+                }
+            if (free_slot < 0)
+                break;
+            const Pending &p = queue_.front();
+            const FrFrame *f = static_cast<const FrFrame *>(p.req.frame);
+            const long cells = (long)f->W * f->H;
+            if (cells_in_use_ + cells > max_cells_)
+                break;
+            Slot &s = slots_[free_slot];
+            FrPlanParams prm;
+            fill_params(prm, p.req, f);
+            const int mx = fr_trunc_f((float)p.req.min_dist_x / 2.0f);
+            const int mz = fr_trunc_f((float)p.req.min_dist_z / 2.0f);
+            s.slab = take_slab(f->W, f->H, mx, mz);
+            FrFrameView v;
+            v.xyz = f->d_xyz;
+            v.costs = f->d_costs;
+            v.n_classes = f->n_classes;
+            eng_.bind(free_slot, *s.slab, prm, v);
+            s.busy = true;
+            s.ticket = p.ticket;
+            s.p = p;
+            s.cells = cells;
+            cells_in_use_ += cells;
+            admitted.push_back(free_slot);
+            queue_.pop_front();
+        }
+        if (admitted.empty())
+            return;
+        /* per-plan setup (goal-connection precompute) for the admitted set */
+        eng_.setup(admitted, true);
+        for (int sl : admitted) {
+            FrCtl c{};
+            c.status = FR_SEARCHING;
+            c.pending_start = 1;
+            std::memcpy(&h_ctl_[sl], &c, sizeof(FrCtl));
+            FR_CHECK(hipMemcpyAsync(eng_.launch().ctl + sl, &h_ctl_[sl], sizeof(FrCtl), hipMemcpyHostToDevice,
+                                    eng_.stream()));
+        }
+        FR_CHECK(hipMemcpyAsync(d_list_, admitted.data(), sizeof(int) * admitted.size(), hipMemcpyHostToDevice,
+                                eng_.stream()));
+        fr_launch_batch_start(eng_.launch(), d_list_, (int)admitted.size(), eng_.stream());
+        FR_CHECK(hipGetLastError());
+        running_ += (int)admitted.size();
+        rebuild_active();
+    }
+
+    void rebuild_active()
+    {
+        std::vector<int> act;
+        for (int i = 0; i < (int)slots_.size(); ++i)
+            if (slots_[i].busy)
+                act.push_back(i);
+        eng_.set_active(act); /* synchronises the stream */
+    }
+
+    void launch_iteration()
+    {
+        if (!graph_exec_) {
+            const bool prof = fr_prof_on();
+            fr_prof_enable(false); /* no event nodes inside the captured graph */
+            FR_CHECK(hipStreamBeginCapture(eng_.stream(), hipStreamCaptureModeThreadLocal));
+            fr_launch_iteration(eng_.launch(), eng_.stream());
+            FR_CHECK(hipStreamEndCapture(eng_.stream(), &graph_));
+            fr_prof_enable(prof);
+            FR_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+        }
+        fr_prof_begin(FR_PROF_ITERATION, eng_.stream());
+        FR_CHECK(hipGraphLaunch(graph_exec_, eng_.stream()));
+        fr_prof_end(FR_PROF_ITERATION, eng_.stream());
+    }
+
+    void poll_and_retire()
+    {
+        const int ns = (int)slots_.size();
+        FR_CHECK(hipMemcpyAsync(h_ctl_, eng_.launch().ctl, sizeof(FrCtl) * ns, hipMemcpyDeviceToHost,
+                                eng_.stream()));
+        FR_CHECK(hipStreamSynchronize(eng_.stream()));
+        std::vector<int> found, failed;
+        const double t = now_ms() - t0_;
+        for (int i = 0; i < ns; ++i) {
+            if (!slots_[i].busy)
+                continue;
+            if (h_ctl_[i].status == FR_FOUND)
+                found.push_back(i);
+            else if (h_ctl_[i].status == FR_FAILED)
+                failed.push_back(i);
+            else if (slots_[i].p.req.deadline_ms > 0 && t - slots_[i].p.t_submit > slots_[i].p.req.deadline_ms)
+                failed.push_back(i); /* the timeout of fastrrt.cpp:110-114, per request */
+        }
+        if (found.empty() && failed.empty())
+            return;
+        if (!found.empty())
+            finalize(found);
+        for (int sl : failed)
+            retire(sl, FR_FAILED, nullptr, 0);
+        rebuild_active();
+    }
+
+    /* path_optimize rounds (fastrrt.cpp:183-196) and getPlannedPath
+     * (:204-232) for every plan that reached its goal */
+    void finalize(const std::vector<int> &sl)
+    {
+        const FrLaunch &L = eng_.launch();
+        hipStream_t st = eng_.stream();
+        int max_N = 0, max_rounds = 0;
+        for (int s : sl) {
+            max_N = std::max(max_N, (int)(slots_[s].slab->W * slots_[s].slab->H));
+            max_rounds = std::max(max_rounds, slots_[s].p.req.optimize_rounds);
+        }
+        for (int s : sl) {
+            h_ctl_[s].opt_active = slots_[s].p.req.optimize_rounds > 0 ? 1 : 0;
+            h_ctl_[s].opt_limit = slots_[s].p.req.optimize_rounds;
+            h_ctl_[s].opt_rounds = 0;
+            FR_CHECK(hipMemcpyAsync(L.ctl + s, &h_ctl_[s], sizeof(FrCtl), hipMemcpyHostToDevice, st));
+        }
+        FR_CHECK(hipMemcpyAsync(d_list_, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice, st));
+        const int n = (int)sl.size();
+        for (int r = 0; r < max_rounds; ++r) {
+            fr_launch_goal_scan_multi(L, d_list_, n, max_N, FR_GATE_OPT, st);
+            fr_launch_best_node(L, d_list_, n, max_N, nullptr, st, FR_GATE_OPT);
+            fr_launch_walk(L, d_list_, n, st, FR_GATE_OPT);
+            fr_launch_opt_round(L, d_list_, n, max_path_len_, st, FR_GATE_OPT);
+            if ((r & 3) == 3) {
+                /* stop early when every plan has converged */
+                FR_CHECK(hipMemcpyAsync(h_ctl_, L.ctl, sizeof(FrCtl) * slots_.size(), hipMemcpyDeviceToHost, st));
+                FR_CHECK(hipStreamSynchronize(st));
+                bool any = false;
+                for (int s : sl)
+                    any |= h_ctl_[s].opt_active != 0;
+                if (!any)
+                    break;
+            }
+        }
+        fr_launch_goal_scan_multi(L, d_list_, n, max_N, FR_GATE_FOUND, st);
+        fr_launch_best_node(L, d_list_, n, max_N, nullptr, st, FR_GATE_FOUND);
+        fr_launch_walk(L, d_list_, n, st, FR_GATE_FOUND);
+        FR_CHECK(hipGetLastError());
+        FR_CHECK(hipMemcpyAsync(h_ctl_, L.ctl, sizeof(FrCtl) * slots_.size(), hipMemcpyDeviceToHost, st));
+        FR_CHECK(hipStreamSynchronize(st));
+        for (int s : sl) {
+            const int len = h_ctl_[s].goal_reached ? std::max(0, h_ctl_[s].path_len) : 0;
+            std::vector<float4> p(len);
+            if (len)
+                FR_CHECK(hipMemcpyAsync(p.data(), slots_[s].slab->path, sizeof(float4) * len, hipMemcpyDeviceToHost,
+                                        st));
+            FR_CHECK(hipStreamSynchronize(st));
+            std::vector<float> trip(3 * (size_t)len);
+            for (int k = 0; k < len; ++k) {
+                trip[3 * k] = (float)(int)p[k].x;
+                trip[3 * k + 1] = (float)(int)p[k].y;
+                trip[3 * k + 2] = p[k].z;
+            }
+            retire(s, FR_FOUND, trip.data(), len);
+        }
+    }
+
+    void retire(int s, int status, const float *path, int len)
+    {
+        Slot &sl = slots_[s];
+        Finished f;
+        f.res.user_tag = sl.p.req.user_tag;
+        f.res.status = status;
+        f.res.iterations = h_ctl_[s].iter;
+        f.res.path_len = len;
+        f.res.submit_ms = sl.p.t_submit;
+        f.res.done_ms = now_ms() - t0_;
+        if (len > 0)
+            f.path.assign(path, path + 3 * (size_t)len);
+        done_.push_back(std::move(f));
+        FrCtl idle{};
+        idle.status = FR_IDLE;
+        std::memcpy(&h_ctl_[s], &idle, sizeof(FrCtl));
+        FR_CHECK(hipMemcpyAsync(eng_.launch().ctl + s, &h_ctl_[s], sizeof(FrCtl), hipMemcpyHostToDevice,
+                                eng_.stream()));
+        FR_CHECK(hipStreamSynchronize(eng_.stream()));
+        give_back(std::move(sl.slab));
+        cells_in_use_ -= sl.cells;
+        sl.busy = false;
+        sl.ticket = -1;
+        running_ -= 1;
+    }
+
+    FrEngine eng_;
+    long max_cells_;
+    long cells_in_use_ = 0;
+    std::vector<Slot> slots_;
+    std::deque<Pending> queue_;
+    std::deque<Finished> done_;
+    std::map<std::tuple<int, int, int, int>, std::vector<std::unique_ptr<FrSlab>>> pool_;
+    FrCtl *h_ctl_ = nullptr;
+    int *d_list_ = nullptr;
+    int running_ = 0;
+    int next_ticket_ = 0;
+    int max_path_len_ = 1024;
+    double t0_ = 0;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+};
+
+#define FR_TRY(ret_on_error, body)                                                                        \
+    try {                                                                                                  \
+        fr_set_error("");                                                                                  \
+        body                                                                                               \
+    } catch (const std::exception &e) {                                                                    \
+        fr_set_error(e.what());                                                                            \
+        return ret_on_error;                                                                               \
+    } catch (...) {                                                                                        \
+        fr_set_error("unknown error");                                                                     \
+        return ret_on_error;                                                                               \
+    }
+
+static FrBatch *B(void *p)
+{
+    if (!p)
+        throw std::runtime_error("null batch handle");
+    return static_cast<FrBatch *>(p);
+}
+
+extern "C" {
+
+void *fastrrt_batch_create(int max_plans, int64_t max_cells)
+{
+    FR_TRY(nullptr, { return (void *)new FrBatch(max_plans, (long)max_cells); })
+}
+
+void fastrrt_batch_destroy(void *batch)
+{
+    try {
+        delete static_cast<FrBatch *>(batch);
+    } catch (...) {
+    }
+}
+
+int fastrrt_batch_submit(void *batch, const fastrrt_plan_request *req)
+{
+    FR_TRY(-1, {
+        if (!req)
+            throw std::runtime_error("null request");
+        return B(batch)->submit(*req);
+    })
+}
+
+int fastrrt_batch_step(void *batch, int iterations) { FR_TRY(-1, { return B(batch)->step(iterations); }) }
+
+bool fastrrt_batch_poll(void *batch, fastrrt_plan_result *result, float *path, int cap)
+{
+    FR_TRY(false, { return B(batch)->poll(result, path, cap); })
+}
+
+void fastrrt_batch_drain(void *batch)
+{
+    try {
+        fr_set_error("");
+        B(batch)->drain();
+    } catch (const std::exception &e) {
+        fr_set_error(e.what());
+    }
+}
+
+double fastrrt_batch_clock_ms(void *batch) { FR_TRY(-1.0, { return B(batch)->clock_ms(); }) }
+
+} /* extern "C" */

@@ -99,10 +99,10 @@ struct FrCtl {
     int opt_rounds;
     int opt_changed;
     float opt_gain_f;
-    int opt_best_i;
+    int opt_limit;            /* batch finalisation: max successful rounds */
     unsigned long long opt_key;
     int full_check;           /* next connect pass must scan all NODE cells */
-    int pad0;
+    int opt_active;           /* batch finalisation: optimisation rounds pending */
 };
 
 /* One queued expansion (one rollout). */

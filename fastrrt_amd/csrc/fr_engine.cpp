@@ -239,6 +239,8 @@ FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
     FR_CHECK(hipMalloc(&d_setup_slots_, sizeof(int) * max_slots));
     FR_CHECK(hipMalloc(&L_.tiles, sizeof(int2) * max_tiles));
     FR_CHECK(hipMalloc(&L_.n_tiles, sizeof(int)));
+    FR_CHECK(hipMalloc(&L_.n_slots, sizeof(int)));
+    FR_CHECK(hipMemsetAsync(L_.n_slots, 0, sizeof(int), st_));
     FR_CHECK(hipMalloc(&L_.counters, sizeof(int) * FR_CNT_N));
     FR_CHECK(hipMalloc(&L_.exp_q, sizeof(FrExp) * (queue_cap + 1)));
     FR_CHECK(hipMalloc(&L_.fresh_q, sizeof(FrFresh) * (2 * queue_cap + max_slots + 1)));
@@ -249,6 +251,7 @@ FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
     L_.grid_tiles = (int)std::min<long>(max_tiles, 2048);
     L_.grid_items = (int)std::max<long>(1, std::min<long>((queue_cap + FR_BLOCK - 1) / FR_BLOCK, 2048));
     L_.grid_slots = 1;
+    L_.max_slots = max_slots;
 }
 
 FrEngine::~FrEngine()
@@ -260,6 +263,7 @@ FrEngine::~FrEngine()
     (void)hipFree(d_setup_slots_);
     (void)hipFree(L_.tiles);
     (void)hipFree(L_.n_tiles);
+    (void)hipFree(L_.n_slots);
     (void)hipFree(L_.counters);
     (void)hipFree(L_.exp_q);
     (void)hipFree(L_.fresh_q);
@@ -375,9 +379,11 @@ void FrEngine::set_active(const std::vector<int> &slots)
         FR_CHECK(hipMemcpyAsync(L_.slots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, st_));
     if (nt)
         FR_CHECK(hipMemcpyAsync(L_.tiles, h_tiles_.data(), sizeof(int2) * nt, hipMemcpyHostToDevice, st_));
-    static thread_local int staged_nt;
-    staged_nt = nt;
-    FR_CHECK(hipMemcpyAsync(L_.n_tiles, &staged_nt, sizeof(int), hipMemcpyHostToDevice, st_));
+    static thread_local int staged[2];
+    staged[0] = nt;
+    staged[1] = (int)slots.size();
+    FR_CHECK(hipMemcpyAsync(L_.n_tiles, &staged[0], sizeof(int), hipMemcpyHostToDevice, st_));
+    FR_CHECK(hipMemcpyAsync(L_.n_slots, &staged[1], sizeof(int), hipMemcpyHostToDevice, st_));
     FR_CHECK(hipStreamSynchronize(st_));
 }
 
@@ -401,7 +407,7 @@ void FrEngine::setup(const std::vector<int> &slots, bool goal_pass)
 
 void FrEngine::iteration()
 {
-    fr_launch_iteration(L_, (int)active_.size(), st_);
+    fr_launch_iteration(L_, st_);
     FR_CHECK(hipGetLastError());
 }
 

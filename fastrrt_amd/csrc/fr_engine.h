@@ -27,7 +27,9 @@ struct FrLaunch {
     int *counters;
     FrExp *exp_q;
     FrFresh *fresh_q;
+    int *n_slots;             /* device: entries of `slots` */
     int grid_tiles, grid_items, grid_slots;
+    int max_slots;
 };
 
 struct FrBestNodeArgs {
@@ -38,15 +40,21 @@ struct FrBestNodeArgs {
 /* kernel launchers (fr_kernels.hip) */
 void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, int max_PH, long max_PWH,
                      bool goal_pass, hipStream_t st);
-void fr_launch_iteration(const FrLaunch &L, int n_slots, hipStream_t st);
+void fr_launch_iteration(const FrLaunch &L, hipStream_t st);
 void fr_launch_clear(const FrLaunch &L, int slot, int N, hipStream_t st);
 void fr_launch_count(const FrLaunch &L, int slot, int N, int *d_out, hipStream_t st);
 void fr_launch_goal_scan(const FrLaunch &L, int slot, int N, int gx, int gz, float gh, float tol, float herr,
                          hipStream_t st);
+/* gate: 0 = always (API calls), 1 = plans with an optimisation round pending,
+ * 2 = plans whose goal was reached (final extraction) */
+enum { FR_GATE_NONE = 0, FR_GATE_OPT = 1, FR_GATE_FOUND = 2 };
 void fr_launch_best_node(const FrLaunch &L, const int *d_slots, int n, int max_N, const FrBestNodeArgs *ov,
-                         hipStream_t st);
-void fr_launch_walk(const FrLaunch &L, const int *d_slots, int n, hipStream_t st);
-void fr_launch_opt_round(const FrLaunch &L, const int *d_slots, int n, int max_path, hipStream_t st);
+                         hipStream_t st, int gate = FR_GATE_NONE);
+void fr_launch_walk(const FrLaunch &L, const int *d_slots, int n, hipStream_t st, int gate = FR_GATE_NONE);
+void fr_launch_opt_round(const FrLaunch &L, const int *d_slots, int n, int max_path, hipStream_t st,
+                         int gate = FR_GATE_NONE);
+void fr_launch_goal_scan_multi(const FrLaunch &L, const int *d_slots, int n, int max_N, int gate, hipStream_t st);
+void fr_launch_batch_start(const FrLaunch &L, const int *d_slots, int n, hipStream_t st);
 void fr_launch_graph_add(const FrLaunch &L, int slot, int x, int z, float h, int px, int pz, float cost, int type,
                          int count_parent, hipStream_t st);
 void fr_launch_set_collision(const FrLaunch &L, int slot, int x, int z, int npx, int npz, float h, float cost,

@@ -394,9 +394,10 @@ __global__ void k_setup_goal_connection(const FrPlanDev *plans, const int *slots
 /* loop iteration                                                           */
 /* ------------------------------------------------------------------------ */
 
-__global__ void k_begin(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n_slots, int *counters)
+__global__ void k_begin(const FrPlanDev *plans, FrCtl *ctl, const int *slots, const int *n_slots_p, int *counters)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n_slots = *n_slots_p;
     if (i == 0) {
         counters[FR_CNT_EXP] = 0;
         counters[FR_CNT_FRESH] = 0;
@@ -855,9 +856,11 @@ __global__ void k_connect_pick(const FrPlanDev *plans, FrCtl *ctl, const FrFresh
 
 /* End of FastRRT::loop (fastrrt.cpp:130-180): reseed, goal-connection adds,
  * goal test, iteration bookkeeping. */
-__global__ void k_finish(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n_slots, const FrFresh *fq)
+__global__ void k_finish(const FrPlanDev *plans, FrCtl *ctl, const int *slots, const int *n_slots_p,
+                         const FrFresh *fq)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n_slots = *n_slots_p;
     if (i >= n_slots)
         return;
     const int s = slots[i];
@@ -949,12 +952,23 @@ __global__ void k_goal_scan(const FrPlanDev *plans, FrCtl *ctl, int slot, int gx
     }
 }
 
-__global__ void k_best_node_init(FrCtl *ctl, const int *slots, int n)
+__device__ __forceinline__ bool dev_gate(const FrCtl &C, int gate)
+{
+    if (gate == 1)
+        return C.status == FR_FOUND && C.opt_active && C.goal_reached;
+    if (gate == 2)
+        return C.status == FR_FOUND && C.goal_reached;
+    return true;
+}
+
+__global__ void k_best_node_init(FrCtl *ctl, const int *slots, int n, int gate)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     FrCtl &C = ctl[slots[i]];
+    if (!dev_gate(C, gate))
+        return;
     C.bn_cost = (long long)~0ull;
     C.bn_pos = 0x7FFFFFFF;
 }
@@ -963,11 +977,13 @@ __global__ void k_best_node_init(FrCtl *ctl, const int *slots, int n)
  * Ties: lowest cell index (second kernel). */
 __global__ void k_best_node_cost(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int tx_override,
                                  int tz_override, float h_override, float r_override, float herr_override,
-                                 int use_override)
+                                 int use_override, int gate)
 {
     const int s = slots[blockIdx.y];
     const FrPlanDev &P = plans[s];
     FrCtl &C = ctl[s];
+    if (!dev_gate(C, gate))
+        return;
     const int tx = use_override ? tx_override : P.gx;
     const int tz = use_override ? tz_override : P.gz;
     const float th = use_override ? h_override : P.gh;
@@ -994,11 +1010,13 @@ __global__ void k_best_node_cost(const FrPlanDev *plans, FrCtl *ctl, const int *
 
 __global__ void k_best_node_pick(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int tx_override,
                                  int tz_override, float h_override, float r_override, float herr_override,
-                                 int use_override)
+                                 int use_override, int gate)
 {
     const int s = slots[blockIdx.y];
     const FrPlanDev &P = plans[s];
     FrCtl &C = ctl[s];
+    if (!dev_gate(C, gate))
+        return;
     const unsigned long long best = (unsigned long long)C.bn_cost;
     if (best >= fr_lcost_key(FR_BIG_COST))
         return;
@@ -1033,7 +1051,7 @@ __global__ void k_best_node_pick(const FrPlanDev *plans, FrCtl *ctl, const int *
  * (-1, -1) for an out-of-bounds or NULL cell, so such a cell is still pushed
  * and ends the walk.  One thread per plan; a path longer than the buffer is
  * reported as empty, as the reference does after 10^6 steps. */
-__global__ void k_walk(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n_slots)
+__global__ void k_walk(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n_slots, int gate)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_slots)
@@ -1041,6 +1059,8 @@ __global__ void k_walk(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int
     const int s = slots[i];
     const FrPlanDev &P = plans[s];
     FrCtl &C = ctl[s];
+    if (!dev_gate(C, gate))
+        return;
     C.path_len = 0;
     if ((unsigned long long)C.bn_cost >= fr_lcost_key(FR_BIG_COST) || C.bn_pos == 0x7FFFFFFF)
         return;
@@ -1069,11 +1089,13 @@ __global__ void k_walk(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int
 
 /* graph_optimization.cu:25-56 for all (i, j >= i + 2): each i keeps the last
  * improving j (atomicMax over j). pd[i].x holds that j (-1: none). */
-__global__ void k_opt_pairs(const FrPlanDev *plans, FrCtl *ctl, const int *slots)
+__global__ void k_opt_pairs(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int gate)
 {
     const int s = slots[blockIdx.y];
     const FrPlanDev &P = plans[s];
     const FrCtl &C = ctl[s];
+    if (!dev_gate(C, gate))
+        return;
     const int n = C.path_len;
     if (n < 3)
         return;
@@ -1092,10 +1114,12 @@ __global__ void k_opt_pairs(const FrPlanDev *plans, FrCtl *ctl, const int *slots
     }
 }
 
-__global__ void k_opt_reset(const FrPlanDev *plans, FrCtl *ctl, const int *slots)
+__global__ void k_opt_reset(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int gate)
 {
     const int s = slots[blockIdx.y];
     const FrPlanDev &P = plans[s];
+    if (!dev_gate(ctl[s], gate))
+        return;
     const int n = ctl[s].path_len;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         int *pi = (int *)&P.pd[i].x;
@@ -1106,13 +1130,15 @@ __global__ void k_opt_reset(const FrPlanDev *plans, FrCtl *ctl, const int *slots
 /* graph_optimization.cu:126-183: the first i with the largest gain wins; the
  * path is spliced, the grid cleared and the chain re-added.  One block per
  * plan; lane 0 does the sequential splice/rebuild, the block clears. */
-__global__ void k_opt_apply(const FrPlanDev *plans, FrCtl *ctl, const int *slots)
+__global__ void k_opt_apply(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int gate)
 {
     const int s = slots[blockIdx.x];
     const FrPlanDev &P = plans[s];
     FrCtl &C = ctl[s];
     __shared__ int s_go;
     __shared__ int s_n;
+    if (!dev_gate(C, gate))
+        return;
     if (threadIdx.x == 0) {
         int n = C.path_len;
         C.opt_changed = 0;
@@ -1144,8 +1170,13 @@ __global__ void k_opt_apply(const FrPlanDev *plans, FrCtl *ctl, const int *slots
             }
             C.path_len = n;
             C.opt_changed = 1;
+            C.opt_rounds += 1;
+            if (gate == 1 && C.opt_rounds >= C.opt_limit)
+                C.opt_active = 0;
             C.goal_reached = 0;
             s_go = 1;
+        } else {
+            C.opt_active = 0;
         }
         s_n = n;
     }
@@ -1200,13 +1231,14 @@ void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, in
     }
 }
 
-void fr_launch_iteration(const FrLaunch &L, int n_slots, hipStream_t st)
+void fr_launch_iteration(const FrLaunch &L, hipStream_t st)
 {
+    const int n_slots = L.max_slots;
     dim3 b(FR_BLOCK);
     const int gs = L.grid_slots;
     fr_prof_begin(FR_PROF_ITERATION, st);
     hipLaunchKernelGGL(k_begin, dim3(fr_grid_for(n_slots, FR_BLOCK, 65535)), b, 0, st, L.plans, L.ctl, L.slots,
-                       n_slots, L.counters);
+                       L.n_slots, L.counters);
     hipLaunchKernelGGL(k_select, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles, L.exp_q,
                        L.counters);
     fr_prof_begin(FR_PROF_EXPAND, st);
@@ -1225,7 +1257,7 @@ void fr_launch_iteration(const FrLaunch &L, int n_slots, hipStream_t st)
     fr_prof_end(FR_PROF_CONNECT, st);
     hipLaunchKernelGGL(k_connect_pick, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);
     hipLaunchKernelGGL(k_finish, dim3(fr_grid_for(n_slots, FR_BLOCK, 65535)), b, 0, st, L.plans, L.ctl, L.slots,
-                       n_slots, L.fresh_q);
+                       L.n_slots, L.fresh_q);
     fr_prof_end(FR_PROF_ITERATION, st);
     (void)gs;
 }
@@ -1249,29 +1281,72 @@ void fr_launch_goal_scan(const FrLaunch &L, int slot, int N, int gx, int gz, flo
 }
 
 void fr_launch_best_node(const FrLaunch &L, const int *d_slots, int n, int max_N, const FrBestNodeArgs *ov,
-                         hipStream_t st)
+                         hipStream_t st, int gate)
 {
     const int use = ov != nullptr;
     const int tx = use ? ov->x : 0, tz = use ? ov->z : 0;
     const float h = use ? ov->heading : 0, r = use ? ov->radius : 0, e = use ? ov->max_herr : 0;
     dim3 g(fr_grid_for(max_N, FR_BLOCK, 1024), n);
-    hipLaunchKernelGGL(k_best_node_init, dim3(fr_grid_for(n, 64, 65535)), dim3(64), 0, st, L.ctl, d_slots, n);
-    hipLaunchKernelGGL(k_best_node_cost, g, dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots, tx, tz, h, r, e, use);
-    hipLaunchKernelGGL(k_best_node_pick, g, dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots, tx, tz, h, r, e, use);
+    hipLaunchKernelGGL(k_best_node_init, dim3(fr_grid_for(n, 64, 65535)), dim3(64), 0, st, L.ctl, d_slots, n, gate);
+    hipLaunchKernelGGL(k_best_node_cost, g, dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots, tx, tz, h, r, e, use,
+                       gate);
+    hipLaunchKernelGGL(k_best_node_pick, g, dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots, tx, tz, h, r, e, use,
+                       gate);
 }
 
-void fr_launch_walk(const FrLaunch &L, const int *d_slots, int n, hipStream_t st)
+void fr_launch_walk(const FrLaunch &L, const int *d_slots, int n, hipStream_t st, int gate)
 {
-    hipLaunchKernelGGL(k_walk, dim3(fr_grid_for(n, 64, 65535)), dim3(64), 0, st, L.plans, L.ctl, d_slots, n);
+    hipLaunchKernelGGL(k_walk, dim3(fr_grid_for(n, 64, 65535)), dim3(64), 0, st, L.plans, L.ctl, d_slots, n, gate);
 }
 
-void fr_launch_opt_round(const FrLaunch &L, const int *d_slots, int n, int max_path, hipStream_t st)
+void fr_launch_opt_round(const FrLaunch &L, const int *d_slots, int n, int max_path, hipStream_t st, int gate)
 {
     dim3 g1(fr_grid_for(max_path, FR_BLOCK, 64), n);
-    hipLaunchKernelGGL(k_opt_reset, g1, dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots);
+    hipLaunchKernelGGL(k_opt_reset, g1, dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots, gate);
     dim3 g2(fr_grid_for((long)max_path * max_path, FR_BLOCK, 4096), n);
-    hipLaunchKernelGGL(k_opt_pairs, g2, dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots);
-    hipLaunchKernelGGL(k_opt_apply, dim3(n), dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots);
+    hipLaunchKernelGGL(k_opt_pairs, g2, dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots, gate);
+    hipLaunchKernelGGL(k_opt_apply, dim3(n), dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots, gate);
+}
+
+/* graph_best_node.cu:182-222 for several plans (blockIdx.y), skipping plans
+ * whose flag is already set (the flag is sticky, :229-230) */
+__global__ void k_goal_scan_multi(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int gate)
+{
+    const int s = slots[blockIdx.y];
+    const FrPlanDev &P = plans[s];
+    FrCtl &C = ctl[s];
+    if (C.status != FR_FOUND || (gate == 1 && !C.opt_active) || C.goal_reached)
+        return;
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+        if (P.type[pos] != FR_TYPE_NODE)
+            continue;
+        const int z = pos / P.W, x = pos - z * P.W;
+        if (dev_goal_hit(P, x, z, P.heading[pos]))
+            C.goal_reached = 1;
+    }
+}
+
+void fr_launch_goal_scan_multi(const FrLaunch &L, const int *d_slots, int n, int max_N, int gate, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_goal_scan_multi, dim3(fr_grid_for(max_N, FR_BLOCK, 256), n), dim3(FR_BLOCK), 0, st,
+                       L.plans, L.ctl, d_slots, gate);
+}
+
+/* search_init for a batch of freshly admitted plans (fastrrt.cpp:85-98): the
+ * slab is zero, so only the start node is added. */
+__global__ void k_batch_start(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int s = slots[i];
+    const FrPlanDev &P = plans[s];
+    dev_add(P, P.sx, P.sz, P.sh, -1, -1, 0.0f);
+}
+
+void fr_launch_batch_start(const FrLaunch &L, const int *d_slots, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_batch_start, dim3(fr_grid_for(n, 64, 65535)), dim3(64), 0, st, L.plans, L.ctl, d_slots, n);
 }
 
 /* ------------------------------------------------------------------------ */

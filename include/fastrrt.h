@@ -179,6 +179,58 @@ void cudagraph_derivate_node(void *ptr, void *frame, float steering_angle, float
 void cudagraph_export_state(void *ptr, int32_t *parent_x, int32_t *parent_z, int32_t *type, int32_t *child_count,
                             float *heading, float *cost, uint8_t *goal_flag, float *goal_heading, float *goal_cost);
 
+/* ---------------------------------------------------------------------- */
+/* Batched and served planning (new; no counterpart in the reference)      */
+/* ---------------------------------------------------------------------- */
+
+typedef struct fastrrt_plan_request {
+    void *frame;              /* fastrrt_frame_create() handle, borrowed until the plan completes */
+    int start_x, start_z;
+    float start_heading_rad;
+    int goal_x, goal_z;
+    float goal_heading_rad;
+    float velocity_m_s;
+    int min_dist_x, min_dist_z;
+    int lower_bound_x, lower_bound_z, upper_bound_x, upper_bound_z; /* ego box, -1 = none */
+    float max_steering_angle_deg, vehicle_length, max_curvature;
+    float max_path_size, dist_to_goal_tolerance, heading_error_tolerance_rad;
+    int max_iterations;       /* loop() iterations before the plan fails (0: none) */
+    int optimize_rounds;      /* path_optimize() calls after the goal is reached */
+    int smart;                /* smart (density-gated) expansion */
+    int pad0;
+    double deadline_ms;       /* from submission; the plan fails when it passes (<= 0: none) */
+    uint64_t seed, stream;    /* Philox key and stream */
+    int64_t user_tag;
+} fastrrt_plan_request;
+
+typedef struct fastrrt_plan_result {
+    int64_t user_tag;
+    int status;               /* 2 = goal reached, 3 = failed (iteration cap or deadline) */
+    int iterations;
+    int path_len;             /* waypoints written (-1: the caller's buffer is too small) */
+    int pad0;
+    double submit_ms, done_ms; /* batch clock */
+} fastrrt_plan_result;
+
+/* A batch of up to max_plans concurrent plans whose grids total at most
+ * max_cells cells.  Each loop iteration of all running plans is one
+ * replay of a captured hipGraph. */
+void *fastrrt_batch_create(int max_plans, int64_t max_cells);
+void fastrrt_batch_destroy(void *batch);
+/* Queue a request; returns a ticket >= 0, or -1 (see fastrrt_last_error). */
+int fastrrt_batch_submit(void *batch, const fastrrt_plan_request *req);
+/* Admit queued requests while slots and cells allow, run `iterations` loop
+ * iterations of every running plan, retire finished plans (optimisation and
+ * path extraction included).  Returns the plans still queued or running. */
+int fastrrt_batch_step(void *batch, int iterations);
+/* Pop one finished plan; its path as (x, z, heading) triples into `path`
+ * (capacity `cap` waypoints).  Returns false when none is finished. */
+bool fastrrt_batch_poll(void *batch, fastrrt_plan_result *result, float *path, int cap);
+/* Step until every submitted plan has finished. */
+void fastrrt_batch_drain(void *batch);
+/* Milliseconds since the batch was created (the clock of submit_ms/done_ms). */
+double fastrrt_batch_clock_ms(void *batch);
+
 #ifdef __cplusplus
 }
 #endif

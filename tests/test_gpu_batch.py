@@ -1,0 +1,99 @@
+"""The batched engine: a plan's result is the same alone, in a mixed batch,
+under continuous admission, and through the plain FastRRT handle; and it is
+the oracle's result."""
+import numpy as np
+import pytest
+
+from _parity import OraclePlanner, Params
+
+from fastrrt_amd import scenes
+from fastrrt_amd.batch import Batch, PlanSpec
+from fastrrt_amd import pyfastrrt as P
+
+pytestmark = pytest.mark.gpu
+
+SIZES = (96, 128, 160, 128, 96, 192)
+
+
+def _scene(i):
+    return scenes.scene_sized(300 + i, SIZES[i % len(SIZES)], 0.06 + 0.02 * (i % 3), min_dist=(10, 16))
+
+
+def _spec(sc, frame, i, max_iter=50, rounds=4, deadline=0.0):
+    return PlanSpec(frame, sc.start, sc.goal, min_dist=(10, 16), max_path_size=30.0, dist_to_goal_tolerance=12.0,
+                    max_iterations=max_iter, optimize_rounds=rounds, seed=777, stream=i, tag=i,
+                    deadline_ms=deadline)
+
+
+def _key(r):
+    return (r.status, r.iterations, r.path.shape, r.path.tobytes())
+
+
+def test_alone_equals_in_the_mix():
+    scs = [_scene(i) for i in range(10)]
+    frames = [P.SearchFrame(sc.frame, sc.class_costs) for sc in scs]
+    alone = {}
+    for i, sc in enumerate(scs):
+        print(f"[batch] alone {i} size={sc.width}", flush=True)
+        b = Batch(max_plans=1, max_cells=sc.width * sc.height)
+        b.submit(_spec(sc, frames[i], i))
+        (r,) = b.drain()
+        alone[i] = _key(r)
+    print("[batch] mixed", flush=True)
+    b = Batch(max_plans=len(scs), max_cells=sum(s.width * s.height for s in scs))
+    for i, sc in enumerate(scs):
+        b.submit(_spec(sc, frames[i], i))
+    mixed = {r.tag: _key(r) for r in b.drain()}
+    assert mixed == alone
+    print("[batch] continuous (3 slots, staggered)", flush=True)
+    b = Batch(max_plans=3, max_cells=3 * 192 * 192)
+    got = {}
+    for i, sc in enumerate(scs):
+        b.submit(_spec(sc, frames[i], i))
+        b.step(2)
+        for r in b.poll():
+            got[r.tag] = _key(r)
+    for r in b.drain():
+        got[r.tag] = _key(r)
+    assert got == alone
+    assert sum(1 for k in alone.values() if k[0] == 2) >= 3
+
+
+def test_batch_equals_plain_handle_and_oracle():
+    prm = Params(min_dist=(10, 16), max_path=30.0, dist_tol=12.0)
+    for i in range(4):
+        sc = _scene(i)
+        frame = P.SearchFrame(sc.frame, sc.class_costs)
+        b = Batch(max_plans=2, max_cells=2 * sc.width * sc.height)
+        b.submit(_spec(sc, frame, i, max_iter=40, rounds=3))
+        (r,) = b.drain()
+        o = OraclePlanner(sc, prm, 777, i)
+        o.search_init()
+        it = 0
+        while not o.goal_reached() and o.loop():
+            it += 1
+            if it >= 40:
+                break
+        if o.goal_reached():
+            k = 0
+            while k < 3 and o.path_optimize():
+                k += 1
+            po = o.path()
+            assert r.status == 2
+            assert r.path.shape == po.shape and (r.path.view(np.int32) == po.view(np.int32)).all(), i
+        else:
+            assert r.status == 3
+
+
+def test_deadline_retires_plans():
+    sc = scenes.scene_sized(901, 256, 0.30, min_dist=(22, 40))
+    frame = P.SearchFrame(sc.frame, sc.class_costs)
+    b = Batch(max_plans=4, max_cells=4 * 256 * 256)
+    for i in range(4):
+        b.submit(PlanSpec(frame, sc.start, sc.goal, max_iterations=0, optimize_rounds=0, deadline_ms=30.0,
+                          seed=1, stream=i, tag=i))
+    res = b.drain()
+    assert len(res) == 4
+    for r in res:
+        assert r.status in (2, 3)
+        assert r.latency_ms < 30.0 + 200.0  # retired at the first poll past the deadline
