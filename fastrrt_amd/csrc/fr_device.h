@@ -55,6 +55,7 @@ struct FrPlanDev {
     /* goal connection */
     uint8_t *gc_flag;
     float *gc_heading, *gc_cost;
+    int32_t *gc_rowsum;       /* per row, exclusive prefix count of gc_flag ((W + 1) per row) */
     /* scratch planes */
     unsigned long long *claim;
     uint8_t *mark;
@@ -119,11 +120,20 @@ struct FrExp {
 struct FrFresh {
     int slot;
     int pos;
-    long long lcost;          /* node-local best, FR_BIG_COST if none */
-    int widx;                 /* window index of the node-local best */
-    int c_pad;
-    float c_cost;             /* node -> window-cell cost */
-    float total;              /* c_cost + window cell's goal cost */
+    int flags;                /* bit0: converted from COLLISION; bit1: window evaluated inline */
+    int pair_base;            /* first (node, window cell) pair of this node */
+    int pair_cnt;
+    int pad0;
+    unsigned long long key;   /* node-local best connection (cost, then last in scan order) */
 };
+
+/* One (node, goal-flagged window cell) Hermite check. */
+struct FrPair {
+    int fresh;
+    int cell;
+};
+
+#define FR_FRESH_CONVERTED 1
+#define FR_FRESH_INLINE 2
 
 #endif

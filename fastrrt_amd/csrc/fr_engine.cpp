@@ -178,6 +178,7 @@ void FrSlab::allocate(int W_, int H_, int mx_, int mz_, int path_cap_, bool with
     const size_t o_gcf = take(N);
     const size_t o_gch = take(4 * N);
     const size_t o_gcc = take(4 * N);
+    const size_t o_rows = take(4 * (size_t)(W + 1) * H);
     const size_t o_mark = take(N);
     const size_t o_trav = take(N);
     const size_t o_region = take(4 * (size_t)2 * dens);
@@ -199,6 +200,7 @@ void FrSlab::allocate(int W_, int H_, int mx_, int mz_, int path_cap_, bool with
     gc_flag = (uint8_t *)(b + o_gcf);
     gc_heading = (float *)(b + o_gch);
     gc_cost = (float *)(b + o_gcc);
+    gc_rowsum = (int32_t *)(b + o_rows);
     mark = (uint8_t *)(b + o_mark);
     trav = (uint8_t *)(b + o_trav);
     region = (uint32_t *)(b + o_region);
@@ -245,12 +247,15 @@ FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
     FR_CHECK(hipMalloc(&L_.exp_q, sizeof(FrExp) * (queue_cap + 1)));
     FR_CHECK(hipMalloc(&L_.fresh_q, sizeof(FrFresh) * (2 * queue_cap + max_slots + 1)));
     FR_CHECK(hipMalloc(&d_scratch_, sizeof(int) * 64));
+    L_.pair_cap = std::min<long>(std::max<long>(16 * queue_cap, 1L << 16), 1L << 26);
+    FR_CHECK(hipMalloc(&L_.pairs, sizeof(FrPair) * L_.pair_cap));
     FR_CHECK(hipMemsetAsync(L_.ctl, 0, sizeof(FrCtl) * max_slots, st_));
     FR_CHECK(hipMemsetAsync(L_.n_tiles, 0, sizeof(int), st_));
     FR_CHECK(hipMemsetAsync(L_.counters, 0, sizeof(int) * FR_CNT_N, st_));
     L_.grid_tiles = (int)std::min<long>(max_tiles, 2048);
     L_.grid_items = (int)std::max<long>(1, std::min<long>((queue_cap + FR_BLOCK - 1) / FR_BLOCK, 2048));
     L_.grid_slots = 1;
+    L_.grid_pairs = 2048;
     L_.max_slots = max_slots;
 }
 
@@ -268,6 +273,7 @@ FrEngine::~FrEngine()
     (void)hipFree(L_.exp_q);
     (void)hipFree(L_.fresh_q);
     (void)hipFree(d_scratch_);
+    (void)hipFree(L_.pairs);
     if (own_stream_)
         (void)hipStreamDestroy(st_);
 }
@@ -320,6 +326,7 @@ void FrEngine::bind(int slot, FrSlab &s, const FrPlanParams &p, const FrFrameVie
     d.gc_flag = s.gc_flag;
     d.gc_heading = s.gc_heading;
     d.gc_cost = s.gc_cost;
+    d.gc_rowsum = s.gc_rowsum;
     d.claim = s.claim;
     d.mark = s.mark;
     d.dist = s.dist;

@@ -16,7 +16,7 @@
 
 #include "fr_device.h"
 
-enum { FR_CNT_EXP = 0, FR_CNT_FRESH = 1, FR_CNT_N = 8 };
+enum { FR_CNT_EXP = 0, FR_CNT_FRESH = 1, FR_CNT_PAIRS = 2, FR_CNT_N = 8 };
 
 struct FrLaunch {
     FrPlanDev *plans;
@@ -27,8 +27,10 @@ struct FrLaunch {
     int *counters;
     FrExp *exp_q;
     FrFresh *fresh_q;
+    FrPair *pairs;
+    long pair_cap;
     int *n_slots;             /* device: entries of `slots` */
-    int grid_tiles, grid_items, grid_slots;
+    int grid_tiles, grid_items, grid_slots, grid_pairs;
     int max_slots;
 };
 
@@ -127,6 +129,7 @@ class FrSlab {
     float *heading = nullptr, *cost = nullptr, *intrinsic = nullptr, *direct = nullptr;
     uint8_t *gc_flag = nullptr;
     float *gc_heading = nullptr, *gc_cost = nullptr;
+    int32_t *gc_rowsum = nullptr;
     unsigned long long *claim = nullptr;
     uint8_t *mark = nullptr, *dist = nullptr, *dist_tmp = nullptr, *trav = nullptr;
     uint32_t *region = nullptr;

@@ -322,29 +322,45 @@ __global__ void k_setup_prep(const FrPlanDev *plans, const int *slots)
 }
 
 /* Row pass of the padded Chebyshev distance plane: horizontal distance to the
- * nearest obstacle pixel of the same row (capped). */
+ * nearest obstacle pixel of the same row (capped), one lane per pixel. */
 __global__ void k_setup_rowdist(const FrPlanDev *plans, const int *slots)
 {
     const FrPlanDev &P = plans[slots[blockIdx.y]];
-    const int zp = blockIdx.x * blockDim.x + threadIdx.x;
-    if (zp >= P.PH)
+    const long total = (long)P.PW * P.PH;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const int zp = (int)(i / P.PW), xp = (int)(i - (long)zp * P.PW);
+        const int z = zp - P.pad;
+        int d = FR_DIST_CAP;
+        if (z >= 0 && z < P.H) {
+            const uint8_t *m = P.mark + (long)z * P.W;
+            const int x = xp - P.pad;
+            for (int k = 0; k < FR_DIST_CAP; ++k) {
+                const int xa = x - k, xb = x + k;
+                if ((xa >= 0 && xa < P.W && m[xa]) || (xb >= 0 && xb < P.W && m[xb])) {
+                    d = k;
+                    break;
+                }
+            }
+        }
+        P.dist_tmp[i] = (uint8_t)d;
+    }
+}
+
+/* exclusive per-row prefix counts of the goal-flag plane */
+__global__ void k_setup_gc_rowsum(const FrPlanDev *plans, const int *slots)
+{
+    const FrPlanDev &P = plans[slots[blockIdx.y]];
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= P.H)
         return;
-    const int z = zp - P.pad;
-    uint8_t *row = P.dist_tmp + (long)zp * P.PW;
-    int d = FR_DIST_CAP;
-    for (int xp = 0; xp < P.PW; ++xp) {
-        const int x = xp - P.pad;
-        const bool ob = (z >= 0 && z < P.H && x >= 0 && x < P.W) && P.mark[z * P.W + x];
-        d = ob ? 0 : (d < FR_DIST_CAP ? d + 1 : FR_DIST_CAP);
-        row[xp] = (uint8_t)d;
+    int32_t *row = P.gc_rowsum + (long)z * (P.W + 1);
+    const uint8_t *f = P.gc_flag + (long)z * P.W;
+    int acc = 0;
+    for (int x = 0; x < P.W; ++x) {
+        row[x] = acc;
+        acc += f[x] ? 1 : 0;
     }
-    d = FR_DIST_CAP;
-    for (int xp = P.PW - 1; xp >= 0; --xp) {
-        const int cur = row[xp];
-        d = (cur == 0) ? 0 : (d < FR_DIST_CAP ? d + 1 : FR_DIST_CAP);
-        if (d < cur)
-            row[xp] = (uint8_t)d;
-    }
+    row[P.W] = acc;
 }
 
 /* Column pass: D(x, z) = min_dz max(|dz|, rowdist(x, z + dz)), capped. */
@@ -401,6 +417,7 @@ __global__ void k_begin(const FrPlanDev *plans, FrCtl *ctl, const int *slots, co
     if (i == 0) {
         counters[FR_CNT_EXP] = 0;
         counters[FR_CNT_FRESH] = 0;
+        counters[FR_CNT_PAIRS] = 0;
     }
     if (i >= n_slots)
         return;
@@ -420,7 +437,7 @@ __global__ void k_begin(const FrPlanDev *plans, FrCtl *ctl, const int *slots, co
     C.found = 0;
     C.n_expansions = 0;
     C.n_fresh = 0;
-    C.best_lcost = FR_BIG_COST;
+    C.best_lcost = (long long)~0ull; /* holds fr_lcost_key() values */
     C.best_node_key = ~0ull;
     C.epoch += 1;
     if (P.smart) {
@@ -693,11 +710,11 @@ __global__ void k_erase_apply(const FrPlanDev *plans, FrCtl *ctl, const int2 *ti
                 FrFresh f;
                 f.slot = tl.x;
                 f.pos = mine[k];
-                f.lcost = FR_BIG_COST;
-                f.widx = -1;
-                f.c_pad = 1; /* converted from COLLISION */
-                f.c_cost = 0;
-                f.total = 0;
+                f.flags = FR_FRESH_CONVERTED;
+                f.pair_base = 0;
+                f.pair_cnt = 0;
+                f.pad0 = 0;
+                f.key = ~0ull;
                 fq[w++] = f;
             }
         __syncthreads();
@@ -710,7 +727,7 @@ __global__ void k_coll_parents(const FrPlanDev *plans, const FrFresh *fq, const 
     const int n = counters[FR_CNT_FRESH];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const FrFresh f = fq[i];
-        if (f.c_pad != 1)
+        if (!(f.flags & FR_FRESH_CONVERTED))
             continue;
         const FrPlanDev &P = plans[f.slot];
         const long pp = (long)P.pz[f.pos] * P.W + P.px[f.pos];
@@ -774,69 +791,219 @@ __global__ void k_accept(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, 
                 FrFresh f;
                 f.slot = tl.x;
                 f.pos = mine[k];
-                f.lcost = FR_BIG_COST;
-                f.widx = -1;
-                f.c_pad = 0;
-                f.c_cost = 0;
-                f.total = 0;
+                f.flags = 0;
+                f.pair_base = 0;
+                f.pair_cnt = 0;
+                f.pad0 = 0;
+                f.key = ~0ull;
                 fq[w++] = f;
             }
         __syncthreads();
     }
 }
 
-/* __CUDA__check_goal_reached_with_direct_connection_cost
- * (graph_check_goal_reached.cu:17-70) for the queued nodes, plus the goal
- * test of __CUDA_KERNEL_checkGoalReached (graph_best_node.cu:182-222). */
-__global__ void k_connect(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, const int *counters)
+/* ---- goal connection of the newly accepted nodes --------------------------
+ * __CUDA__check_goal_reached_with_direct_connection_cost and
+ * __CUDA__check_goal_reached_with_direct_connection
+ * (graph_check_goal_reached.cu:17-139): every node is tried against every
+ * goal-flagged cell of its (2r)^2 window; the pair with the least
+ * floor(100 * total cost) wins.  Here each (node, flagged cell) pair is one
+ * lane.  A node's window is enumerated in the reference's scan order so that
+ * ties inside one node keep the last cell, as the reference's sequential
+ * per-thread scan does; ties across nodes keep the lowest cell index.
+ * Only nodes that became NODE this iteration are tried: every older node was
+ * tried when it appeared, and had it connected the search would have ended.
+ */
+
+#define FR_RANK_BITS 21
+#define FR_RANK_MASK ((1ull << FR_RANK_BITS) - 1)
+
+__device__ __forceinline__ unsigned long long dev_node_key(long long lc, int rank)
+{
+    const long long lim = (1LL << 42) - 1;
+    if (lc > lim)
+        lc = lim;
+    if (lc < -lim)
+        lc = -lim;
+    unsigned long long r = (unsigned long long)(rank < 0 ? 0 : rank);
+    if (r > FR_RANK_MASK)
+        r = FR_RANK_MASK;
+    return ((unsigned long long)(lc + (1LL << 42)) << FR_RANK_BITS) | (FR_RANK_MASK - r);
+}
+
+__device__ __forceinline__ long long dev_key_lcost(unsigned long long key)
+{
+    return (long long)(key >> FR_RANK_BITS) - (1LL << 42);
+}
+
+__device__ __forceinline__ int dev_key_rank(unsigned long long key) { return (int)(FR_RANK_MASK - (key & FR_RANK_MASK)); }
+
+/* window bounds of graph_check_goal_reached.cu:295-296 (float radius) */
+__device__ __forceinline__ void dev_window(const FrPlanDev &P, int x, int z, int *z0, int *z1, int *x0, int *x1)
+{
+    const float r = P.dist_tol;
+    int a = fr_trunc_f((float)z - r), b = a;
+    while ((float)b < (float)z + r)
+        b++;
+    int c = fr_trunc_f((float)x - r), d = c;
+    while ((float)d < (float)x + r)
+        d++;
+    *z0 = a < 0 ? 0 : a;
+    *z1 = b > P.H ? P.H : b;
+    *x0 = c < 0 ? 0 : c;
+    *x1 = d > P.W ? P.W : d;
+}
+
+__device__ __forceinline__ int dev_window_count(const FrPlanDev &P, int z0, int z1, int x0, int x1)
+{
+    if (z0 >= z1 || x0 >= x1)
+        return 0;
+    int n = 0;
+    for (int zp = z0; zp < z1; ++zp) {
+        const int32_t *row = P.gc_rowsum + (long)zp * (P.W + 1);
+        n += row[x1] - row[x0];
+    }
+    return n;
+}
+
+/* node -> flagged cell check; returns the lcost or FR_BIG_COST */
+__device__ __forceinline__ long long dev_pair_lcost(const FrPlanDev &P, int node, int cell, float *c_out)
+{
+    const int z = node / P.W, x = node - z * P.W;
+    const int zp = cell / P.W, xp = cell - zp * P.W;
+    const float c = dev_check_direct(P, P.max_curv_loop, x, z, P.heading[node], xp, zp, P.gc_heading[cell], true,
+                                     false);
+    if (c_out)
+        *c_out = c;
+    if (c < 0)
+        return FR_BIG_COST;
+    const float total = c + P.gc_cost[cell];
+    return (long long)floor((double)(100.0f * total));
+}
+
+/* Goal test of the new nodes (graph_best_node.cu:182-222), window counts,
+ * and the pair queue (block prefix sums + one atomic per block). */
+__global__ void k_connect_emit(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, FrPair *pairs, long pair_cap,
+                               int *counters)
+{
+    const int n = counters[FR_CNT_FRESH];
+    for (int base = blockIdx.x * FR_BLOCK; base < n; base += gridDim.x * FR_BLOCK) {
+        const int i = base + threadIdx.x;
+        int cnt = 0, z0 = 0, z1 = 0, x0 = 0, x1 = 0, x = 0, z = 0;
+        const FrPlanDev *Pp = nullptr;
+        if (i < n) {
+            FrFresh &f = fq[i];
+            const FrPlanDev &P = plans[f.slot];
+            FrCtl &C = ctl[f.slot];
+            Pp = &P;
+            f.key = ~0ull;
+            f.pair_cnt = 0;
+            if (P.type[f.pos] == FR_TYPE_NODE) {
+                z = f.pos / P.W;
+                x = f.pos - z * P.W;
+                atomicAdd(&C.n_fresh, 1);
+                if (dev_goal_hit(P, x, z, P.heading[f.pos]))
+                    C.goal_reached = 1;
+                dev_window(P, x, z, &z0, &z1, &x0, &x1);
+                cnt = dev_window_count(P, z0, z1, x0, x1);
+            }
+        }
+        int prefix;
+        const int total = block_exclusive_scan(cnt, &prefix);
+        __shared__ int s_base;
+        if (threadIdx.x == 0)
+            s_base = total ? atomicAdd(&counters[FR_CNT_PAIRS], total) : 0;
+        __syncthreads();
+        if (i < n && cnt > 0) {
+            FrFresh &f = fq[i];
+            const long o0 = (long)s_base + prefix;
+            if (o0 + cnt > pair_cap) {
+                f.flags |= FR_FRESH_INLINE;
+            } else {
+                const FrPlanDev &P = *Pp;
+                long o = o0;
+                for (int zp = z0; zp < z1; ++zp) {
+                    const int32_t *row = P.gc_rowsum + (long)zp * (P.W + 1);
+                    if (row[x1] == row[x0])
+                        continue;
+                    for (int xp = x0; xp < x1; ++xp) {
+                        const int q = zp * P.W + xp;
+                        if (P.gc_flag[q]) {
+                            FrPair pr;
+                            pr.fresh = i;
+                            pr.cell = q;
+                            pairs[o++] = pr;
+                        }
+                    }
+                }
+                f.pair_base = (int)o0;
+                f.pair_cnt = cnt;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_connect_eval(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, const FrPair *pairs, long pair_cap,
+                               const int *counters)
+{
+    long n = counters[FR_CNT_PAIRS];
+    if (n > pair_cap)
+        n = pair_cap;
+    const int n_fresh = counters[FR_CNT_FRESH];
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
+        const FrPair pr = pairs[p];
+        if (pr.fresh < 0 || pr.fresh >= n_fresh)
+            continue; /* slot reserved by a node whose pairs did not fit */
+        FrFresh &f = fq[pr.fresh];
+        if (f.flags & FR_FRESH_INLINE)
+            continue;
+        if ((long)f.pair_base + f.pair_cnt <= p || p < f.pair_base)
+            continue; /* a pair slot past the capacity check */
+        const FrPlanDev &P = plans[f.slot];
+        const long long lc = dev_pair_lcost(P, f.pos, pr.cell, nullptr);
+        if (lc >= FR_BIG_COST)
+            continue;
+        atomicMin(&f.key, dev_node_key(lc, (int)(p - f.pair_base)));
+        atomicMin((unsigned long long *)&ctl[f.slot].best_lcost, fr_lcost_key(lc));
+    }
+}
+
+/* nodes whose pairs did not fit the queue scan their window in-thread */
+__global__ void k_connect_inline(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, const int *counters)
 {
     const int n = counters[FR_CNT_FRESH];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         FrFresh &f = fq[i];
+        if (!(f.flags & FR_FRESH_INLINE))
+            continue;
         const FrPlanDev &P = plans[f.slot];
-        FrCtl &C = ctl[f.slot];
-        const int pos = f.pos;
-        if (P.type[pos] != FR_TYPE_NODE)
+        if (P.type[f.pos] != FR_TYPE_NODE)
             continue;
-        const int z = pos / P.W, x = pos - z * P.W;
-        const float heading = P.heading[pos];
-        atomicAdd(&C.n_fresh, 1);
-        if (dev_goal_hit(P, x, z, heading))
-            C.goal_reached = 1;
-        const float r = P.dist_tol;
-        long long best = FR_BIG_COST;
-        int bq = -1;
-        float bc = 0, btot = 0;
-        for (int zp = fr_trunc_f((float)z - r); (float)zp < (float)z + r; zp++) {
-            if (zp < 0 || zp >= P.H)
-                continue;
-            for (int xp = fr_trunc_f((float)x - r); (float)xp < (float)x + r; xp++) {
-                if (xp < 0 || xp >= P.W)
+        const int z = f.pos / P.W, x = f.pos - z * P.W;
+        int z0, z1, x0, x1;
+        dev_window(P, x, z, &z0, &z1, &x0, &x1);
+        int rank = 0;
+        unsigned long long best = ~0ull;
+        long long bl = FR_BIG_COST;
+        for (int zp = z0; zp < z1; ++zp)
+            for (int xp = x0; xp < x1; ++xp) {
+                const int q = zp * P.W + xp;
+                if (!P.gc_flag[q])
                     continue;
-                const int qq = zp * P.W + xp;
-                if (!P.gc_flag[qq])
-                    continue;
-                const float lih = P.gc_heading[qq];
-                const float c = dev_check_direct(P, P.max_curv_loop, x, z, heading, xp, zp, lih, true, false);
-                if (c < 0)
-                    continue;
-                const float total = c + P.gc_cost[qq];
-                const long long lc = (long long)floor((double)(100.0f * total));
-                if (lc <= best) {
-                    best = lc;
-                    bq = qq;
-                    bc = c;
-                    btot = total;
+                const long long lc = dev_pair_lcost(P, f.pos, q, nullptr);
+                if (lc < FR_BIG_COST) {
+                    const unsigned long long k = dev_node_key(lc, rank);
+                    if (k < best) {
+                        best = k;
+                        bl = lc;
+                    }
                 }
+                rank++;
             }
-        }
-        if (bq < 0)
-            continue;
-        f.lcost = best;
-        f.widx = bq;
-        f.c_cost = bc;
-        f.total = btot;
-        atomicMin((unsigned long long *)&C.best_lcost, fr_lcost_key(best));
+        f.key = best;
+        if (bl < FR_BIG_COST)
+            atomicMin((unsigned long long *)&ctl[f.slot].best_lcost, fr_lcost_key(bl));
     }
 }
 
@@ -845,13 +1012,38 @@ __global__ void k_connect_pick(const FrPlanDev *plans, FrCtl *ctl, const FrFresh
     const int n = counters[FR_CNT_FRESH];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const FrFresh f = fq[i];
-        if (f.widx < 0)
+        if (f.key == ~0ull)
             continue;
         FrCtl &C = ctl[f.slot];
-        if (fr_lcost_key(f.lcost) != (unsigned long long)C.best_lcost)
+        if (fr_lcost_key(dev_key_lcost(f.key)) != (unsigned long long)C.best_lcost)
             continue;
         atomicMin(&C.best_node_key, ((unsigned long long)(unsigned)f.pos << 32) | (unsigned)i);
     }
+}
+
+/* the rank-th goal-flagged cell of a node's window, in scan order */
+__device__ int dev_window_cell(const FrPlanDev &P, int node, int rank)
+{
+    const int z = node / P.W, x = node - z * P.W;
+    int z0, z1, x0, x1;
+    dev_window(P, x, z, &z0, &z1, &x0, &x1);
+    for (int zp = z0; zp < z1; ++zp) {
+        const int32_t *row = P.gc_rowsum + (long)zp * (P.W + 1);
+        const int c = row[x1] - row[x0];
+        if (rank >= c) {
+            rank -= c;
+            continue;
+        }
+        for (int xp = x0; xp < x1; ++xp) {
+            const int q = zp * P.W + xp;
+            if (P.gc_flag[q]) {
+                if (rank == 0)
+                    return q;
+                rank--;
+            }
+        }
+    }
+    return -1;
 }
 
 /* End of FastRRT::loop (fastrrt.cpp:130-180): reseed, goal-connection adds,
@@ -881,10 +1073,13 @@ __global__ void k_finish(const FrPlanDev *plans, FrCtl *ctl, const int *slots, c
         if (C.best_node_key != ~0ull) {
             const FrFresh f = fq[(unsigned)(C.best_node_key & 0xFFFFFFFFu)];
             const int nz = f.pos / P.W, nx = f.pos - nz * P.W;
-            const int cz = f.widx / P.W, cx = f.widx - cz * P.W;
+            const int cell = dev_window_cell(P, f.pos, dev_key_rank(f.key));
+            float c_cost = 0;
+            (void)dev_pair_lcost(P, f.pos, cell, &c_cost);
+            const int cz = cell / P.W, cx = cell - cz * P.W;
             const float base = P.cost[f.pos];
-            const float ch = P.gc_heading[f.widx];
-            dev_add(P, cx, cz, ch, nx, nz, f.c_cost + base);
+            const float ch = P.gc_heading[cell];
+            dev_add(P, cx, cz, ch, nx, nz, c_cost + base);
             dev_add(P, P.gx, P.gz, P.gh, cx, cz, ch + base);
             C.found = 1;
             if (dev_goal_hit(P, cx, cz, ch))
@@ -1220,7 +1415,7 @@ void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, in
         return;
     dim3 b(FR_BLOCK);
     hipLaunchKernelGGL(k_setup_prep, dim3(fr_grid_for(max_N, FR_BLOCK, 1024), n), b, 0, st, L.plans, d_slots);
-    hipLaunchKernelGGL(k_setup_rowdist, dim3(fr_grid_for(max_PH, FR_BLOCK, 64), n), b, 0, st, L.plans, d_slots);
+    hipLaunchKernelGGL(k_setup_rowdist, dim3(fr_grid_for(max_PWH, FR_BLOCK, 1024), n), b, 0, st, L.plans, d_slots);
     hipLaunchKernelGGL(k_setup_coldist, dim3(fr_grid_for(max_PWH, FR_BLOCK, 1024), n), b, 0, st, L.plans,
                        d_slots);
     if (goal_pass) {
@@ -1228,6 +1423,8 @@ void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, in
         hipLaunchKernelGGL(k_setup_goal_connection, dim3(fr_grid_for(max_N, FR_BLOCK, 4096), n), b, 0, st, L.plans,
                            d_slots);
         fr_prof_end(FR_PROF_GOAL_CONNECTION, st);
+        hipLaunchKernelGGL(k_setup_gc_rowsum, dim3(fr_grid_for(max_PH, 64, 64), n), dim3(64), 0, st, L.plans,
+                           d_slots);
     }
 }
 
@@ -1253,7 +1450,11 @@ void fr_launch_iteration(const FrLaunch &L, hipStream_t st)
     hipLaunchKernelGGL(k_accept, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles, L.fresh_q,
                        L.counters);
     fr_prof_begin(FR_PROF_CONNECT, st);
-    hipLaunchKernelGGL(k_connect, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);
+    hipLaunchKernelGGL(k_connect_emit, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.pairs, L.pair_cap,
+                       L.counters);
+    hipLaunchKernelGGL(k_connect_eval, dim3(L.grid_pairs), b, 0, st, L.plans, L.ctl, L.fresh_q, L.pairs, L.pair_cap,
+                       L.counters);
+    hipLaunchKernelGGL(k_connect_inline, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);
     fr_prof_end(FR_PROF_CONNECT, st);
     hipLaunchKernelGGL(k_connect_pick, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);
     hipLaunchKernelGGL(k_finish, dim3(fr_grid_for(n_slots, FR_BLOCK, 65535)), b, 0, st, L.plans, L.ctl, L.slots,
