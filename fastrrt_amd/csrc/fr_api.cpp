@@ -795,7 +795,7 @@ class FrFastRRT {
         g.eng.read_ctl(0, &c);
         if (c.path_len < 3)
             return false;
-        fr_launch_opt_round(g.eng.launch(), g.eng.d_active(), 1, c.path_len, g.st());
+        fr_launch_opt_round(g.eng.launch(), g.eng.d_active(), 1, c.path_len, g.prm.W * g.prm.H, g.st());
         g.eng.read_ctl(0, &c);
         if (c.opt_changed) {
             g.goal_valid = false;

@@ -333,7 +333,7 @@ class FrBatch {
             fr_launch_goal_scan_multi(L, d_list_, n, max_N, FR_GATE_OPT, st);
             fr_launch_best_node(L, d_list_, n, max_N, nullptr, st, FR_GATE_OPT);
             fr_launch_walk(L, d_list_, n, st, FR_GATE_OPT);
-            fr_launch_opt_round(L, d_list_, n, max_path_len_, st, FR_GATE_OPT);
+            fr_launch_opt_round(L, d_list_, n, max_path_len_, max_N, st, FR_GATE_OPT);
             if ((r & 3) == 3) {
                 /* stop early when every plan has converged */
                 FR_CHECK(hipMemcpyAsync(h_ctl_, L.ctl, sizeof(FrCtl) * slots_.size(), hipMemcpyDeviceToHost, st));

@@ -21,6 +21,7 @@
 #define FR_TILE 1024          /* cells per tile of the flattened tile table */
 #define FR_BLOCK 256          /* threads per block of the tile kernels */
 #define FR_DIST_CAP 63        /* cap of the Chebyshev obstacle-distance plane */
+#define FR_CHK_STRIDE 64      /* deferred footprint tests kept per rollout */
 #define FR_BIG_COST 99999999999LL
 
 enum FrStatus {
@@ -113,7 +114,7 @@ struct FrExp {
     int end;                  /* -1: rejected */
     float heading;
     float cost;
-    int pad0;
+    int n_chk;                /* footprint tests deferred to the wave pass */
 };
 
 /* One node to goal-check / connect this iteration. */

@@ -249,13 +249,15 @@ FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
     FR_CHECK(hipMalloc(&d_scratch_, sizeof(int) * 64));
     L_.pair_cap = std::min<long>(std::max<long>(16 * queue_cap, 1L << 16), 1L << 26);
     FR_CHECK(hipMalloc(&L_.pairs, sizeof(FrPair) * L_.pair_cap));
+    L_.chk_cap = std::min<long>((queue_cap + 1) * FR_CHK_STRIDE, 1L << 24);
+    FR_CHECK(hipMalloc(&L_.chk, sizeof(int2) * L_.chk_cap));
     FR_CHECK(hipMemsetAsync(L_.ctl, 0, sizeof(FrCtl) * max_slots, st_));
     FR_CHECK(hipMemsetAsync(L_.n_tiles, 0, sizeof(int), st_));
     FR_CHECK(hipMemsetAsync(L_.counters, 0, sizeof(int) * FR_CNT_N, st_));
     L_.grid_tiles = (int)std::min<long>(max_tiles, 2048);
     L_.grid_items = (int)std::max<long>(1, std::min<long>((queue_cap + FR_BLOCK - 1) / FR_BLOCK, 2048));
     L_.grid_slots = 1;
-    L_.grid_pairs = 2048;
+    L_.grid_waves = 4096; /* blocks of 4 one-item waves */
     L_.max_slots = max_slots;
 }
 
@@ -274,6 +276,7 @@ FrEngine::~FrEngine()
     (void)hipFree(L_.fresh_q);
     (void)hipFree(d_scratch_);
     (void)hipFree(L_.pairs);
+    (void)hipFree(L_.chk);
     if (own_stream_)
         (void)hipStreamDestroy(st_);
 }

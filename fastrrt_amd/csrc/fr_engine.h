@@ -29,8 +29,10 @@ struct FrLaunch {
     FrFresh *fresh_q;
     FrPair *pairs;
     long pair_cap;
+    int2 *chk;                /* per rollout: (cell, heading bits) footprint tests, FR_CHK_STRIDE each */
+    long chk_cap;
     int *n_slots;             /* device: entries of `slots` */
-    int grid_tiles, grid_items, grid_slots, grid_pairs;
+    int grid_tiles, grid_items, grid_slots, grid_waves;
     int max_slots;
 };
 
@@ -53,7 +55,7 @@ enum { FR_GATE_NONE = 0, FR_GATE_OPT = 1, FR_GATE_FOUND = 2 };
 void fr_launch_best_node(const FrLaunch &L, const int *d_slots, int n, int max_N, const FrBestNodeArgs *ov,
                          hipStream_t st, int gate = FR_GATE_NONE);
 void fr_launch_walk(const FrLaunch &L, const int *d_slots, int n, hipStream_t st, int gate = FR_GATE_NONE);
-void fr_launch_opt_round(const FrLaunch &L, const int *d_slots, int n, int max_path, hipStream_t st,
+void fr_launch_opt_round(const FrLaunch &L, const int *d_slots, int n, int max_path, int max_N, hipStream_t st,
                          int gate = FR_GATE_NONE);
 void fr_launch_goal_scan_multi(const FrLaunch &L, const int *d_slots, int n, int max_N, int gate, hipStream_t st);
 void fr_launch_batch_start(const FrLaunch &L, const int *d_slots, int n, hipStream_t st);

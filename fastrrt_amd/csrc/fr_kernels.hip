@@ -80,9 +80,19 @@ __device__ bool dev_feasible(const FrPlanDev &P, int x, int z, float h)
     return true;
 }
 
+/* The centre pixel's obstacle distance certifies the whole footprint: every
+ * sample pixel lies within Chebyshev distance pad - 1 = ceil(hypot(mx, mz)) + 1
+ * of the centre (rotation keeps |offset| <= hypot, truncation adds < 1). */
+__device__ __forceinline__ bool dev_fp_certified(const FrPlanDev &P, int x, int z, float h)
+{
+    if (P.trav[z * P.W + x] & 1)
+        return true;
+    return (h - h) == 0.0f && P.dist[(z + P.pad) * P.PW + x + P.pad] >= P.pad;
+}
+
 /* kinematics.cu:123-231.  Returns 0 when rejected. */
 __device__ int dev_rollout(const FrPlanDev &P, int sx, int sz, float steer, float path_size, int *ex, int *ez,
-                           float *eh, float *ec)
+                           float *eh, float *ec, int2 *defer = nullptr, int *n_defer = nullptr)
 {
     const int W = P.W, H = P.H;
     float x = P.og_y - (float)sz;
@@ -123,8 +133,14 @@ __device__ int dev_rollout(const FrPlanDev &P, int sx, int sz, float steer, floa
             break;
         size += 1;
         node_cost += (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
-        if (!dev_feasible(P, cx, cz, heading))
-            return 0;
+        /* Feasibility only rejects the rollout, so a test may run later (in
+         * the wave pass) without changing the result. */
+        if (!dev_fp_certified(P, cx, cz, heading)) {
+            if (defer && *n_defer < FR_CHK_STRIDE)
+                defer[(*n_defer)++] = make_int2(cz * W + cx, __float_as_int(heading));
+            else if (!dev_feasible(P, cx, cz, heading))
+                return 0;
+        }
         last_x = cx;
         last_z = cz;
     }
@@ -218,6 +234,217 @@ __device__ float dev_check_direct(const FrPlanDev &P, float max_curv, int x, int
                 const float heading = (float)((double)fr_atan2f((float)ddz, (float)ddx) + FR_HALF_PI_D);
                 if (!dev_feasible(P, cx, cz, heading))
                     return -1;
+            }
+        }
+    }
+    if (last_x != gx && last_z != gz)
+        return -1;
+    return node_cost;
+}
+
+/* ------------------------------------------------------------------------ */
+/* wave-cooperative forms: every lane of a wave calls with the same          */
+/* arguments and gets the same result; loads are spread over the 64 lanes    */
+/* instead of walked serially by one lane.                                   */
+/* ------------------------------------------------------------------------ */
+
+__device__ __forceinline__ int dev_lane() { return (int)(threadIdx.x & 63); }
+
+__device__ __forceinline__ int dev_bcast_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+__device__ __forceinline__ float dev_bcast_f(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+/* path_feasible.cu:13-78 with the 64 lanes sharing the (2 mx + 1) x (2 mz + 1)
+ * samples; the sample coordinates are the reference's expressions. */
+__device__ bool dev_wave_feasible(const FrPlanDev &P, int x, int z, float h)
+{
+    if (dev_fp_certified(P, x, z, h))
+        return true;
+    float s, c;
+    fr_sincosf(h, &s, &c);
+    const uint8_t *D = P.dist;
+    const int PW = P.PW, pad = P.pad;
+    if (c != c || s != s)
+        return D[pad * PW + pad] != 0;
+    const int mx = P.mx, mz = P.mz;
+    const int nx = 2 * mx + 1, ns = nx * (2 * mz + 1);
+    const int lane = dev_lane();
+    int i = lane / nx, j = lane - (lane / nx) * nx;
+    const int di = 64 / nx, dj = 64 - di * nx;
+    const float fxc = (float)x, fzc = (float)z;
+    for (int k0 = 0; k0 < ns; k0 += 8 * 64) {
+        bool bad = false;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            if (k0 + r * 64 + lane < ns) {
+                const float is = (float)(i - mz) * s;
+                const float ic = (float)(i - mz) * c;
+                const float fj = (float)(j - mx);
+                const float fx = fj * c - is + fxc;
+                const float fz = fj * s + ic + fzc;
+                const int xl = (int)fx;
+                const int zl = (int)fz;
+                bad |= D[(zl + pad) * PW + (xl + pad)] == 0;
+            }
+            i += di;
+            j += dj;
+            if (j >= nx) {
+                j -= nx;
+                i += 1;
+            }
+        }
+        if (__ballot(bad))
+            return false;
+    }
+    return true;
+}
+
+/* One Hermite point of graph_direct_goal_connection.cu:16-167: position,
+ * in-bounds cell, first derivative. */
+struct FrHermite {
+    double dx0, dz0, dgx, dgz, t1x, t1y, t2x, t2y, denom;
+};
+
+__device__ __forceinline__ bool dev_hermite_point(const FrPlanDev &P, const FrHermite &hm, int i, bool host_nan,
+                                                  int *cx, int *cz, double *t_out)
+{
+    const double t = (0.0 + (double)i) / hm.denom;
+    const double t2 = t * t, t3 = t2 * t;
+    const double h00 = 2 * t3 - 3 * t2 + 1;
+    const double h10 = t3 - 2 * t2 + t;
+    const double h01 = -2 * t3 + 3 * t2;
+    const double h11 = t3 - t2;
+    const double px = h00 * hm.dx0 + h10 * hm.t1x + h01 * hm.dgx + h11 * hm.t2x;
+    const double pz = h00 * hm.dz0 + h10 * hm.t1y + h01 * hm.dgz + h11 * hm.t2y;
+    *t_out = t;
+    if (px < 0 || px >= P.W)
+        return false;
+    if (pz < 0 || pz >= P.H)
+        return false;
+    const int x = host_nan ? fr_host_trunc_d(px) : fr_trunc_d(px);
+    const int z = host_nan ? fr_host_trunc_d(pz) : fr_trunc_d(pz);
+    *cx = x;
+    *cz = z;
+    return x >= 0 && x < P.W && z >= 0 && z < P.H;
+}
+
+/* dev_check_direct with the curve's points spread over the lanes.  A point
+ * contributes iff its cell is in the grid and differs from the cell of the
+ * previous such point (the reference's last_x / last_z test: duplicates of
+ * the last cell never change it, so "last accepted" = "last in-grid").  The
+ * cost is summed lane by lane in point order, as the reference adds it. */
+__device__ float dev_wave_check_direct(const FrPlanDev &P, float max_curv, int x, int z, float lh, int gx, int gz,
+                                       float gh, bool safe_zone, bool host_nan)
+{
+    const int W = P.W;
+    const int lane = dev_lane();
+    const int pos = z * W + x;
+    const int dxi = gx - x, dzi = gz - z;
+    const float distance = sqrtf((float)(dxi * dxi + dzi * dzi));
+    const int num_points = fr_trunc_f(distance);
+    if (num_points <= 0)
+        return -1;
+    const float a1 = (float)((double)lh - FR_HALF_PI_D);
+    const float a2 = (float)((double)gh - FR_HALF_PI_D);
+    float s1, c1, s2, c2;
+    fr_sincosf(a1, &s1, &c1);
+    fr_sincosf(a2, &s2, &c2);
+    FrHermite hm;
+    hm.t1x = (double)(distance * c1);
+    hm.t1y = (double)(distance * s1);
+    hm.t2x = (double)(distance * c2);
+    hm.t2y = (double)(distance * s2);
+    hm.dx0 = (double)x;
+    hm.dz0 = (double)z;
+    hm.dgx = (double)gx;
+    hm.dgz = (double)gz;
+    hm.denom = (double)(num_points - 1);
+    const bool skip_fp = safe_zone && ((P.trav[pos] & 2) != 0);
+    float node_cost = P.cost[pos];
+    int last_x = -1, last_z = -1;
+    for (int pass = 0; pass < 2; ++pass) {
+        last_x = -1;
+        last_z = -1;
+        for (int b = 0; b < num_points; b += 64) {
+            const int i = b + lane;
+            int cx = 0, cz = 0;
+            double t = 0;
+            const bool cand = i < num_points && dev_hermite_point(P, hm, i, host_nan, &cx, &cz, &t);
+            const unsigned long long m = __ballot(cand);
+            const unsigned long long lower = m & ((1ull << lane) - 1ull);
+            const int src = lower ? 63 - __clzll((long long)lower) : lane;
+            int pcx = __shfl(cx, src, 64), pcz = __shfl(cz, src, 64);
+            if (!lower) {
+                pcx = last_x;
+                pcz = last_z;
+            }
+            const bool acc = cand && !(cx == pcx && cz == pcz);
+            if (m) {
+                const int hi = 63 - __clzll((long long)m);
+                last_x = dev_bcast_i(cx, hi);
+                last_z = dev_bcast_i(cz, hi);
+            }
+            const unsigned long long am = __ballot(acc);
+            if (!am)
+                continue;
+            double ddx = 0, ddz = 0;
+            if (acc) {
+                const double t2 = t * t;
+                const double t00 = 6 * t2 - 6 * t;
+                const double t10 = 3 * t2 - 4 * t + 1;
+                const double t01 = -6 * t2 + 6 * t;
+                const double t11 = 3 * t2 - 2 * t;
+                ddx = t00 * hm.dx0 + t10 * hm.t1x + t01 * hm.dgx + t11 * hm.t2x;
+                ddz = t00 * hm.dz0 + t10 * hm.t1y + t01 * hm.dgz + t11 * hm.t2y;
+            }
+            if (pass == 0) {
+                bool bad = false;
+                if (acc) {
+                    if (max_curv > 0) {
+                        const double d00 = 12 * t - 6;
+                        const double d10 = 6 * t - 4;
+                        const double d01 = -12 * t + 6;
+                        const double d11 = 6 * t - 2;
+                        const double dd2x = d00 * hm.dx0 + d10 * hm.t1x + d01 * hm.dgx + d11 * hm.t2x;
+                        const double dd2z = d00 * hm.dz0 + d10 * hm.t1y + d01 * hm.dgz + d11 * hm.t2y;
+                        const double v = ddx * ddx + ddz * ddz;
+                        const double cr = ddx * dd2z - ddz * dd2x;
+                        const float k = (float)((cr < 0 ? -cr : cr) / (v * __builtin_sqrt(v)));
+                        if (k > max_curv)
+                            bad = true;
+                    }
+                    if (!skip_fp && !(P.trav[cz * W + cx] & 1) &&
+                        P.dist[(cz + P.pad) * P.PW + cx + P.pad] == 0)
+                        bad = true;
+                }
+                if (__ballot(bad))
+                    return -1;
+            } else {
+                float add = 0, hd = 0;
+                bool need = false;
+                if (acc) {
+                    add = (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
+                    if (!skip_fp) {
+                        hd = (float)((double)fr_atan2f((float)ddz, (float)ddx) + FR_HALF_PI_D);
+                        need = !dev_fp_certified(P, cx, cz, hd);
+                    }
+                }
+                unsigned long long nm = __ballot(need);
+                while (nm) {
+                    const int l = __ffsll((long long)nm) - 1;
+                    nm &= nm - 1;
+                    if (!dev_wave_feasible(P, dev_bcast_i(cx, l), dev_bcast_i(cz, l), dev_bcast_f(hd, l)))
+                        return -1;
+                }
+                unsigned long long cm = am;
+                while (cm) {
+                    const int l = __ffsll((long long)cm) - 1;
+                    cm &= cm - 1;
+                    node_cost += dev_bcast_f(add, l);
+                }
             }
         }
     }
@@ -346,21 +573,27 @@ __global__ void k_setup_rowdist(const FrPlanDev *plans, const int *slots)
     }
 }
 
-/* exclusive per-row prefix counts of the goal-flag plane */
+/* exclusive per-row prefix counts of the goal-flag plane; one wave per row */
 __global__ void k_setup_gc_rowsum(const FrPlanDev *plans, const int *slots)
 {
     const FrPlanDev &P = plans[slots[blockIdx.y]];
-    const int z = blockIdx.x * blockDim.x + threadIdx.x;
-    if (z >= P.H)
-        return;
-    int32_t *row = P.gc_rowsum + (long)z * (P.W + 1);
-    const uint8_t *f = P.gc_flag + (long)z * P.W;
-    int acc = 0;
-    for (int x = 0; x < P.W; ++x) {
-        row[x] = acc;
-        acc += f[x] ? 1 : 0;
+    const int wpb = blockDim.x >> 6;
+    const int lane = dev_lane();
+    for (int z = blockIdx.x * wpb + (int)(threadIdx.x >> 6); z < P.H; z += gridDim.x * wpb) {
+        int32_t *row = P.gc_rowsum + (long)z * (P.W + 1);
+        const uint8_t *f = P.gc_flag + (long)z * P.W;
+        int acc = 0;
+        for (int b = 0; b < P.W; b += 64) {
+            const int x = b + lane;
+            const bool on = x < P.W && f[x] != 0;
+            const unsigned long long m = __ballot(on);
+            if (x < P.W)
+                row[x] = acc + __popcll(m & ((1ull << lane) - 1ull));
+            acc += __popcll(m);
+        }
+        if (lane == 0)
+            row[P.W] = acc;
     }
-    row[P.W] = acc;
 }
 
 /* Column pass: D(x, z) = min_dz max(|dz|, rowdist(x, z + dz)), capped. */
@@ -389,20 +622,24 @@ __global__ void k_setup_coldist(const FrPlanDev *plans, const int *slots)
     }
 }
 
-/* graph_direct_goal_connection.cu:169-198: per cell, Hermite to the goal. */
+/* graph_direct_goal_connection.cu:169-198: per cell, Hermite to the goal.
+ * One wave per cell, the curve's points over the lanes. */
 __global__ void k_setup_goal_connection(const FrPlanDev *plans, const int *slots)
 {
     const FrPlanDev &P = plans[slots[blockIdx.y]];
-    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    for (int pos = blockIdx.x * wpb + (int)(threadIdx.x >> 6); pos < P.N; pos += nw) {
         const int z = pos / P.W, x = pos - z * P.W;
-        P.gc_flag[pos] = 0;
         const float lh = (float)dev_compute_heading(x, z, P.gx, P.gz);
-        const float c = dev_check_direct(P, P.max_curv_goal, x, z, lh, P.gx, P.gz, P.gh, false, false);
-        if (c < 0)
-            continue;
-        P.gc_flag[pos] = 1;
-        P.gc_heading[pos] = lh;
-        P.gc_cost[pos] = c;
+        const float c = dev_wave_check_direct(P, P.max_curv_goal, x, z, lh, P.gx, P.gz, P.gh, false, false);
+        if (dev_lane() == 0) {
+            P.gc_flag[pos] = c < 0 ? 0 : 1;
+            if (c >= 0) {
+                P.gc_heading[pos] = lh;
+                P.gc_cost[pos] = c;
+            }
+        }
     }
 }
 
@@ -513,20 +750,26 @@ __global__ void k_select(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, 
                 e.end = -1;
                 e.heading = 0;
                 e.cost = 0;
-                e.pad0 = 0;
+                e.n_chk = 0;
                 q[w++] = e;
             }
         __syncthreads();
     }
 }
 
-/* expand_node (graph_expansion.cu:103-145) split in two: rollout + claim here,
- * the commit in k_commit, so every rollout reads the pre-expansion grid. */
-__global__ void k_expand(const FrPlanDev *plans, FrCtl *ctl, FrExp *q, const int *counters)
+/* expand_node (graph_expansion.cu:103-145) split in three so every rollout
+ * reads the pre-expansion grid: k_expand runs the kinematics (one lane per
+ * rollout) and defers the footprint tests that the centre-distance
+ * certificate cannot settle; k_footprint runs them one wave per rollout and
+ * claims the end cell; k_commit writes the winners. */
+__global__ void k_expand(const FrPlanDev *plans, FrCtl *ctl, FrExp *q, int2 *chk, long chk_cap,
+                         const int *counters)
 {
     const int n = counters[FR_CNT_EXP];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         FrExp &e = q[i];
+        e.end = -1;
+        e.n_chk = 0;
         const FrPlanDev &P = plans[e.slot];
         const FrCtl &C = ctl[e.slot];
         const int src = e.src;
@@ -537,9 +780,10 @@ __global__ void k_expand(const FrPlanDev *plans, FrCtl *ctl, FrExp *q, const int
         float psize = P.max_path * ul;
         if (psize < 5.0f)
             psize = 5.0f;
-        int ex, ez;
+        int ex, ez, nd = 0;
         float eh, ec;
-        if (!dev_rollout(P, x, z, steer, psize, &ex, &ez, &eh, &ec))
+        int2 *defer = ((long)i + 1) * FR_CHK_STRIDE <= chk_cap ? chk + (long)i * FR_CHK_STRIDE : nullptr;
+        if (!dev_rollout(P, x, z, steer, psize, &ex, &ez, &eh, &ec, defer, &nd))
             continue;
         if ((float)ex == P.og_x && (float)ez == P.og_y)
             continue;
@@ -552,8 +796,39 @@ __global__ void k_expand(const FrPlanDev *plans, FrCtl *ctl, FrExp *q, const int
         e.end = epos;
         e.heading = eh;
         e.cost = ec;
-        const unsigned long long key = ((unsigned long long)(0xFFFFFFFFu - C.epoch) << 32) | (unsigned)src;
-        atomicMin(&P.claim[epos], key);
+        e.n_chk = nd;
+    }
+}
+
+__global__ void k_footprint(const FrPlanDev *plans, const FrCtl *ctl, FrExp *q, const int2 *chk,
+                            const int *counters)
+{
+    const int n = counters[FR_CNT_EXP];
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    for (int i = blockIdx.x * wpb + (int)(threadIdx.x >> 6); i < n; i += nw) {
+        const FrExp e = q[i];
+        if (e.end < 0)
+            continue;
+        const FrPlanDev &P = plans[e.slot];
+        bool ok = true;
+        for (int k = 0; k < e.n_chk; ++k) {
+            const int2 c = chk[(long)i * FR_CHK_STRIDE + k];
+            const int cz = c.x / P.W, cx = c.x - cz * P.W;
+            if (!dev_wave_feasible(P, cx, cz, __int_as_float(c.y))) {
+                ok = false;
+                break;
+            }
+        }
+        if (dev_lane() == 0) {
+            if (!ok) {
+                q[i].end = -1;
+            } else {
+                const unsigned long long key =
+                    ((unsigned long long)(0xFFFFFFFFu - ctl[e.slot].epoch) << 32) | (unsigned)e.src;
+                atomicMin(&P.claim[e.end], key);
+            }
+        }
     }
 }
 
@@ -866,13 +1141,17 @@ __device__ __forceinline__ int dev_window_count(const FrPlanDev &P, int z0, int 
     return n;
 }
 
-/* node -> flagged cell check; returns the lcost or FR_BIG_COST */
-__device__ __forceinline__ long long dev_pair_lcost(const FrPlanDev &P, int node, int cell, float *c_out)
+/* node -> flagged cell check; returns the lcost or FR_BIG_COST.  `wave`:
+ * every lane of the wave calls with the same node and cell. */
+__device__ __forceinline__ long long dev_pair_lcost(const FrPlanDev &P, int node, int cell, float *c_out,
+                                                    bool wave = false)
 {
     const int z = node / P.W, x = node - z * P.W;
     const int zp = cell / P.W, xp = cell - zp * P.W;
-    const float c = dev_check_direct(P, P.max_curv_loop, x, z, P.heading[node], xp, zp, P.gc_heading[cell], true,
-                                     false);
+    const float c = wave ? dev_wave_check_direct(P, P.max_curv_loop, x, z, P.heading[node], xp, zp,
+                                                 P.gc_heading[cell], true, false)
+                         : dev_check_direct(P, P.max_curv_loop, x, z, P.heading[node], xp, zp, P.gc_heading[cell],
+                                            true, false);
     if (c_out)
         *c_out = c;
     if (c < 0)
@@ -944,6 +1223,7 @@ __global__ void k_connect_emit(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, 
     }
 }
 
+/* one wave per (node, flagged cell) pair */
 __global__ void k_connect_eval(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, const FrPair *pairs, long pair_cap,
                                const int *counters)
 {
@@ -951,7 +1231,9 @@ __global__ void k_connect_eval(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, 
     if (n > pair_cap)
         n = pair_cap;
     const int n_fresh = counters[FR_CNT_FRESH];
-    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
+    const int wpb = blockDim.x >> 6;
+    const long nw = (long)gridDim.x * wpb;
+    for (long p = blockIdx.x * (long)wpb + (threadIdx.x >> 6); p < n; p += nw) {
         const FrPair pr = pairs[p];
         if (pr.fresh < 0 || pr.fresh >= n_fresh)
             continue; /* slot reserved by a node whose pairs did not fit */
@@ -961,8 +1243,8 @@ __global__ void k_connect_eval(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, 
         if ((long)f.pair_base + f.pair_cnt <= p || p < f.pair_base)
             continue; /* a pair slot past the capacity check */
         const FrPlanDev &P = plans[f.slot];
-        const long long lc = dev_pair_lcost(P, f.pos, pr.cell, nullptr);
-        if (lc >= FR_BIG_COST)
+        const long long lc = dev_pair_lcost(P, f.pos, pr.cell, nullptr, true);
+        if (lc >= FR_BIG_COST || dev_lane() != 0)
             continue;
         atomicMin(&f.key, dev_node_key(lc, (int)(p - f.pair_base)));
         atomicMin((unsigned long long *)&ctl[f.slot].best_lcost, fr_lcost_key(lc));
@@ -1047,11 +1329,12 @@ __device__ int dev_window_cell(const FrPlanDev &P, int node, int rank)
 }
 
 /* End of FastRRT::loop (fastrrt.cpp:130-180): reseed, goal-connection adds,
- * goal test, iteration bookkeeping. */
+ * goal test, iteration bookkeeping.  One wave per plan: the lanes share the
+ * winner's Hermite check, lane 0 writes. */
 __global__ void k_finish(const FrPlanDev *plans, FrCtl *ctl, const int *slots, const int *n_slots_p,
                          const FrFresh *fq)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
     const int n_slots = *n_slots_p;
     if (i >= n_slots)
         return;
@@ -1060,9 +1343,26 @@ __global__ void k_finish(const FrPlanDev *plans, FrCtl *ctl, const int *slots, c
     const FrPlanDev &P = plans[s];
     if (C.status != FR_SEARCHING)
         return;
+    const bool lead = dev_lane() == 0;
+    const bool reseed = C.n_temp == 0 && C.n_all == 0;
+    const unsigned long long bk = C.best_node_key;
+    int nx = 0, nz = 0, cell = -1, npos = 0;
+    float c_cost = 0;
+    unsigned long long fkey = 0;
+    if (!reseed && bk != ~0ull) {
+        const FrFresh f = fq[(unsigned)(bk & 0xFFFFFFFFu)];
+        npos = f.pos;
+        fkey = f.key;
+        nz = f.pos / P.W;
+        nx = f.pos - nz * P.W;
+        cell = dev_window_cell(P, f.pos, dev_key_rank(fkey));
+        (void)dev_pair_lcost(P, f.pos, cell, &c_cost, true);
+    }
+    if (!lead)
+        return;
     C.last_temp = C.n_temp;
     C.iter += 1;
-    if (C.n_temp == 0 && C.n_all == 0) {
+    if (reseed) {
         dev_add(P, P.sx, P.sz, P.sh, -1, -1, 0.0f);
         C.reseeded = 1;
         C.pending_start = 1;
@@ -1070,14 +1370,9 @@ __global__ void k_finish(const FrPlanDev *plans, FrCtl *ctl, const int *slots, c
     } else {
         C.pending_start = 0;
         C.full_check = 0;
-        if (C.best_node_key != ~0ull) {
-            const FrFresh f = fq[(unsigned)(C.best_node_key & 0xFFFFFFFFu)];
-            const int nz = f.pos / P.W, nx = f.pos - nz * P.W;
-            const int cell = dev_window_cell(P, f.pos, dev_key_rank(f.key));
-            float c_cost = 0;
-            (void)dev_pair_lcost(P, f.pos, cell, &c_cost);
+        if (bk != ~0ull) {
             const int cz = cell / P.W, cx = cell - cz * P.W;
-            const float base = P.cost[f.pos];
+            const float base = P.cost[npos];
             const float ch = P.gc_heading[cell];
             dev_add(P, cx, cz, ch, nx, nz, c_cost + base);
             dev_add(P, P.gx, P.gz, P.gh, cx, cz, ch + base);
@@ -1282,8 +1577,9 @@ __global__ void k_walk(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int
     C.path_len = n;
 }
 
-/* graph_optimization.cu:25-56 for all (i, j >= i + 2): each i keeps the last
- * improving j (atomicMax over j). pd[i].x holds that j (-1: none). */
+/* graph_optimization.cu:25-56 for all (i, j >= i + 2), one wave per pair:
+ * each i keeps the last improving j with its connection cost, as one 64-bit
+ * key (j << 32 | cost bits) in pd[i].xy, reduced with atomicMax (0: none). */
 __global__ void k_opt_pairs(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int gate)
 {
     const int s = slots[blockIdx.y];
@@ -1296,16 +1592,19 @@ __global__ void k_opt_pairs(const FrPlanDev *plans, FrCtl *ctl, const int *slots
         return;
     const long npairs = (long)n * n;
     const float max_curv = P.max_curv_loop;
-    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < npairs; k += (long)gridDim.x * blockDim.x) {
+    const int wpb = blockDim.x >> 6;
+    const long nw = (long)gridDim.x * wpb;
+    for (long k = blockIdx.x * (long)wpb + (threadIdx.x >> 6); k < npairs; k += nw) {
         const int i = (int)(k / n), j = (int)(k - (long)i * n);
         if (i >= n - 2 || j < i + 2)
             continue;
         const float4 a = P.path[i], b = P.path[j];
         const float cur = b.w;
-        const float nc = dev_check_direct(P, max_curv, fr_trunc_f(a.x), fr_trunc_f(a.y), a.z, fr_trunc_f(b.x),
-                                          fr_trunc_f(b.y), b.z, true, true);
-        if (nc > 0 && nc < cur)
-            atomicMax((int *)&P.pd[i].x, j);
+        const float nc = dev_wave_check_direct(P, max_curv, fr_trunc_f(a.x), fr_trunc_f(a.y), a.z,
+                                               fr_trunc_f(b.x), fr_trunc_f(b.y), b.z, true, true);
+        if (dev_lane() == 0 && nc > 0 && nc < cur)
+            atomicMax((unsigned long long *)&P.pd[i].x,
+                      ((unsigned long long)(unsigned)j << 32) | (unsigned)__float_as_int(nc));
     }
 }
 
@@ -1316,82 +1615,92 @@ __global__ void k_opt_reset(const FrPlanDev *plans, FrCtl *ctl, const int *slots
     if (!dev_gate(ctl[s], gate))
         return;
     const int n = ctl[s].path_len;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        int *pi = (int *)&P.pd[i].x;
-        *pi = -1;
-    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        *(unsigned long long *)&P.pd[i].x = 0ull;
 }
 
-/* graph_optimization.cu:126-183: the first i with the largest gain wins; the
- * path is spliced, the grid cleared and the chain re-added.  One block per
- * plan; lane 0 does the sequential splice/rebuild, the block clears. */
-__global__ void k_opt_apply(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int gate)
+/* graph_optimization.cu:126-183: the first i with the largest gain wins and
+ * the path is spliced (one thread per plan); k_opt_clear then empties the
+ * grid and k_opt_rebuild re-adds the chain.  opt_changed = 2 marks the plans
+ * this round changed until the rebuild. */
+__global__ void k_opt_apply(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n_slots, int gate)
 {
-    const int s = slots[blockIdx.x];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_slots)
+        return;
+    const int s = slots[t];
     const FrPlanDev &P = plans[s];
     FrCtl &C = ctl[s];
-    __shared__ int s_go;
-    __shared__ int s_n;
     if (!dev_gate(C, gate))
         return;
-    if (threadIdx.x == 0) {
-        int n = C.path_len;
-        C.opt_changed = 0;
-        int max_pos = -1;
-        float max_gain = -1, max_nc = 0;
-        for (int i = 0; i < n - 2; ++i) {
-            const int j = *(const int *)&P.pd[i].x;
-            if (j < 0)
-                continue;
-            const float4 a = P.path[i], b = P.path[j];
-            const float nc = dev_check_direct(P, P.max_curv_loop, fr_trunc_f(a.x), fr_trunc_f(a.y), a.z,
-                                              fr_trunc_f(b.x), fr_trunc_f(b.y), b.z, true, true);
-            const float gain = b.w - nc;
-            if (max_gain < gain) {
-                max_gain = gain;
-                max_pos = i;
-                max_nc = nc;
-            }
+    int n = C.path_len;
+    C.opt_changed = 0;
+    int max_pos = -1, next_pos = -1;
+    float max_gain = -1, max_nc = 0;
+    for (int i = 0; i < n - 2; ++i) {
+        const unsigned long long key = *(const unsigned long long *)&P.pd[i].x;
+        if (!key)
+            continue;
+        const int j = (int)(key >> 32);
+        const float nc = __int_as_float((int)(unsigned)(key & 0xFFFFFFFFull));
+        const float gain = P.path[j].w - nc;
+        if (max_gain < gain) {
+            max_gain = gain;
+            max_pos = i;
+            max_nc = nc;
+            next_pos = j;
         }
-        s_go = 0;
-        if (max_pos >= 0) {
-            const int next_pos = *(const int *)&P.pd[max_pos].x;
-            if (next_pos > max_pos + 1) {
-                P.path[next_pos].w = max_nc;
-                const int rm = next_pos - max_pos - 1;
-                for (int i = max_pos + 1; i + rm < n; ++i)
-                    P.path[i] = P.path[i + rm];
-                n -= rm;
-            }
-            C.path_len = n;
-            C.opt_changed = 1;
-            C.opt_rounds += 1;
-            if (gate == 1 && C.opt_rounds >= C.opt_limit)
-                C.opt_active = 0;
-            C.goal_reached = 0;
-            s_go = 1;
-        } else {
-            C.opt_active = 0;
-        }
-        s_n = n;
     }
-    __syncthreads();
-    if (!s_go)
+    if (max_pos < 0) {
+        C.opt_active = 0;
         return;
-    for (int pos = threadIdx.x; pos < P.N; pos += blockDim.x)
-        P.type[pos] = FR_TYPE_NULL;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int n = s_n;
-        int ppx = fr_trunc_f(P.path[0].x), ppz = fr_trunc_f(P.path[0].y);
-        dev_add(P, ppx, ppz, P.path[0].z, -1, -1, P.path[0].w);
-        for (int i = 1; i < n; ++i) {
-            const int cx = fr_trunc_f(P.path[i].x), cz = fr_trunc_f(P.path[i].y);
-            dev_add(P, cx, cz, P.path[i].z, ppx, ppz, P.path[i].w);
-            ppx = cx;
-            ppz = cz;
-        }
     }
+    if (next_pos > max_pos + 1) {
+        P.path[next_pos].w = max_nc;
+        const int rm = next_pos - max_pos - 1;
+        for (int i = max_pos + 1; i + rm < n; ++i)
+            P.path[i] = P.path[i + rm];
+        n -= rm;
+    }
+    C.path_len = n;
+    C.opt_changed = 2;
+    C.opt_rounds += 1;
+    if (gate == 1 && C.opt_rounds >= C.opt_limit)
+        C.opt_active = 0;
+    C.goal_reached = 0;
+}
+
+/* graph.cu:400-414 for the plans the round changed */
+__global__ void k_opt_clear(const FrPlanDev *plans, const FrCtl *ctl, const int *slots)
+{
+    const int s = slots[blockIdx.y];
+    if (ctl[s].opt_changed != 2)
+        return;
+    const FrPlanDev &P = plans[s];
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x)
+        P.type[pos] = FR_TYPE_NULL;
+}
+
+__global__ void k_opt_rebuild(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n_slots)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_slots)
+        return;
+    const int s = slots[t];
+    FrCtl &C = ctl[s];
+    if (C.opt_changed != 2)
+        return;
+    const FrPlanDev &P = plans[s];
+    const int n = C.path_len;
+    int ppx = fr_trunc_f(P.path[0].x), ppz = fr_trunc_f(P.path[0].y);
+    dev_add(P, ppx, ppz, P.path[0].z, -1, -1, P.path[0].w);
+    for (int i = 1; i < n; ++i) {
+        const int cx = fr_trunc_f(P.path[i].x), cz = fr_trunc_f(P.path[i].y);
+        dev_add(P, cx, cz, P.path[i].z, ppx, ppz, P.path[i].w);
+        ppx = cx;
+        ppz = cz;
+    }
+    C.opt_changed = 1;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1420,11 +1729,11 @@ void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, in
                        d_slots);
     if (goal_pass) {
         fr_prof_begin(FR_PROF_GOAL_CONNECTION, st);
-        hipLaunchKernelGGL(k_setup_goal_connection, dim3(fr_grid_for(max_N, FR_BLOCK, 4096), n), b, 0, st, L.plans,
-                           d_slots);
+        hipLaunchKernelGGL(k_setup_goal_connection, dim3(fr_grid_for(max_N, FR_BLOCK / 64, 16384), n), b, 0, st,
+                           L.plans, d_slots);
         fr_prof_end(FR_PROF_GOAL_CONNECTION, st);
-        hipLaunchKernelGGL(k_setup_gc_rowsum, dim3(fr_grid_for(max_PH, 64, 64), n), dim3(64), 0, st, L.plans,
-                           d_slots);
+        hipLaunchKernelGGL(k_setup_gc_rowsum, dim3(fr_grid_for(max_PH, FR_BLOCK / 64, 1024), n), b, 0, st,
+                           L.plans, d_slots);
     }
 }
 
@@ -1439,7 +1748,9 @@ void fr_launch_iteration(const FrLaunch &L, hipStream_t st)
     hipLaunchKernelGGL(k_select, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles, L.exp_q,
                        L.counters);
     fr_prof_begin(FR_PROF_EXPAND, st);
-    hipLaunchKernelGGL(k_expand, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.exp_q, L.counters);
+    hipLaunchKernelGGL(k_expand, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.exp_q, L.chk, L.chk_cap,
+                       L.counters);
+    hipLaunchKernelGGL(k_footprint, dim3(L.grid_waves), b, 0, st, L.plans, L.ctl, L.exp_q, L.chk, L.counters);
     fr_prof_end(FR_PROF_EXPAND, st);
     hipLaunchKernelGGL(k_commit, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.exp_q, L.counters);
     hipLaunchKernelGGL(k_density, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles);
@@ -1452,13 +1763,13 @@ void fr_launch_iteration(const FrLaunch &L, hipStream_t st)
     fr_prof_begin(FR_PROF_CONNECT, st);
     hipLaunchKernelGGL(k_connect_emit, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.pairs, L.pair_cap,
                        L.counters);
-    hipLaunchKernelGGL(k_connect_eval, dim3(L.grid_pairs), b, 0, st, L.plans, L.ctl, L.fresh_q, L.pairs, L.pair_cap,
-                       L.counters);
+    hipLaunchKernelGGL(k_connect_eval, dim3(L.grid_waves), b, 0, st, L.plans, L.ctl, L.fresh_q, L.pairs,
+                       L.pair_cap, L.counters);
     hipLaunchKernelGGL(k_connect_inline, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);
     fr_prof_end(FR_PROF_CONNECT, st);
     hipLaunchKernelGGL(k_connect_pick, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);
-    hipLaunchKernelGGL(k_finish, dim3(fr_grid_for(n_slots, FR_BLOCK, 65535)), b, 0, st, L.plans, L.ctl, L.slots,
-                       L.n_slots, L.fresh_q);
+    hipLaunchKernelGGL(k_finish, dim3(fr_grid_for(n_slots, FR_BLOCK / 64, 65535)), b, 0, st, L.plans, L.ctl,
+                       L.slots, L.n_slots, L.fresh_q);
     fr_prof_end(FR_PROF_ITERATION, st);
     (void)gs;
 }
@@ -1500,13 +1811,19 @@ void fr_launch_walk(const FrLaunch &L, const int *d_slots, int n, hipStream_t st
     hipLaunchKernelGGL(k_walk, dim3(fr_grid_for(n, 64, 65535)), dim3(64), 0, st, L.plans, L.ctl, d_slots, n, gate);
 }
 
-void fr_launch_opt_round(const FrLaunch &L, const int *d_slots, int n, int max_path, hipStream_t st, int gate)
+void fr_launch_opt_round(const FrLaunch &L, const int *d_slots, int n, int max_path, int max_N, hipStream_t st,
+                         int gate)
 {
     dim3 g1(fr_grid_for(max_path, FR_BLOCK, 64), n);
     hipLaunchKernelGGL(k_opt_reset, g1, dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots, gate);
-    dim3 g2(fr_grid_for((long)max_path * max_path, FR_BLOCK, 4096), n);
+    dim3 g2(fr_grid_for((long)max_path * max_path, FR_BLOCK / 64, 8192), n);
     hipLaunchKernelGGL(k_opt_pairs, g2, dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots, gate);
-    hipLaunchKernelGGL(k_opt_apply, dim3(n), dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots, gate);
+    hipLaunchKernelGGL(k_opt_apply, dim3(fr_grid_for(n, 64, 65535)), dim3(64), 0, st, L.plans, L.ctl, d_slots, n,
+                       gate);
+    hipLaunchKernelGGL(k_opt_clear, dim3(fr_grid_for(max_N, FR_BLOCK, 256), n), dim3(FR_BLOCK), 0, st, L.plans,
+                       L.ctl, d_slots);
+    hipLaunchKernelGGL(k_opt_rebuild, dim3(fr_grid_for(n, 64, 65535)), dim3(64), 0, st, L.plans, L.ctl, d_slots,
+                       n);
 }
 
 /* graph_best_node.cu:182-222 for several plans (blockIdx.y), skipping plans
