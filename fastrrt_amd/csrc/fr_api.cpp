@@ -240,17 +240,28 @@ class FrGraph {
         bound_costs = c;
         bound_n = n;
         rebind();
-        run_setup(false);
+        run_setup(FR_GOAL_NONE);
         frame_prepped = frame;
         prepped_costs = c;
         prepped_gen = frame_gen(frame);
     }
 
-    /* prep + distance planes, and optionally the goal-connection planes */
-    void run_setup(bool goal_connection)
+    /* prep + distance planes, and the goal-connection planes per goal_mode */
+    void run_setup(int goal_mode)
     {
         std::vector<int> one{0};
-        eng.setup(one, goal_connection);
+        eng.setup(one, goal_mode);
+        if (goal_mode != FR_GOAL_NONE)
+            gc_pending = goal_mode == FR_GOAL_LAZY;
+    }
+
+    /* evaluate what a lazy goal-connection pass has not evaluated yet */
+    void goal_complete()
+    {
+        if (gc_pending) {
+            eng.goal_complete(0);
+            gc_pending = false;
+        }
     }
 
     static unsigned long frame_gen(const FrFrame *f) { return f ? gen_of(f) : 0; }
@@ -378,8 +389,12 @@ class FrGraph {
         return {h[0], h[1]};
     }
 
-    /* graph_direct_goal_connection.cu:200-227 (planes from the frame's costs) */
-    void process_direct_goal_connection(const FrFrame *frame, int gx, int gz, float gh, float max_curv)
+    /* graph_direct_goal_connection.cu:200-227 (planes from the frame's costs).
+     * lazy: a cell's planes are evaluated when a node's connection window
+     * first covers it (same values: the evaluation is a function of the
+     * frame, the goal and the cost plane at this call, which is kept). */
+    void process_direct_goal_connection(const FrFrame *frame, int gx, int gz, float gh, float max_curv,
+                                        bool lazy = false)
     {
         if (!frame)
             throw std::runtime_error("null frame");
@@ -389,11 +404,12 @@ class FrGraph {
         prm.gz = gz;
         prm.gh = gh;
         prm.max_curv_goal = max_curv;
+        prm.gc_lazy = lazy ? 1 : 0;
         bound_xyz = frame->d_xyz;
         bound_costs = frame->d_costs;
         bound_n = frame->n_classes;
         rebind();
-        run_setup(true);
+        run_setup(lazy ? FR_GOAL_LAZY : FR_GOAL_EAGER);
         frame_prepped = frame;
         prepped_costs = frame->d_costs;
         prepped_gen = frame_gen(frame);
@@ -566,6 +582,8 @@ class FrGraph {
     {
         const size_t n = (size_t)N();
         std::vector<uint8_t> ty;
+        if (ogc_flag || ogc_heading || ogc_cost)
+            goal_complete();
         if (opx)
             FR_CHECK(hipMemcpyAsync(opx, slab.px, 4 * n, hipMemcpyDeviceToHost, st()));
         if (opz)
@@ -629,6 +647,7 @@ class FrGraph {
     const FrFrame *frame_prepped = nullptr;
     const float *prepped_costs = nullptr;
     unsigned long prepped_gen = 0;
+    bool gc_pending = false; /* lazy goal-connection planes not fully evaluated */
     int *d_tmp = nullptr;
     uint32_t *d_region = nullptr;
 };
@@ -678,7 +697,7 @@ class FrFastRRT {
         g.prm.herr_tol = herr;
         g.set_search(mdx, mdz, lbx_, lbz_, ubx_, ubz_);
         g.graph_costs = frame->h_costs;
-        g.process_direct_goal_connection(frame, gx, gz, gh, 0.8f);
+        g.process_direct_goal_connection(frame, gx, gz, gh, 0.8f, true);
     }
 
     /* fastrrt.cpp:85-98 */

@@ -184,7 +184,7 @@ class FrBatch {
         if (it != pool_.end() && !it->second.empty()) {
             s = std::move(it->second.back());
             it->second.pop_back();
-            s->allocate(W, H, mx, mz, path_cap(W, H), false, eng_.stream()); /* re-zero */
+            s->reset(eng_.stream());
         } else {
             s.reset(new FrSlab());
             s->allocate(W, H, mx, mz, path_cap(W, H), false, eng_.stream());
@@ -222,6 +222,7 @@ class FrBatch {
             Slot &s = slots_[free_slot];
             FrPlanParams prm;
             fill_params(prm, p.req, f);
+            prm.gc_lazy = 1;
             const int mx = fr_trunc_f((float)p.req.min_dist_x / 2.0f);
             const int mz = fr_trunc_f((float)p.req.min_dist_z / 2.0f);
             s.slab = take_slab(f->W, f->H, mx, mz);
@@ -241,7 +242,7 @@ class FrBatch {
         if (admitted.empty())
             return;
         /* per-plan setup (goal-connection precompute) for the admitted set */
-        eng_.setup(admitted, true);
+        eng_.setup(admitted, FR_GOAL_LAZY);
         for (int sl : admitted) {
             FrCtl c{};
             c.status = FR_SEARCHING;

@@ -56,7 +56,8 @@ struct FrPlanDev {
     /* goal connection */
     uint8_t *gc_flag;
     float *gc_heading, *gc_cost;
-    int32_t *gc_rowsum;       /* per row, exclusive prefix count of gc_flag ((W + 1) per row) */
+    uint32_t *gc_done;        /* bit per cell: goal-connection planes evaluated */
+    int gc_lazy;              /* evaluate a cell when a node's window first covers it */
     /* scratch planes */
     unsigned long long *claim;
     uint8_t *mark;

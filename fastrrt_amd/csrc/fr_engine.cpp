@@ -178,7 +178,7 @@ void FrSlab::allocate(int W_, int H_, int mx_, int mz_, int path_cap_, bool with
     const size_t o_gcf = take(N);
     const size_t o_gch = take(4 * N);
     const size_t o_gcc = take(4 * N);
-    const size_t o_rows = take(4 * (size_t)(W + 1) * H);
+    const size_t o_done = take(4 * ((N + 31) / 32));
     const size_t o_mark = take(N);
     const size_t o_trav = take(N);
     const size_t o_region = take(4 * (size_t)2 * dens);
@@ -200,18 +200,24 @@ void FrSlab::allocate(int W_, int H_, int mx_, int mz_, int path_cap_, bool with
     gc_flag = (uint8_t *)(b + o_gcf);
     gc_heading = (float *)(b + o_gch);
     gc_cost = (float *)(b + o_gcc);
-    gc_rowsum = (int32_t *)(b + o_rows);
+    gc_done = (uint32_t *)(b + o_done);
     mark = (uint8_t *)(b + o_mark);
     trav = (uint8_t *)(b + o_trav);
     region = (uint32_t *)(b + o_region);
     path = (float4 *)(b + o_path);
     pd = (float4 *)(b + o_pd);
     claim = (unsigned long long *)(b + o_claim);
-    FR_CHECK(hipMemsetAsync(base_, 0, zero_end, st));
-    FR_CHECK(hipMemsetAsync(claim, 0xFF, 8 * N, st));
+    zero_end_ = zero_end;
+    reset(st);
     mx = -1;
     mz = -1;
     set_footprint(mx_, mz_, st);
+}
+
+void FrSlab::reset(hipStream_t st)
+{
+    FR_CHECK(hipMemsetAsync(base_, 0, zero_end_, st));
+    FR_CHECK(hipMemsetAsync(claim, 0xFF, 8 * (size_t)W * H, st));
 }
 
 FrFrame::~FrFrame()
@@ -251,6 +257,8 @@ FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
     FR_CHECK(hipMalloc(&L_.pairs, sizeof(FrPair) * L_.pair_cap));
     L_.chk_cap = std::min<long>((queue_cap + 1) * FR_CHK_STRIDE, 1L << 24);
     FR_CHECK(hipMalloc(&L_.chk, sizeof(int2) * L_.chk_cap));
+    L_.demand_cap = queue_cap + 1;
+    FR_CHECK(hipMalloc(&L_.demand, sizeof(int2) * L_.demand_cap));
     FR_CHECK(hipMemsetAsync(L_.ctl, 0, sizeof(FrCtl) * max_slots, st_));
     FR_CHECK(hipMemsetAsync(L_.n_tiles, 0, sizeof(int), st_));
     FR_CHECK(hipMemsetAsync(L_.counters, 0, sizeof(int) * FR_CNT_N, st_));
@@ -277,6 +285,7 @@ FrEngine::~FrEngine()
     (void)hipFree(d_scratch_);
     (void)hipFree(L_.pairs);
     (void)hipFree(L_.chk);
+    (void)hipFree(L_.demand);
     if (own_stream_)
         (void)hipStreamDestroy(st_);
 }
@@ -329,7 +338,8 @@ void FrEngine::bind(int slot, FrSlab &s, const FrPlanParams &p, const FrFrameVie
     d.gc_flag = s.gc_flag;
     d.gc_heading = s.gc_heading;
     d.gc_cost = s.gc_cost;
-    d.gc_rowsum = s.gc_rowsum;
+    d.gc_done = s.gc_done;
+    d.gc_lazy = p.gc_lazy;
     d.claim = s.claim;
     d.mark = s.mark;
     d.dist = s.dist;
@@ -397,7 +407,7 @@ void FrEngine::set_active(const std::vector<int> &slots)
     FR_CHECK(hipStreamSynchronize(st_));
 }
 
-void FrEngine::setup(const std::vector<int> &slots, bool goal_pass)
+void FrEngine::setup(const std::vector<int> &slots, int goal_mode)
 {
     if (slots.empty())
         return;
@@ -410,7 +420,21 @@ void FrEngine::setup(const std::vector<int> &slots, bool goal_pass)
         max_PWH = std::max(max_PWH, (long)d.PW * d.PH);
     }
     FR_CHECK(hipMemcpyAsync(d_setup_slots_, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, st_));
-    fr_launch_setup(L_, d_setup_slots_, (int)slots.size(), max_N, max_PH, max_PWH, goal_pass, st_);
+    for (int s : slots) {
+        if (goal_mode != FR_GOAL_NONE && h_plans_[s].gc_lazy != (goal_mode == FR_GOAL_LAZY)) {
+            h_plans_[s].gc_lazy = goal_mode == FR_GOAL_LAZY;
+            upload_plan(s);
+        }
+    }
+    fr_launch_setup(L_, d_setup_slots_, (int)slots.size(), max_N, max_PH, max_PWH, goal_mode, st_);
+    FR_CHECK(hipGetLastError());
+    FR_CHECK(hipStreamSynchronize(st_));
+}
+
+void FrEngine::goal_complete(int slot)
+{
+    FR_CHECK(hipMemcpyAsync(d_setup_slots_, &slot, sizeof(int), hipMemcpyHostToDevice, st_));
+    fr_launch_goal_complete(L_, d_setup_slots_, 1, h_plans_[slot].N, st_);
     FR_CHECK(hipGetLastError());
     FR_CHECK(hipStreamSynchronize(st_));
 }

@@ -16,7 +16,11 @@
 
 #include "fr_device.h"
 
-enum { FR_CNT_EXP = 0, FR_CNT_FRESH = 1, FR_CNT_PAIRS = 2, FR_CNT_N = 8 };
+enum { FR_CNT_EXP = 0, FR_CNT_FRESH = 1, FR_CNT_PAIRS = 2, FR_CNT_DEMAND = 3, FR_CNT_N = 8 };
+
+/* goal-connection planes at setup: untouched, evaluated when first needed
+ * (lazy), or evaluated for every cell (eager) */
+enum { FR_GOAL_NONE = 0, FR_GOAL_LAZY = 1, FR_GOAL_EAGER = 2 };
 
 struct FrLaunch {
     FrPlanDev *plans;
@@ -31,6 +35,8 @@ struct FrLaunch {
     long pair_cap;
     int2 *chk;                /* per rollout: (cell, heading bits) footprint tests, FR_CHK_STRIDE each */
     long chk_cap;
+    int2 *demand;             /* (slot, cell) goal-connection evaluations queued this iteration */
+    long demand_cap;
     int *n_slots;             /* device: entries of `slots` */
     int grid_tiles, grid_items, grid_slots, grid_waves;
     int max_slots;
@@ -43,7 +49,9 @@ struct FrBestNodeArgs {
 
 /* kernel launchers (fr_kernels.hip) */
 void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, int max_PH, long max_PWH,
-                     bool goal_pass, hipStream_t st);
+                     int goal_mode, hipStream_t st);
+/* evaluate the goal-connection planes of every cell not yet evaluated */
+void fr_launch_goal_complete(const FrLaunch &L, const int *d_slots, int n, int max_N, hipStream_t st);
 void fr_launch_iteration(const FrLaunch &L, hipStream_t st);
 void fr_launch_clear(const FrLaunch &L, int slot, int N, hipStream_t st);
 void fr_launch_count(const FrLaunch &L, int slot, int N, int *d_out, hipStream_t st);
@@ -110,6 +118,7 @@ struct FrPlanParams {
     uint64_t seed = 0, stream = 0;
     int path_cap = 0;             /* 0: W*H + 1 */
     bool with_intrinsic = false;
+    int gc_lazy = 0;              /* goal-connection planes evaluated on demand */
 };
 
 /* Device planes of one plan (one allocation, sub-allocated). */
@@ -122,6 +131,8 @@ class FrSlab {
     void allocate(int W, int H, int mx, int mz, int path_cap, bool with_intrinsic, hipStream_t st);
     void release();
     void set_footprint(int mx, int mz, hipStream_t st);
+    /* back to the state allocate() leaves: planes zero, claims all-ones */
+    void reset(hipStream_t st);
     size_t bytes() const { return bytes_; }
     bool allocated() const { return base_ != nullptr; }
     int W = 0, H = 0, pad = 0, PW = 0, PH = 0, mx = -1, mz = -1, path_cap = 0;
@@ -131,7 +142,7 @@ class FrSlab {
     float *heading = nullptr, *cost = nullptr, *intrinsic = nullptr, *direct = nullptr;
     uint8_t *gc_flag = nullptr;
     float *gc_heading = nullptr, *gc_cost = nullptr;
-    int32_t *gc_rowsum = nullptr;
+    uint32_t *gc_done = nullptr;
     unsigned long long *claim = nullptr;
     uint8_t *mark = nullptr, *dist = nullptr, *dist_tmp = nullptr, *trav = nullptr;
     uint32_t *region = nullptr;
@@ -141,6 +152,7 @@ class FrSlab {
     void *base_ = nullptr;
     void *dist_base_ = nullptr;
     size_t bytes_ = 0;
+    size_t zero_end_ = 0;
 };
 
 /* Device pointers a plan reads its obstacles from. */
@@ -185,7 +197,8 @@ class FrEngine {
     int n_active() const { return (int)active_.size(); }
     const int *d_active() const { return L_.slots; }
 
-    void setup(const std::vector<int> &slots, bool goal_pass = true); /* per-plan precompute */
+    void setup(const std::vector<int> &slots, int goal_mode); /* per-plan precompute */
+    void goal_complete(int slot);                  /* finish a lazy goal-connection pass */
     void iteration();                              /* one loop() for every active plan */
     void sync();
 

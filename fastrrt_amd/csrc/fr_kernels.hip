@@ -337,7 +337,7 @@ __device__ __forceinline__ bool dev_hermite_point(const FrPlanDev &P, const FrHe
  * the last cell never change it, so "last accepted" = "last in-grid").  The
  * cost is summed lane by lane in point order, as the reference adds it. */
 __device__ float dev_wave_check_direct(const FrPlanDev &P, float max_curv, int x, int z, float lh, int gx, int gz,
-                                       float gh, bool safe_zone, bool host_nan)
+                                       float gh, bool safe_zone, bool host_nan, float base)
 {
     const int W = P.W;
     const int lane = dev_lane();
@@ -363,7 +363,7 @@ __device__ float dev_wave_check_direct(const FrPlanDev &P, float max_curv, int x
     hm.dgz = (double)gz;
     hm.denom = (double)(num_points - 1);
     const bool skip_fp = safe_zone && ((P.trav[pos] & 2) != 0);
-    float node_cost = P.cost[pos];
+    float node_cost = base;
     int last_x = -1, last_z = -1;
     for (int pass = 0; pass < 2; ++pass) {
         last_x = -1;
@@ -573,29 +573,6 @@ __global__ void k_setup_rowdist(const FrPlanDev *plans, const int *slots)
     }
 }
 
-/* exclusive per-row prefix counts of the goal-flag plane; one wave per row */
-__global__ void k_setup_gc_rowsum(const FrPlanDev *plans, const int *slots)
-{
-    const FrPlanDev &P = plans[slots[blockIdx.y]];
-    const int wpb = blockDim.x >> 6;
-    const int lane = dev_lane();
-    for (int z = blockIdx.x * wpb + (int)(threadIdx.x >> 6); z < P.H; z += gridDim.x * wpb) {
-        int32_t *row = P.gc_rowsum + (long)z * (P.W + 1);
-        const uint8_t *f = P.gc_flag + (long)z * P.W;
-        int acc = 0;
-        for (int b = 0; b < P.W; b += 64) {
-            const int x = b + lane;
-            const bool on = x < P.W && f[x] != 0;
-            const unsigned long long m = __ballot(on);
-            if (x < P.W)
-                row[x] = acc + __popcll(m & ((1ull << lane) - 1ull));
-            acc += __popcll(m);
-        }
-        if (lane == 0)
-            row[P.W] = acc;
-    }
-}
-
 /* Column pass: D(x, z) = min_dz max(|dz|, rowdist(x, z + dz)), capped. */
 __global__ void k_setup_coldist(const FrPlanDev *plans, const int *slots)
 {
@@ -622,24 +599,48 @@ __global__ void k_setup_coldist(const FrPlanDev *plans, const int *slots)
     }
 }
 
-/* graph_direct_goal_connection.cu:169-198: per cell, Hermite to the goal.
- * One wave per cell, the curve's points over the lanes. */
-__global__ void k_setup_goal_connection(const FrPlanDev *plans, const int *slots)
+/* graph_direct_goal_connection.cu:169-198 for one cell: Hermite to the goal
+ * (one wave).  The start cost is the cost plane as it was at setup, kept in
+ * gc_cost by k_setup_gc_reset, so an evaluation made later gives the same
+ * planes as the reference's pass at set-up time. */
+__device__ void dev_gc_cell(const FrPlanDev &P, int pos)
+{
+    const int z = pos / P.W, x = pos - z * P.W;
+    const float lh = (float)dev_compute_heading(x, z, P.gx, P.gz);
+    const float c =
+        dev_wave_check_direct(P, P.max_curv_goal, x, z, lh, P.gx, P.gz, P.gh, false, false, P.gc_cost[pos]);
+    if (dev_lane() == 0) {
+        P.gc_flag[pos] = c < 0 ? 0 : 1;
+        if (c >= 0) {
+            P.gc_heading[pos] = lh;
+            P.gc_cost[pos] = c;
+        }
+        atomicOr(&P.gc_done[pos >> 5], 1u << (pos & 31));
+    }
+}
+
+/* snapshot of the cost plane, flags cleared, nothing evaluated */
+__global__ void k_setup_gc_reset(const FrPlanDev *plans, const int *slots)
+{
+    const FrPlanDev &P = plans[slots[blockIdx.y]];
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+        P.gc_cost[pos] = P.cost[pos];
+        P.gc_flag[pos] = 0;
+        if ((pos & 31) == 0)
+            P.gc_done[pos >> 5] = 0;
+    }
+}
+
+/* every cell (skip_done: every cell not yet evaluated), one wave per cell */
+__global__ void k_setup_goal_connection(const FrPlanDev *plans, const int *slots, int skip_done)
 {
     const FrPlanDev &P = plans[slots[blockIdx.y]];
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
     for (int pos = blockIdx.x * wpb + (int)(threadIdx.x >> 6); pos < P.N; pos += nw) {
-        const int z = pos / P.W, x = pos - z * P.W;
-        const float lh = (float)dev_compute_heading(x, z, P.gx, P.gz);
-        const float c = dev_wave_check_direct(P, P.max_curv_goal, x, z, lh, P.gx, P.gz, P.gh, false, false);
-        if (dev_lane() == 0) {
-            P.gc_flag[pos] = c < 0 ? 0 : 1;
-            if (c >= 0) {
-                P.gc_heading[pos] = lh;
-                P.gc_cost[pos] = c;
-            }
-        }
+        if (skip_done && (P.gc_done[pos >> 5] & (1u << (pos & 31))))
+            continue;
+        dev_gc_cell(P, pos);
     }
 }
 
@@ -655,6 +656,7 @@ __global__ void k_begin(const FrPlanDev *plans, FrCtl *ctl, const int *slots, co
         counters[FR_CNT_EXP] = 0;
         counters[FR_CNT_FRESH] = 0;
         counters[FR_CNT_PAIRS] = 0;
+        counters[FR_CNT_DEMAND] = 0;
     }
     if (i >= n_slots)
         return;
@@ -1129,16 +1131,11 @@ __device__ __forceinline__ void dev_window(const FrPlanDev &P, int x, int z, int
     *x1 = d > P.W ? P.W : d;
 }
 
-__device__ __forceinline__ int dev_window_count(const FrPlanDev &P, int z0, int z1, int x0, int x1)
+/* the q-th cell (row-major) of a window whose rows are wx cells wide */
+__device__ __forceinline__ int dev_window_at(const FrPlanDev &P, int z0, int x0, int wx, int q)
 {
-    if (z0 >= z1 || x0 >= x1)
-        return 0;
-    int n = 0;
-    for (int zp = z0; zp < z1; ++zp) {
-        const int32_t *row = P.gc_rowsum + (long)zp * (P.W + 1);
-        n += row[x1] - row[x0];
-    }
-    return n;
+    const int r = q / wx;
+    return (z0 + r) * P.W + x0 + (q - r * wx);
 }
 
 /* node -> flagged cell check; returns the lcost or FR_BIG_COST.  `wave`:
@@ -1149,7 +1146,7 @@ __device__ __forceinline__ long long dev_pair_lcost(const FrPlanDev &P, int node
     const int z = node / P.W, x = node - z * P.W;
     const int zp = cell / P.W, xp = cell - zp * P.W;
     const float c = wave ? dev_wave_check_direct(P, P.max_curv_loop, x, z, P.heading[node], xp, zp,
-                                                 P.gc_heading[cell], true, false)
+                                                 P.gc_heading[cell], true, false, P.cost[node])
                          : dev_check_direct(P, P.max_curv_loop, x, z, P.heading[node], xp, zp, P.gc_heading[cell],
                                             true, false);
     if (c_out)
@@ -1160,66 +1157,131 @@ __device__ __forceinline__ long long dev_pair_lcost(const FrPlanDev &P, int node
     return (long long)floor((double)(100.0f * total));
 }
 
-/* Goal test of the new nodes (graph_best_node.cu:182-222), window counts,
- * and the pair queue (block prefix sums + one atomic per block). */
+/* Cells of the new nodes' windows whose goal-connection planes are not yet
+ * evaluated (lazy plans): claimed with one atomicOr per cell, queued. */
+__global__ void k_gc_demand(const FrPlanDev *plans, const FrFresh *fq, int2 *dq, long dq_cap, int *counters)
+{
+    const int n = counters[FR_CNT_FRESH];
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    const int lane = dev_lane();
+    for (int i = blockIdx.x * wpb + (int)(threadIdx.x >> 6); i < n; i += nw) {
+        const FrFresh f = fq[i];
+        const FrPlanDev &P = plans[f.slot];
+        if (!P.gc_lazy || P.type[f.pos] != FR_TYPE_NODE)
+            continue;
+        const int z = f.pos / P.W, x = f.pos - z * P.W;
+        int z0, z1, x0, x1;
+        dev_window(P, x, z, &z0, &z1, &x0, &x1);
+        const int wx = x1 - x0, wz = z1 - z0;
+        if (wx <= 0 || wz <= 0)
+            continue;
+        const int total = wx * wz;
+        for (int b = 0; b < total; b += 64) {
+            const int k = b + lane;
+            bool mine = false;
+            int q = 0;
+            if (k < total) {
+                q = dev_window_at(P, z0, x0, wx, k);
+                const uint32_t bit = 1u << (q & 31);
+                if (!(P.gc_done[q >> 5] & bit))
+                    mine = !(atomicOr(&P.gc_done[q >> 5], bit) & bit);
+            }
+            const unsigned long long m = __ballot(mine);
+            if (!m)
+                continue;
+            int base = 0;
+            if (lane == 0)
+                base = atomicAdd(&counters[FR_CNT_DEMAND], __popcll(m));
+            base = dev_bcast_i(base, 0);
+            const long o = (long)base + __popcll(m & ((1ull << lane) - 1ull));
+            if (mine && o < dq_cap) /* each cell is queued once per plan: sum N <= dq_cap */
+                dq[o] = make_int2(f.slot, q);
+        }
+    }
+}
+
+__global__ void k_gc_eval(const FrPlanDev *plans, const int2 *dq, long dq_cap, const int *counters)
+{
+    long n = counters[FR_CNT_DEMAND];
+    if (n > dq_cap)
+        n = dq_cap;
+    const int wpb = blockDim.x >> 6;
+    const long nw = (long)gridDim.x * wpb;
+    for (long i = blockIdx.x * (long)wpb + (threadIdx.x >> 6); i < n; i += nw) {
+        const int2 d = dq[i];
+        dev_gc_cell(plans[d.x], d.y);
+    }
+}
+
+/* Goal test of the new nodes (graph_best_node.cu:182-222), window counts and
+ * the pair queue; one wave per node, pairs in the window's row-major order. */
 __global__ void k_connect_emit(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, FrPair *pairs, long pair_cap,
                                int *counters)
 {
     const int n = counters[FR_CNT_FRESH];
-    for (int base = blockIdx.x * FR_BLOCK; base < n; base += gridDim.x * FR_BLOCK) {
-        const int i = base + threadIdx.x;
-        int cnt = 0, z0 = 0, z1 = 0, x0 = 0, x1 = 0, x = 0, z = 0;
-        const FrPlanDev *Pp = nullptr;
-        if (i < n) {
-            FrFresh &f = fq[i];
-            const FrPlanDev &P = plans[f.slot];
-            FrCtl &C = ctl[f.slot];
-            Pp = &P;
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    const int lane = dev_lane();
+    for (int i = blockIdx.x * wpb + (int)(threadIdx.x >> 6); i < n; i += nw) {
+        FrFresh &f = fq[i];
+        const int slot = f.slot, fpos = f.pos;
+        const FrPlanDev &P = plans[slot];
+        FrCtl &C = ctl[slot];
+        const bool node = P.type[fpos] == FR_TYPE_NODE;
+        if (lane == 0) {
             f.key = ~0ull;
             f.pair_cnt = 0;
-            if (P.type[f.pos] == FR_TYPE_NODE) {
-                z = f.pos / P.W;
-                x = f.pos - z * P.W;
-                atomicAdd(&C.n_fresh, 1);
-                if (dev_goal_hit(P, x, z, P.heading[f.pos]))
-                    C.goal_reached = 1;
-                dev_window(P, x, z, &z0, &z1, &x0, &x1);
-                cnt = dev_window_count(P, z0, z1, x0, x1);
-            }
         }
-        int prefix;
-        const int total = block_exclusive_scan(cnt, &prefix);
-        __shared__ int s_base;
-        if (threadIdx.x == 0)
-            s_base = total ? atomicAdd(&counters[FR_CNT_PAIRS], total) : 0;
-        __syncthreads();
-        if (i < n && cnt > 0) {
-            FrFresh &f = fq[i];
-            const long o0 = (long)s_base + prefix;
-            if (o0 + cnt > pair_cap) {
+        if (!node)
+            continue;
+        const int z = fpos / P.W, x = fpos - z * P.W;
+        if (lane == 0) {
+            atomicAdd(&C.n_fresh, 1);
+            if (dev_goal_hit(P, x, z, P.heading[fpos]))
+                C.goal_reached = 1;
+        }
+        int z0, z1, x0, x1;
+        dev_window(P, x, z, &z0, &z1, &x0, &x1);
+        const int wx = x1 - x0, wz = z1 - z0;
+        if (wx <= 0 || wz <= 0)
+            continue;
+        const int total = wx * wz;
+        int cnt = 0;
+        for (int b = 0; b < total; b += 64) {
+            const int k = b + lane;
+            const bool on = k < total && P.gc_flag[dev_window_at(P, z0, x0, wx, k)] != 0;
+            cnt += __popcll(__ballot(on));
+        }
+        if (cnt == 0)
+            continue;
+        int base = 0;
+        if (lane == 0)
+            base = atomicAdd(&counters[FR_CNT_PAIRS], cnt);
+        base = dev_bcast_i(base, 0);
+        if ((long)base + cnt > pair_cap) {
+            if (lane == 0)
                 f.flags |= FR_FRESH_INLINE;
-            } else {
-                const FrPlanDev &P = *Pp;
-                long o = o0;
-                for (int zp = z0; zp < z1; ++zp) {
-                    const int32_t *row = P.gc_rowsum + (long)zp * (P.W + 1);
-                    if (row[x1] == row[x0])
-                        continue;
-                    for (int xp = x0; xp < x1; ++xp) {
-                        const int q = zp * P.W + xp;
-                        if (P.gc_flag[q]) {
-                            FrPair pr;
-                            pr.fresh = i;
-                            pr.cell = q;
-                            pairs[o++] = pr;
-                        }
-                    }
-                }
-                f.pair_base = (int)o0;
-                f.pair_cnt = cnt;
-            }
+            continue;
         }
-        __syncthreads();
+        long o = base;
+        for (int b = 0; b < total; b += 64) {
+            const int k = b + lane;
+            const int q = k < total ? dev_window_at(P, z0, x0, wx, k) : 0;
+            const bool on = k < total && P.gc_flag[q] != 0;
+            const unsigned long long m = __ballot(on);
+            if (on) {
+                FrPair pr;
+                pr.fresh = i;
+                pr.cell = q;
+                pairs[o + __popcll(m & ((1ull << lane) - 1ull))] = pr;
+            }
+            o += __popcll(m);
+        }
+        if (lane == 0) {
+            f.pair_base = base;
+            f.pair_cnt = cnt;
+        }
     }
 }
 
@@ -1303,27 +1365,31 @@ __global__ void k_connect_pick(const FrPlanDev *plans, FrCtl *ctl, const FrFresh
     }
 }
 
-/* the rank-th goal-flagged cell of a node's window, in scan order */
-__device__ int dev_window_cell(const FrPlanDev &P, int node, int rank)
+/* the rank-th goal-flagged cell of a node's window, in row-major order
+ * (one wave) */
+__device__ int dev_wave_window_cell(const FrPlanDev &P, int node, int rank)
 {
     const int z = node / P.W, x = node - z * P.W;
     int z0, z1, x0, x1;
     dev_window(P, x, z, &z0, &z1, &x0, &x1);
-    for (int zp = z0; zp < z1; ++zp) {
-        const int32_t *row = P.gc_rowsum + (long)zp * (P.W + 1);
-        const int c = row[x1] - row[x0];
-        if (rank >= c) {
-            rank -= c;
-            continue;
+    const int wx = x1 - x0, wz = z1 - z0;
+    if (wx <= 0 || wz <= 0)
+        return -1;
+    const int total = wx * wz;
+    const int lane = dev_lane();
+    int seen = 0;
+    for (int b = 0; b < total; b += 64) {
+        const int k = b + lane;
+        const int q = k < total ? dev_window_at(P, z0, x0, wx, k) : 0;
+        const bool on = k < total && P.gc_flag[q] != 0;
+        unsigned long long m = __ballot(on);
+        const int c = __popcll(m);
+        if (rank < seen + c) {
+            for (int t = rank - seen; t > 0; --t)
+                m &= m - 1;
+            return dev_bcast_i(q, __ffsll((long long)m) - 1);
         }
-        for (int xp = x0; xp < x1; ++xp) {
-            const int q = zp * P.W + xp;
-            if (P.gc_flag[q]) {
-                if (rank == 0)
-                    return q;
-                rank--;
-            }
-        }
+        seen += c;
     }
     return -1;
 }
@@ -1345,18 +1411,19 @@ __global__ void k_finish(const FrPlanDev *plans, FrCtl *ctl, const int *slots, c
         return;
     const bool lead = dev_lane() == 0;
     const bool reseed = C.n_temp == 0 && C.n_all == 0;
-    const unsigned long long bk = C.best_node_key;
+    unsigned long long bk = C.best_node_key;
     int nx = 0, nz = 0, cell = -1, npos = 0;
     float c_cost = 0;
-    unsigned long long fkey = 0;
     if (!reseed && bk != ~0ull) {
         const FrFresh f = fq[(unsigned)(bk & 0xFFFFFFFFu)];
         npos = f.pos;
-        fkey = f.key;
         nz = f.pos / P.W;
         nx = f.pos - nz * P.W;
-        cell = dev_window_cell(P, f.pos, dev_key_rank(fkey));
-        (void)dev_pair_lcost(P, f.pos, cell, &c_cost, true);
+        cell = dev_wave_window_cell(P, f.pos, dev_key_rank(f.key));
+        if (cell >= 0)
+            (void)dev_pair_lcost(P, f.pos, cell, &c_cost, true);
+        else
+            bk = ~0ull; /* unreachable: the rank counts flags present since the pair pass */
     }
     if (!lead)
         return;
@@ -1600,8 +1667,9 @@ __global__ void k_opt_pairs(const FrPlanDev *plans, FrCtl *ctl, const int *slots
             continue;
         const float4 a = P.path[i], b = P.path[j];
         const float cur = b.w;
-        const float nc = dev_wave_check_direct(P, max_curv, fr_trunc_f(a.x), fr_trunc_f(a.y), a.z,
-                                               fr_trunc_f(b.x), fr_trunc_f(b.y), b.z, true, true);
+        const int ax = fr_trunc_f(a.x), az = fr_trunc_f(a.y);
+        const float nc = dev_wave_check_direct(P, max_curv, ax, az, a.z, fr_trunc_f(b.x), fr_trunc_f(b.y), b.z,
+                                               true, true, P.cost[az * P.W + ax]);
         if (dev_lane() == 0 && nc > 0 && nc < cur)
             atomicMax((unsigned long long *)&P.pd[i].x,
                       ((unsigned long long)(unsigned)j << 32) | (unsigned)__float_as_int(nc));
@@ -1718,7 +1786,7 @@ static int fr_grid_for(long work, int per_block, int cap)
 }
 
 void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, int max_PH, long max_PWH,
-                     bool goal_pass, hipStream_t st)
+                     int goal_mode, hipStream_t st)
 {
     if (n <= 0)
         return;
@@ -1727,14 +1795,24 @@ void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, in
     hipLaunchKernelGGL(k_setup_rowdist, dim3(fr_grid_for(max_PWH, FR_BLOCK, 1024), n), b, 0, st, L.plans, d_slots);
     hipLaunchKernelGGL(k_setup_coldist, dim3(fr_grid_for(max_PWH, FR_BLOCK, 1024), n), b, 0, st, L.plans,
                        d_slots);
-    if (goal_pass) {
+    (void)max_PH;
+    if (goal_mode == FR_GOAL_NONE)
+        return;
+    hipLaunchKernelGGL(k_setup_gc_reset, dim3(fr_grid_for(max_N, FR_BLOCK, 1024), n), b, 0, st, L.plans, d_slots);
+    if (goal_mode == FR_GOAL_EAGER) {
         fr_prof_begin(FR_PROF_GOAL_CONNECTION, st);
         hipLaunchKernelGGL(k_setup_goal_connection, dim3(fr_grid_for(max_N, FR_BLOCK / 64, 16384), n), b, 0, st,
-                           L.plans, d_slots);
+                           L.plans, d_slots, 0);
         fr_prof_end(FR_PROF_GOAL_CONNECTION, st);
-        hipLaunchKernelGGL(k_setup_gc_rowsum, dim3(fr_grid_for(max_PH, FR_BLOCK / 64, 1024), n), b, 0, st,
-                           L.plans, d_slots);
     }
+}
+
+void fr_launch_goal_complete(const FrLaunch &L, const int *d_slots, int n, int max_N, hipStream_t st)
+{
+    if (n <= 0)
+        return;
+    hipLaunchKernelGGL(k_setup_goal_connection, dim3(fr_grid_for(max_N, FR_BLOCK / 64, 16384), n), dim3(FR_BLOCK), 0,
+                       st, L.plans, d_slots, 1);
 }
 
 void fr_launch_iteration(const FrLaunch &L, hipStream_t st)
@@ -1761,8 +1839,13 @@ void fr_launch_iteration(const FrLaunch &L, hipStream_t st)
     hipLaunchKernelGGL(k_accept, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles, L.fresh_q,
                        L.counters);
     fr_prof_begin(FR_PROF_CONNECT, st);
-    hipLaunchKernelGGL(k_connect_emit, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.pairs, L.pair_cap,
+    fr_prof_begin(FR_PROF_GOAL_CONNECTION, st);
+    hipLaunchKernelGGL(k_gc_demand, dim3(L.grid_waves), b, 0, st, L.plans, L.fresh_q, L.demand, L.demand_cap,
                        L.counters);
+    hipLaunchKernelGGL(k_gc_eval, dim3(L.grid_waves), b, 0, st, L.plans, L.demand, L.demand_cap, L.counters);
+    fr_prof_end(FR_PROF_GOAL_CONNECTION, st);
+    hipLaunchKernelGGL(k_connect_emit, dim3(L.grid_waves), b, 0, st, L.plans, L.ctl, L.fresh_q, L.pairs,
+                       L.pair_cap, L.counters);
     hipLaunchKernelGGL(k_connect_eval, dim3(L.grid_waves), b, 0, st, L.plans, L.ctl, L.fresh_q, L.pairs,
                        L.pair_cap, L.counters);
     hipLaunchKernelGGL(k_connect_inline, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);

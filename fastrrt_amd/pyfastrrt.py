@@ -251,19 +251,22 @@ class FastRRT:
         _lib.load().load_graph_state(self._ptr, filename.encode("utf-8"))
         _lib.check("load_graph_state")
 
-    def export_state(self):
-        return _export_state(_lib.load().fastrrt_export_state, self._ptr, self._w, self._h)
+    def export_state(self, goal=True):
+        """Grid planes as (H, W) arrays; goal=False leaves out the goal-connection
+        planes (exporting them finishes a lazily evaluated goal pass)."""
+        return _export_state(_lib.load().fastrrt_export_state, self._ptr, self._w, self._h, goal)
 
 
-def _export_state(fn, ptr, w, h):
+def _export_state(fn, ptr, w, h, goal=True):
     n = w * h
     out = {k: np.zeros(n, np.int32) for k in ("parent_x", "parent_z", "type", "child_count")}
     out.update({k: np.zeros(n, np.float32) for k in ("heading", "cost", "goal_heading", "goal_cost")})
     out["goal_flag"] = np.zeros(n, np.uint8)
     fn(ptr, *(out[k].ctypes.data_as(_lib.i32p) for k in ("parent_x", "parent_z", "type", "child_count")),
        out["heading"].ctypes.data_as(_lib.fp), out["cost"].ctypes.data_as(_lib.fp),
-       out["goal_flag"].ctypes.data_as(_lib.u8p), out["goal_heading"].ctypes.data_as(_lib.fp),
-       out["goal_cost"].ctypes.data_as(_lib.fp))
+       out["goal_flag"].ctypes.data_as(_lib.u8p) if goal else None,
+       out["goal_heading"].ctypes.data_as(_lib.fp) if goal else None,
+       out["goal_cost"].ctypes.data_as(_lib.fp) if goal else None)
     _lib.check("export_state")
     return {k: v.reshape(h, w) for k, v in out.items()}
 

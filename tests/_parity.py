@@ -100,7 +100,7 @@ class GpuPlanner:
         self.rrt.set_plan_data(sp)
 
     def state(self):
-        s = self.rrt.export_state()
+        s = self.rrt.export_state(goal=False)
         return {k: s[k] for k in STATE_KEYS}
 
     def goal_planes(self):

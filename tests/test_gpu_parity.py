@@ -25,12 +25,17 @@ SMALL = Params(min_dist=(10, 16), max_path=30.0, dist_tol=12.0)
 FULL = Params()
 
 
-def _run_side_by_side(sc, prm, seed, stream, max_iter, smart=False):
+def _run_side_by_side(sc, prm, seed, stream, max_iter, smart=False, goal_planes_first=False):
+    """Oracle and GPU planner in lock step, grids compared bit for bit after
+    every loop().  The handle evaluates the goal-connection planes lazily; they
+    are compared before the search (which finishes the pass up front) or after
+    it (the search ran on lazily evaluated cells)."""
     print(f"[parity] seed={seed} size={sc.width} cov={sc.coverage} smart={smart}", flush=True)
     o = OraclePlanner(sc, prm, seed, stream)
     g = GpuPlanner(sc, prm, seed, stream)
-    bad = diff_goal_planes(o.goal_planes(), g.goal_planes())
-    assert not bad, f"goal-connection planes differ: {bad}"
+    if goal_planes_first:
+        bad = diff_goal_planes(o.goal_planes(), g.goal_planes())
+        assert not bad, f"goal-connection planes differ: {bad}"
     o.search_init()
     g.search_init()
     assert o.goal_reached() == g.goal_reached()
@@ -44,13 +49,16 @@ def _run_side_by_side(sc, prm, seed, stream, max_iter, smart=False):
         assert go == gg, f"iteration {it}: loop() returned {go} (oracle) vs {gg} (gpu)"
         if not go or it >= max_iter:
             break
+    if not goal_planes_first:
+        bad = diff_goal_planes(o.goal_planes(), g.goal_planes())
+        assert not bad, f"goal-connection planes differ after the search: {bad}"
     return o, g, it
 
 
 @pytest.mark.parametrize("seed", [11, 12, 13, 14])
 def test_loop_parity_small(seed):
     sc = _small_scene(seed)
-    o, g, it = _run_side_by_side(sc, SMALL, 1234 + seed, seed, max_iter=60)
+    o, g, it = _run_side_by_side(sc, SMALL, 1234 + seed, seed, max_iter=60, goal_planes_first=seed % 2 == 0)
     assert o.goal_reached() == g.goal_reached()
     po, pg = o.path(), g.path()
     assert po.shape == pg.shape and (po.view(np.int32) == pg.view(np.int32)).all()
