@@ -759,78 +759,164 @@ __global__ void k_select(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, 
     }
 }
 
-/* expand_node (graph_expansion.cu:103-145) split in three so every rollout
- * reads the pre-expansion grid: k_expand runs the kinematics (one lane per
- * rollout) and defers the footprint tests that the centre-distance
- * certificate cannot settle; k_footprint runs them one wave per rollout and
- * claims the end cell; k_commit writes the winners. */
-__global__ void k_expand(const FrPlanDev *plans, FrCtl *ctl, FrExp *q, int2 *chk, long chk_cap,
-                         const int *counters)
+/* expand_node (graph_expansion.cu:103-145) with check_kinematic_new_path
+ * (kinematics.cu:123-231), one wave per rollout.  The step chain is walked
+ * 64 steps at a time: the heading and position recurrences are float sums
+ * whose rounding order matters, so they are accumulated step by step (every
+ * lane runs the uniform chain and keeps its own step's value); the per-step
+ * sincos, the cell of each step and the footprint tests are spread over the
+ * lanes.  A step's cell is new iff it differs from the previous step's cell
+ * (the reference's last_x / last_z: a repeated cell never changes it, a new
+ * in-grid cell becomes it, a new out-of-grid cell ends the walk).  The walk
+ * ends before step k when k > loop_max, or after the step that is out of the
+ * grid or that completes max_size cells; the end heading is the chain value
+ * after the last step executed.  Every rollout reads the pre-expansion grid;
+ * the end cell is claimed here and written by k_commit. */
+__global__ void k_expand(const FrPlanDev *plans, FrCtl *ctl, FrExp *q, const int *counters)
 {
     const int n = counters[FR_CNT_EXP];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        FrExp &e = q[i];
-        e.end = -1;
-        e.n_chk = 0;
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    const int lane = dev_lane();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int i = blockIdx.x * wpb + (int)(threadIdx.x >> 6); i < n; i += nw) {
+        const FrExp e = q[i];
         const FrPlanDev &P = plans[e.slot];
         const FrCtl &C = ctl[e.slot];
         const int src = e.src;
-        const int z = src / P.W, x = src - z * P.W;
+        const int W = P.W, H = P.H;
+        const int sz = src / W, sx = src - sz * W;
         float us, ul;
         fr_expansion_draws(P.seed, P.stream, (uint32_t)C.iter, (uint32_t)src, &us, &ul);
-        const float steer = (2.0f * us - 1.0f) * (float)P.max_steer_rad;
+        float steer = (2.0f * us - 1.0f) * (float)P.max_steer_rad;
         float psize = P.max_path * ul;
         if (psize < 5.0f)
             psize = 5.0f;
-        int ex, ez, nd = 0;
-        float eh, ec;
-        int2 *defer = ((long)i + 1) * FR_CHK_STRIDE <= chk_cap ? chk + (long)i * FR_CHK_STRIDE : nullptr;
-        if (!dev_rollout(P, x, z, steer, psize, &ex, &ez, &eh, &ec, defer, &nd))
+        /* kinematics.cu:150-180 */
+        float xf = P.og_y - (float)sz;
+        float yf = (float)sx - P.og_x;
+        if ((double)steer > P.max_steer_rad)
+            steer = (float)P.max_steer_rad;
+        else if ((double)steer < -P.max_steer_rad)
+            steer = (float)(-P.max_steer_rad);
+        const float t = fr_tanf(steer);
+        const float dt = (float)0.1;
+        const float ds = P.v * dt;
+        const float beta = fr_atanf(t / 2.0f);
+        const float hif = (float)((double)(ds * fr_cosf(beta) * t) / (2.0 * P.lr));
+        const int max_size = fr_trunc_f(psize) + 1;
+        const int loop_max = fr_trunc_f((float)(100 * max_size) / dt);
+        const double ogx = (double)P.og_x, ogy = (double)P.og_y;
+        float h = P.heading[src];
+        float node_cost = P.cost[src];
+        int size = 0, last_x = sx, last_z = sz, prev_x = sx, prev_z = sz;
+        float end_h = h;
+        bool ok = true;
+        for (int k0 = 0;; k0 += 64) {
+            const int k = k0 + lane;
+            /* heading before (hk) and after (hk1) this lane's step */
+            float hk = h, hh = h;
+            for (int j = 0; j < 64; ++j) {
+                hk = j == lane ? hh : hk;
+                hh = hh + hif;
+            }
+            const float hk1 = hk + hif;
+            float sn, cs;
+            fr_sincosf(hk + beta, &sn, &cs);
+            const float dx = ds * cs, dy = ds * sn;
+            float xk = xf, yk = yf, xx = xf, yy = yf;
+            for (int j = 0; j < 64; ++j) {
+                xx = xx + dev_bcast_f(dx, j);
+                yy = yy + dev_bcast_f(dy, j);
+                xk = j == lane ? xx : xk;
+                yk = j == lane ? yy : yk;
+            }
+            const int cx = fr_trunc_d(ogx + (double)yk);
+            const int cz = fr_trunc_d(ogy - (double)xk);
+            int px = __shfl_up(cx, 1, 64), pz = __shfl_up(cz, 1, 64);
+            if (lane == 0) {
+                px = prev_x;
+                pz = prev_z;
+            }
+            const bool fresh = !(cx == px && cz == pz);
+            const bool out = cx < 0 || cx >= W || cz < 0 || cz >= H;
+            const unsigned long long m_stop = __ballot(k > loop_max);
+            const unsigned long long m_oob = __ballot(fresh && out);
+            const unsigned long long m_in = __ballot(fresh && !out);
+            /* first lane at which the walk ends: before its step (loop cap),
+             * or after it (out of the grid, or max_size reached) */
+            const int t_before = m_stop ? __ffsll((long long)m_stop) - 1 : 64;
+            int t_at = m_oob ? __ffsll((long long)m_oob) - 1 : 64;
+            const int need = max_size - size;
+            if (__popcll(m_in) >= need) {
+                unsigned long long mm = m_in;
+                for (int r = 1; r < need; ++r)
+                    mm &= mm - 1;
+                const int t_reach = __ffsll((long long)mm) - 1;
+                t_at = t_reach < t_at ? t_reach : t_at;
+            }
+            const bool ends_before = t_before <= t_at && t_before < 64;
+            const bool ends_at = !ends_before && t_at < 64;
+            const int lim = ends_before ? t_before : (ends_at ? t_at + 1 : 64);
+            const unsigned long long acc = m_in & (lim >= 64 ? ~0ull : ((1ull << lim) - 1ull));
+            /* cost in step order, then the footprints (any failure rejects) */
+            float add = 0;
+            bool need_fp = false;
+            if ((acc >> lane) & 1ull) {
+                add = (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
+                need_fp = !dev_fp_certified(P, cx, cz, hk1);
+            }
+            for (unsigned long long cm = acc; cm; cm &= cm - 1)
+                node_cost += dev_bcast_f(add, __ffsll((long long)cm) - 1);
+            for (unsigned long long fm = __ballot(need_fp); fm; fm &= fm - 1) {
+                const int l = __ffsll((long long)fm) - 1;
+                if (!dev_wave_feasible(P, dev_bcast_i(cx, l), dev_bcast_i(cz, l), dev_bcast_f(hk1, l))) {
+                    ok = false;
+                    break;
+                }
+            }
+            if (!ok)
+                break;
+            size += __popcll(acc);
+            if (acc) {
+                const int l = 63 - __clzll((long long)acc);
+                last_x = dev_bcast_i(cx, l);
+                last_z = dev_bcast_i(cz, l);
+            }
+            if (ends_before) {
+                end_h = dev_bcast_f(hk, t_before);
+                break;
+            }
+            if (ends_at) {
+                end_h = dev_bcast_f(hk1, t_at);
+                break;
+            }
+            h = hh;
+            xf = xx;
+            yf = yy;
+            prev_x = dev_bcast_i(cx, 63);
+            prev_z = dev_bcast_i(cz, 63);
+        }
+        if (lane != 0)
             continue;
-        if ((float)ex == P.og_x && (float)ez == P.og_y)
+        FrExp &o = q[i];
+        o.end = -1;
+        o.n_chk = 0;
+        if (!ok || size < 1)
             continue;
-        const int epos = ez * P.W + ex;
+        if ((float)last_x == P.og_x && (float)last_z == P.og_y)
+            continue;
+        const int epos = last_z * W + last_x;
         if (epos == src)
             continue;
         const int st = P.type[epos];
         if (st != FR_TYPE_NULL && st != FR_TYPE_NODE)
             continue;
-        e.end = epos;
-        e.heading = eh;
-        e.cost = ec;
-        e.n_chk = nd;
-    }
-}
-
-__global__ void k_footprint(const FrPlanDev *plans, const FrCtl *ctl, FrExp *q, const int2 *chk,
-                            const int *counters)
-{
-    const int n = counters[FR_CNT_EXP];
-    const int wpb = blockDim.x >> 6;
-    const int nw = gridDim.x * wpb;
-    for (int i = blockIdx.x * wpb + (int)(threadIdx.x >> 6); i < n; i += nw) {
-        const FrExp e = q[i];
-        if (e.end < 0)
-            continue;
-        const FrPlanDev &P = plans[e.slot];
-        bool ok = true;
-        for (int k = 0; k < e.n_chk; ++k) {
-            const int2 c = chk[(long)i * FR_CHK_STRIDE + k];
-            const int cz = c.x / P.W, cx = c.x - cz * P.W;
-            if (!dev_wave_feasible(P, cx, cz, __int_as_float(c.y))) {
-                ok = false;
-                break;
-            }
-        }
-        if (dev_lane() == 0) {
-            if (!ok) {
-                q[i].end = -1;
-            } else {
-                const unsigned long long key =
-                    ((unsigned long long)(0xFFFFFFFFu - ctl[e.slot].epoch) << 32) | (unsigned)e.src;
-                atomicMin(&P.claim[e.end], key);
-            }
-        }
+        o.end = epos;
+        o.heading = end_h;
+        o.cost = node_cost;
+        const unsigned long long key = ((unsigned long long)(0xFFFFFFFFu - C.epoch) << 32) | (unsigned)src;
+        atomicMin(&P.claim[epos], key);
     }
 }
 
@@ -1826,9 +1912,7 @@ void fr_launch_iteration(const FrLaunch &L, hipStream_t st)
     hipLaunchKernelGGL(k_select, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles, L.exp_q,
                        L.counters);
     fr_prof_begin(FR_PROF_EXPAND, st);
-    hipLaunchKernelGGL(k_expand, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.exp_q, L.chk, L.chk_cap,
-                       L.counters);
-    hipLaunchKernelGGL(k_footprint, dim3(L.grid_waves), b, 0, st, L.plans, L.ctl, L.exp_q, L.chk, L.counters);
+    hipLaunchKernelGGL(k_expand, dim3(L.grid_waves), b, 0, st, L.plans, L.ctl, L.exp_q, L.counters);
     fr_prof_end(FR_PROF_EXPAND, st);
     hipLaunchKernelGGL(k_commit, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.exp_q, L.counters);
     hipLaunchKernelGGL(k_density, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles);
