@@ -48,7 +48,7 @@ class PlanSpec:
 
 
 class Result:
-    __slots__ = ("tag", "status", "iterations", "path", "submit_ms", "done_ms")
+    __slots__ = ("tag", "status", "iterations", "expansions", "path", "submit_ms", "done_ms")
 
     @property
     def ok(self):
@@ -96,7 +96,7 @@ class Batch:
         out = []
         while L.fastrrt_batch_poll(self._ptr, ctypes.byref(res), self._buf.ctypes.data_as(_lib.fp), self._cap):
             r = Result()
-            r.tag, r.status, r.iterations = res.user_tag, res.status, res.iterations
+            r.tag, r.status, r.iterations, r.expansions = res.user_tag, res.status, res.iterations, res.expansions
             n = res.path_len
             r.path = self._buf[:3 * n].reshape(n, 3).copy() if n > 0 else np.zeros((0, 3), np.float32)
             r.submit_ms, r.done_ms = res.submit_ms, res.done_ms

@@ -74,10 +74,7 @@ class FrBatch {
     ~FrBatch()
     {
         (void)hipStreamSynchronize(eng_.stream());
-        if (graph_exec_)
-            (void)hipGraphExecDestroy(graph_exec_);
-        if (graph_)
-            (void)hipGraphDestroy(graph_);
+        drop_graphs();
         if (h_ctl_)
             (void)hipHostFree(h_ctl_);
         if (d_list_)
@@ -268,20 +265,64 @@ class FrBatch {
         eng_.set_active(act); /* synchronises the stream */
     }
 
+    /* One loop iteration of every running plan: one replay of the captured
+     * iteration graph.  With kernel timing on, the iteration is captured as
+     * three graphs (before k_expand, k_expand, after) and k_expand is timed
+     * with events recorded on the stream between their replays. */
     void launch_iteration()
     {
-        if (!graph_exec_) {
-            const bool prof = fr_prof_on();
-            fr_prof_enable(false); /* no event nodes inside the captured graph */
-            FR_CHECK(hipStreamBeginCapture(eng_.stream(), hipStreamCaptureModeThreadLocal));
-            fr_launch_iteration(eng_.launch(), eng_.stream());
-            FR_CHECK(hipStreamEndCapture(eng_.stream(), &graph_));
-            fr_prof_enable(prof);
-            FR_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+        const bool prof = fr_prof_on();
+        if (!graph_[0] || prof != graph_timed_)
+            capture(prof);
+        hipStream_t st = eng_.stream();
+        fr_prof_begin(FR_PROF_ITERATION, st);
+        if (!graph_timed_) {
+            FR_CHECK(hipGraphLaunch(exec_[0], st));
+        } else {
+            FR_CHECK(hipGraphLaunch(exec_[0], st));
+            fr_prof_begin(FR_PROF_EXPAND, st);
+            FR_CHECK(hipGraphLaunch(exec_[1], st));
+            fr_prof_end(FR_PROF_EXPAND, st);
+            FR_CHECK(hipGraphLaunch(exec_[2], st));
         }
-        fr_prof_begin(FR_PROF_ITERATION, eng_.stream());
-        FR_CHECK(hipGraphLaunch(graph_exec_, eng_.stream()));
-        fr_prof_end(FR_PROF_ITERATION, eng_.stream());
+        fr_prof_end(FR_PROF_ITERATION, st);
+    }
+
+    void capture(bool timed)
+    {
+        drop_graphs();
+        hipStream_t st = eng_.stream();
+        const FrLaunch &L = eng_.launch();
+        const bool prof = fr_prof_on();
+        fr_prof_enable(false); /* no event nodes inside the graphs */
+        for (int part = 0; part < (timed ? 3 : 1); ++part) {
+            FR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            if (!timed) {
+                fr_launch_iteration(L, st);
+            } else if (part == 0) {
+                fr_launch_iteration_pre(L, st);
+            } else if (part == 1) {
+                fr_launch_iteration_expand(L, st);
+            } else {
+                fr_launch_iteration_post(L, st);
+            }
+            FR_CHECK(hipStreamEndCapture(st, &graph_[part]));
+            FR_CHECK(hipGraphInstantiate(&exec_[part], graph_[part], nullptr, nullptr, 0));
+        }
+        fr_prof_enable(prof);
+        graph_timed_ = timed;
+    }
+
+    void drop_graphs()
+    {
+        for (int i = 0; i < 3; ++i) {
+            if (exec_[i])
+                (void)hipGraphExecDestroy(exec_[i]);
+            if (graph_[i])
+                (void)hipGraphDestroy(graph_[i]);
+            exec_[i] = nullptr;
+            graph_[i] = nullptr;
+        }
     }
 
     void poll_and_retire()
@@ -292,15 +333,24 @@ class FrBatch {
         FR_CHECK(hipStreamSynchronize(eng_.stream()));
         std::vector<int> found, failed;
         const double t = now_ms() - t0_;
+        /* time between polls: a plan whose deadline falls before the next poll
+         * is retired now (the timeout of fastrrt.cpp:110-114, per request, as
+         * a completion deadline) */
+        if (last_poll_ >= 0) {
+            const double d = t - last_poll_;
+            poll_ema_ = poll_ema_ > 0 ? 0.75 * poll_ema_ + 0.25 * d : d;
+        }
+        last_poll_ = t;
         for (int i = 0; i < ns; ++i) {
             if (!slots_[i].busy)
                 continue;
+            const double dl = slots_[i].p.req.deadline_ms;
             if (h_ctl_[i].status == FR_FOUND)
                 found.push_back(i);
             else if (h_ctl_[i].status == FR_FAILED)
                 failed.push_back(i);
-            else if (slots_[i].p.req.deadline_ms > 0 && t - slots_[i].p.t_submit > slots_[i].p.req.deadline_ms)
-                failed.push_back(i); /* the timeout of fastrrt.cpp:110-114, per request */
+            else if (dl > 0 && t - slots_[i].p.t_submit + poll_ema_ > dl)
+                failed.push_back(i);
         }
         if (found.empty() && failed.empty())
             return;
@@ -376,6 +426,7 @@ class FrBatch {
         f.res.user_tag = sl.p.req.user_tag;
         f.res.status = status;
         f.res.iterations = h_ctl_[s].iter;
+        f.res.expansions = h_ctl_[s].total_expansions;
         f.res.path_len = len;
         f.res.submit_ms = sl.p.t_submit;
         f.res.done_ms = now_ms() - t0_;
@@ -385,9 +436,9 @@ class FrBatch {
         FrCtl idle{};
         idle.status = FR_IDLE;
         std::memcpy(&h_ctl_[s], &idle, sizeof(FrCtl));
+        /* the stream is synchronised by rebuild_active() before h_ctl_ is reused */
         FR_CHECK(hipMemcpyAsync(eng_.launch().ctl + s, &h_ctl_[s], sizeof(FrCtl), hipMemcpyHostToDevice,
                                 eng_.stream()));
-        FR_CHECK(hipStreamSynchronize(eng_.stream()));
         give_back(std::move(sl.slab));
         cells_in_use_ -= sl.cells;
         sl.busy = false;
@@ -408,8 +459,10 @@ class FrBatch {
     int next_ticket_ = 0;
     int max_path_len_ = 1024;
     double t0_ = 0;
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t graph_exec_ = nullptr;
+    hipGraph_t graph_[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t exec_[3] = {nullptr, nullptr, nullptr};
+    bool graph_timed_ = false;
+    double last_poll_ = -1, poll_ema_ = 0;
 };
 
 #define FR_TRY(ret_on_error, body)                                                                        \

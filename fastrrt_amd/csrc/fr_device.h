@@ -105,6 +105,7 @@ struct FrCtl {
     int opt_limit;            /* batch finalisation: max successful rounds */
     unsigned long long opt_key;
     int full_check;           /* next connect pass must scan all NODE cells */
+    int total_expansions;     /* rollouts since the search started */
     int opt_active;           /* batch finalisation: optimisation rounds pending */
 };
 

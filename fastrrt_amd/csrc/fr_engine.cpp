@@ -97,6 +97,13 @@ void fr_prof_read(int k, long *launches, double *total_ms)
     *launches = p.launches[k];
     *total_ms = p.total_ms[k];
 }
+void fr_prof_add(int k, double ms)
+{
+    ProfState &p = prof();
+    std::lock_guard<std::mutex> g(p.mu);
+    p.launches[k] += 1;
+    p.total_ms[k] += ms;
+}
 void fr_prof_reset()
 {
     ProfState &p = prof();

@@ -53,6 +53,10 @@ void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, in
 /* evaluate the goal-connection planes of every cell not yet evaluated */
 void fr_launch_goal_complete(const FrLaunch &L, const int *d_slots, int n, int max_N, hipStream_t st);
 void fr_launch_iteration(const FrLaunch &L, hipStream_t st);
+/* the same iteration in three parts: before, k_expand, after */
+void fr_launch_iteration_pre(const FrLaunch &L, hipStream_t st);
+void fr_launch_iteration_expand(const FrLaunch &L, hipStream_t st);
+void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st);
 void fr_launch_clear(const FrLaunch &L, int slot, int N, hipStream_t st);
 void fr_launch_count(const FrLaunch &L, int slot, int N, int *d_out, hipStream_t st);
 void fr_launch_goal_scan(const FrLaunch &L, int slot, int N, int gx, int gz, float gh, float tol, float herr,
@@ -88,6 +92,8 @@ void fr_prof_end(int k, hipStream_t st);
 /* synchronises the recorded events; returns launches and total ms */
 void fr_prof_read(int k, long *launches, double *total_ms);
 void fr_prof_reset();
+/* one launch of kernel k timed elsewhere (events inside a captured graph) */
+void fr_prof_add(int k, double ms);
 
 /* error plumbing: every HIP call of the library goes through FR_CHECK */
 void fr_set_error(const std::string &msg);

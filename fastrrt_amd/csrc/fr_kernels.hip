@@ -302,6 +302,124 @@ __device__ bool dev_wave_feasible(const FrPlanDev &P, int x, int z, float h)
     return true;
 }
 
+/* Footprint tests along a sequence of nearby poses (consecutive points of a
+ * curve or of a rollout), one wave.  Each lane keeps, for its samples
+ * k = lane + 64 r, the value A_at + D(p) where p is the sample's pixel at the
+ * last pose it was computed for and A_at the accumulated displacement then.
+ * A is a bound on how far any sample pixel can have moved since tracking
+ * began: between two poses a sample (i, j) moves by at most
+ * mx |dc| + mz |ds| + |dx| (resp. mx |ds| + mz |dc| + |dz|) before rounding,
+ * and truncation adds at most one pixel.  By the triangle inequality of the
+ * Chebyshev distance, a sample with A < A_at + D(p) is on a free pixel; only
+ * the others are recomputed (the reference's expressions) and loaded.  The
+ * result is the reference's boolean for every pose. */
+#define FR_TRACK_R 16
+
+/* per lane: lanemin = the smallest limit of its samples; the limits
+ * themselves live in LDS (FR_TRACK_R words per lane, lane-contiguous), read
+ * only when a sample's certificate has run out */
+struct FrTrack {
+    int A;
+    int lanemin;
+    int has_ref;
+    int x, z;
+    float s, c;
+    float margin;
+};
+
+__device__ __forceinline__ void dev_track_init(const FrPlanDev &P, FrTrack &T)
+{
+    T.A = 0;
+    T.lanemin = 0;
+    T.has_ref = 0;
+    T.x = T.z = 0;
+    T.s = T.c = 0;
+    /* float rounding of the sample coordinates (a few ulp of the largest) */
+    T.margin = 1e-3f + 1e-6f * (float)(P.W + P.H + 2 * P.pad);
+}
+
+__device__ __forceinline__ bool dev_track_usable(const FrPlanDev &P)
+{
+    return (2 * P.mx + 1) * (2 * P.mz + 1) <= 64 * FR_TRACK_R;
+}
+
+/* s, c = fr_sincosf(h) of a finite heading; all lanes, uniform arguments;
+ * blocks of FR_BLOCK threads */
+__device__ __noinline__ bool dev_track_feasible(const FrPlanDev &P, FrTrack &T, int x, int z, float s, float c)
+{
+    __shared__ int s_lim[FR_BLOCK / 64][FR_TRACK_R][64];
+    const int mx = P.mx, mz = P.mz;
+    const bool first = !T.has_ref;
+    if (!first) {
+        const double dc = fabs((double)c - (double)T.c), dsn = fabs((double)s - (double)T.s);
+        const double bx = (double)mx * dc + (double)mz * dsn + fabs((double)(x - T.x));
+        const double bz = (double)mx * dsn + (double)mz * dc + fabs((double)(z - T.z));
+        const double b = (bx > bz ? bx : bz) + (double)T.margin;
+        T.A += (int)floor(b) + 1;
+    }
+    T.x = x;
+    T.z = z;
+    T.s = s;
+    T.c = c;
+    T.has_ref = 1;
+    const bool need = first || T.A >= T.lanemin;
+    if (!__ballot(need))
+        return true;
+    bool bad = false;
+    if (need) {
+        int *lim = &s_lim[threadIdx.x >> 6][0][0];
+        const uint8_t *D = P.dist;
+        const int PW = P.PW, pad = P.pad;
+        const int nx = 2 * mx + 1, ns = nx * (2 * mz + 1);
+        const int lane = dev_lane();
+        int i = lane / nx, j = lane - (lane / nx) * nx;
+        const int di = 64 / nx, dj = 64 - di * nx;
+        const float fxc = (float)x, fzc = (float)z;
+        int mn = 0x7FFFFFFF;
+        for (int r = 0; r < FR_TRACK_R; ++r) {
+            if (lane + 64 * r < ns) {
+                int l = first ? 0 : lim[r * 64 + lane];
+                if (first || T.A >= l) {
+                    const float is = (float)(i - mz) * s;
+                    const float ic = (float)(i - mz) * c;
+                    const float fj = (float)(j - mx);
+                    const float fx = fj * c - is + fxc;
+                    const float fz = fj * s + ic + fzc;
+                    const int xl = (int)fx;
+                    const int zl = (int)fz;
+                    const int d = D[(zl + pad) * PW + (xl + pad)];
+                    bad |= d == 0;
+                    l = T.A + d;
+                    lim[r * 64 + lane] = l;
+                }
+                mn = l < mn ? l : mn;
+            }
+            i += di;
+            j += dj;
+            if (j >= nx) {
+                j -= nx;
+                i += 1;
+            }
+        }
+        T.lanemin = mn;
+    }
+    return !__ballot(bad);
+}
+
+/* dev_wave_feasible through a tracker (falls back to the full test for
+ * footprints above 64 * FR_TRACK_R samples) */
+__device__ __forceinline__ bool dev_track_check(const FrPlanDev &P, FrTrack &T, int x, int z, float h, float s,
+                                                float c)
+{
+    if (P.trav[z * P.W + x] & 1)
+        return true;
+    if (c != c || s != s)
+        return P.dist[P.pad * P.PW + P.pad] != 0;
+    if (!dev_track_usable(P))
+        return dev_wave_feasible(P, x, z, h);
+    return dev_track_feasible(P, T, x, z, s, c);
+}
+
 /* One Hermite point of graph_direct_goal_connection.cu:16-167: position,
  * in-bounds cell, first derivative. */
 struct FrHermite {
@@ -365,6 +483,8 @@ __device__ float dev_wave_check_direct(const FrPlanDev &P, float max_curv, int x
     const bool skip_fp = safe_zone && ((P.trav[pos] & 2) != 0);
     float node_cost = base;
     int last_x = -1, last_z = -1;
+    FrTrack trk;
+    dev_track_init(P, trk);
     for (int pass = 0; pass < 2; ++pass) {
         last_x = -1;
         last_z = -1;
@@ -423,20 +543,23 @@ __device__ float dev_wave_check_direct(const FrPlanDev &P, float max_curv, int x
                 if (__ballot(bad))
                     return -1;
             } else {
-                float add = 0, hd = 0;
+                float add = 0, hd = 0, hs = 0, hc = 0;
                 bool need = false;
                 if (acc) {
                     add = (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
                     if (!skip_fp) {
                         hd = (float)((double)fr_atan2f((float)ddz, (float)ddx) + FR_HALF_PI_D);
                         need = !dev_fp_certified(P, cx, cz, hd);
+                        if (need)
+                            fr_sincosf(hd, &hs, &hc);
                     }
                 }
                 unsigned long long nm = __ballot(need);
                 while (nm) {
                     const int l = __ffsll((long long)nm) - 1;
                     nm &= nm - 1;
-                    if (!dev_wave_feasible(P, dev_bcast_i(cx, l), dev_bcast_i(cz, l), dev_bcast_f(hd, l)))
+                    if (!dev_track_check(P, trk, dev_bcast_i(cx, l), dev_bcast_i(cz, l), dev_bcast_f(hd, l),
+                                         dev_bcast_f(hs, l), dev_bcast_f(hc, l)))
                         return -1;
                 }
                 unsigned long long cm = am;
@@ -532,7 +655,7 @@ __device__ int block_exclusive_scan(int v, int *out_prefix)
  * pixel whose class cost is negative (path_feasible.cu:472-475); pixels in
  * the ego box are skipped by the reference (:469-470) and count as free; a
  * class id outside the table counts as an obstacle. */
-__global__ void k_setup_prep(const FrPlanDev *plans, const int *slots)
+__global__ void __launch_bounds__(FR_BLOCK) k_setup_prep(const FrPlanDev *plans, const int *slots)
 {
     const FrPlanDev &P = plans[slots[blockIdx.y]];
     for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
@@ -550,7 +673,7 @@ __global__ void k_setup_prep(const FrPlanDev *plans, const int *slots)
 
 /* Row pass of the padded Chebyshev distance plane: horizontal distance to the
  * nearest obstacle pixel of the same row (capped), one lane per pixel. */
-__global__ void k_setup_rowdist(const FrPlanDev *plans, const int *slots)
+__global__ void __launch_bounds__(FR_BLOCK) k_setup_rowdist(const FrPlanDev *plans, const int *slots)
 {
     const FrPlanDev &P = plans[slots[blockIdx.y]];
     const long total = (long)P.PW * P.PH;
@@ -574,7 +697,7 @@ __global__ void k_setup_rowdist(const FrPlanDev *plans, const int *slots)
 }
 
 /* Column pass: D(x, z) = min_dz max(|dz|, rowdist(x, z + dz)), capped. */
-__global__ void k_setup_coldist(const FrPlanDev *plans, const int *slots)
+__global__ void __launch_bounds__(FR_BLOCK) k_setup_coldist(const FrPlanDev *plans, const int *slots)
 {
     const FrPlanDev &P = plans[slots[blockIdx.y]];
     const long total = (long)P.PW * P.PH;
@@ -620,7 +743,7 @@ __device__ void dev_gc_cell(const FrPlanDev &P, int pos)
 }
 
 /* snapshot of the cost plane, flags cleared, nothing evaluated */
-__global__ void k_setup_gc_reset(const FrPlanDev *plans, const int *slots)
+__global__ void __launch_bounds__(FR_BLOCK) k_setup_gc_reset(const FrPlanDev *plans, const int *slots)
 {
     const FrPlanDev &P = plans[slots[blockIdx.y]];
     for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
@@ -632,7 +755,7 @@ __global__ void k_setup_gc_reset(const FrPlanDev *plans, const int *slots)
 }
 
 /* every cell (skip_done: every cell not yet evaluated), one wave per cell */
-__global__ void k_setup_goal_connection(const FrPlanDev *plans, const int *slots, int skip_done)
+__global__ void __launch_bounds__(FR_BLOCK) k_setup_goal_connection(const FrPlanDev *plans, const int *slots, int skip_done)
 {
     const FrPlanDev &P = plans[slots[blockIdx.y]];
     const int wpb = blockDim.x >> 6;
@@ -648,7 +771,7 @@ __global__ void k_setup_goal_connection(const FrPlanDev *plans, const int *slots
 /* loop iteration                                                           */
 /* ------------------------------------------------------------------------ */
 
-__global__ void k_begin(const FrPlanDev *plans, FrCtl *ctl, const int *slots, const int *n_slots_p, int *counters)
+__global__ void __launch_bounds__(FR_BLOCK) k_begin(const FrPlanDev *plans, FrCtl *ctl, const int *slots, const int *n_slots_p, int *counters)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n_slots = *n_slots_p;
@@ -690,7 +813,7 @@ __global__ void k_begin(const FrPlanDev *plans, FrCtl *ctl, const int *slots, co
 /* Gate of __CUDA_random_node_expansion (graph_expansion.cu:157-164) and of
  * __CUDA_smart_node_expansion (graph_expansion_search_zones.cu:461-473):
  * queue one rollout per eligible NODE cell. */
-__global__ void k_select(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, const int *n_tiles_p,
+__global__ void __launch_bounds__(FR_BLOCK) k_select(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, const int *n_tiles_p,
                          FrExp *q, int *counters)
 {
     const int n_tiles = *n_tiles_p;
@@ -772,7 +895,7 @@ __global__ void k_select(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, 
  * grid or that completes max_size cells; the end heading is the chain value
  * after the last step executed.  Every rollout reads the pre-expansion grid;
  * the end cell is claimed here and written by k_commit. */
-__global__ void k_expand(const FrPlanDev *plans, FrCtl *ctl, FrExp *q, const int *counters)
+__global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *plans, FrCtl *ctl, FrExp *q, const int *counters)
 {
     const int n = counters[FR_CNT_EXP];
     const int wpb = blockDim.x >> 6;
@@ -812,6 +935,8 @@ __global__ void k_expand(const FrPlanDev *plans, FrCtl *ctl, FrExp *q, const int
         int size = 0, last_x = sx, last_z = sz, prev_x = sx, prev_z = sz;
         float end_h = h;
         bool ok = true;
+        FrTrack trk;
+        dev_track_init(P, trk);
         for (int k0 = 0;; k0 += 64) {
             const int k = k0 + lane;
             /* heading before (hk) and after (hk1) this lane's step */
@@ -860,17 +985,20 @@ __global__ void k_expand(const FrPlanDev *plans, FrCtl *ctl, FrExp *q, const int
             const int lim = ends_before ? t_before : (ends_at ? t_at + 1 : 64);
             const unsigned long long acc = m_in & (lim >= 64 ? ~0ull : ((1ull << lim) - 1ull));
             /* cost in step order, then the footprints (any failure rejects) */
-            float add = 0;
+            float add = 0, hs = 0, hc = 0;
             bool need_fp = false;
             if ((acc >> lane) & 1ull) {
                 add = (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
                 need_fp = !dev_fp_certified(P, cx, cz, hk1);
+                if (need_fp)
+                    fr_sincosf(hk1, &hs, &hc);
             }
             for (unsigned long long cm = acc; cm; cm &= cm - 1)
                 node_cost += dev_bcast_f(add, __ffsll((long long)cm) - 1);
             for (unsigned long long fm = __ballot(need_fp); fm; fm &= fm - 1) {
                 const int l = __ffsll((long long)fm) - 1;
-                if (!dev_wave_feasible(P, dev_bcast_i(cx, l), dev_bcast_i(cz, l), dev_bcast_f(hk1, l))) {
+                if (!dev_track_check(P, trk, dev_bcast_i(cx, l), dev_bcast_i(cz, l), dev_bcast_f(hk1, l),
+                                     dev_bcast_f(hs, l), dev_bcast_f(hc, l))) {
                     ok = false;
                     break;
                 }
@@ -920,7 +1048,7 @@ __global__ void k_expand(const FrPlanDev *plans, FrCtl *ctl, FrExp *q, const int
     }
 }
 
-__global__ void k_commit(const FrPlanDev *plans, FrCtl *ctl, const FrExp *q, const int *counters)
+__global__ void __launch_bounds__(FR_BLOCK) k_commit(const FrPlanDev *plans, FrCtl *ctl, const FrExp *q, const int *counters)
 {
     const int n = counters[FR_CNT_EXP];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -951,7 +1079,7 @@ __global__ void k_commit(const FrPlanDev *plans, FrCtl *ctl, const FrExp *q, con
 }
 
 /* __CUDA_count_nodes_in_density_region (graph_expansion_search_zones.cu:42-66) */
-__global__ void k_density(const FrPlanDev *plans, const FrCtl *ctl, const int2 *tiles, const int *n_tiles_p)
+__global__ void __launch_bounds__(FR_BLOCK) k_density(const FrPlanDev *plans, const FrCtl *ctl, const int2 *tiles, const int *n_tiles_p)
 {
     const int n_tiles = *n_tiles_p;
     for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
@@ -993,7 +1121,7 @@ __device__ __forceinline__ bool dev_solve_runs(const FrCtl &C)
 
 /* __CUDA_solveGraphCollision_erase_trees (graph_collision.cu:33-83), deciding
  * on the pre-erase grid: marks only. */
-__global__ void k_erase_mark(const FrPlanDev *plans, const FrCtl *ctl, const int2 *tiles, const int *n_tiles_p)
+__global__ void __launch_bounds__(FR_BLOCK) k_erase_mark(const FrPlanDev *plans, const FrCtl *ctl, const int2 *tiles, const int *n_tiles_p)
 {
     const int n_tiles = *n_tiles_p;
     for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
@@ -1034,7 +1162,7 @@ __global__ void k_erase_mark(const FrPlanDev *plans, const FrCtl *ctl, const int
  * (graph_collision.cu:85-104, first half: COLLISION -> NODE, count 0).  The
  * converted nodes are queued: they need their parents' counts incremented
  * and a goal test. */
-__global__ void k_erase_apply(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, const int *n_tiles_p,
+__global__ void __launch_bounds__(FR_BLOCK) k_erase_apply(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, const int *n_tiles_p,
                               FrFresh *fq, int *counters)
 {
     const int n_tiles = *n_tiles_p;
@@ -1085,7 +1213,7 @@ __global__ void k_erase_apply(const FrPlanDev *plans, FrCtl *ctl, const int2 *ti
 }
 
 /* second half of set_nodes: the parents of converted nodes gain a child */
-__global__ void k_coll_parents(const FrPlanDev *plans, const FrFresh *fq, const int *counters)
+__global__ void __launch_bounds__(FR_BLOCK) k_coll_parents(const FrPlanDev *plans, const FrFresh *fq, const int *counters)
 {
     const int n = counters[FR_CNT_FRESH];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1103,7 +1231,7 @@ __global__ void k_coll_parents(const FrPlanDev *plans, const FrFresh *fq, const 
  * (fastrrt.cpp:130-142, graph_expansion.cu:36-65).  Accepted nodes, the
  * re-added start and (after external edits) every node are queued for the
  * goal test and the goal connection search. */
-__global__ void k_accept(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, const int *n_tiles_p,
+__global__ void __launch_bounds__(FR_BLOCK) k_accept(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, const int *n_tiles_p,
                          FrFresh *fq, int *counters)
 {
     const int n_tiles = *n_tiles_p;
@@ -1245,7 +1373,7 @@ __device__ __forceinline__ long long dev_pair_lcost(const FrPlanDev &P, int node
 
 /* Cells of the new nodes' windows whose goal-connection planes are not yet
  * evaluated (lazy plans): claimed with one atomicOr per cell, queued. */
-__global__ void k_gc_demand(const FrPlanDev *plans, const FrFresh *fq, int2 *dq, long dq_cap, int *counters)
+__global__ void __launch_bounds__(FR_BLOCK) k_gc_demand(const FrPlanDev *plans, const FrFresh *fq, int2 *dq, long dq_cap, int *counters)
 {
     const int n = counters[FR_CNT_FRESH];
     const int wpb = blockDim.x >> 6;
@@ -1287,7 +1415,7 @@ __global__ void k_gc_demand(const FrPlanDev *plans, const FrFresh *fq, int2 *dq,
     }
 }
 
-__global__ void k_gc_eval(const FrPlanDev *plans, const int2 *dq, long dq_cap, const int *counters)
+__global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *plans, const int2 *dq, long dq_cap, const int *counters)
 {
     long n = counters[FR_CNT_DEMAND];
     if (n > dq_cap)
@@ -1302,7 +1430,7 @@ __global__ void k_gc_eval(const FrPlanDev *plans, const int2 *dq, long dq_cap, c
 
 /* Goal test of the new nodes (graph_best_node.cu:182-222), window counts and
  * the pair queue; one wave per node, pairs in the window's row-major order. */
-__global__ void k_connect_emit(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, FrPair *pairs, long pair_cap,
+__global__ void __launch_bounds__(FR_BLOCK) k_connect_emit(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, FrPair *pairs, long pair_cap,
                                int *counters)
 {
     const int n = counters[FR_CNT_FRESH];
@@ -1372,7 +1500,7 @@ __global__ void k_connect_emit(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, 
 }
 
 /* one wave per (node, flagged cell) pair */
-__global__ void k_connect_eval(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, const FrPair *pairs, long pair_cap,
+__global__ void __launch_bounds__(FR_BLOCK) k_connect_eval(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, const FrPair *pairs, long pair_cap,
                                const int *counters)
 {
     long n = counters[FR_CNT_PAIRS];
@@ -1400,7 +1528,7 @@ __global__ void k_connect_eval(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, 
 }
 
 /* nodes whose pairs did not fit the queue scan their window in-thread */
-__global__ void k_connect_inline(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, const int *counters)
+__global__ void __launch_bounds__(FR_BLOCK) k_connect_inline(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, const int *counters)
 {
     const int n = counters[FR_CNT_FRESH];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1437,7 +1565,7 @@ __global__ void k_connect_inline(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq
     }
 }
 
-__global__ void k_connect_pick(const FrPlanDev *plans, FrCtl *ctl, const FrFresh *fq, const int *counters)
+__global__ void __launch_bounds__(FR_BLOCK) k_connect_pick(const FrPlanDev *plans, FrCtl *ctl, const FrFresh *fq, const int *counters)
 {
     const int n = counters[FR_CNT_FRESH];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1483,7 +1611,7 @@ __device__ int dev_wave_window_cell(const FrPlanDev &P, int node, int rank)
 /* End of FastRRT::loop (fastrrt.cpp:130-180): reseed, goal-connection adds,
  * goal test, iteration bookkeeping.  One wave per plan: the lanes share the
  * winner's Hermite check, lane 0 writes. */
-__global__ void k_finish(const FrPlanDev *plans, FrCtl *ctl, const int *slots, const int *n_slots_p,
+__global__ void __launch_bounds__(FR_BLOCK) k_finish(const FrPlanDev *plans, FrCtl *ctl, const int *slots, const int *n_slots_p,
                          const FrFresh *fq)
 {
     const int i = blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
@@ -1515,6 +1643,7 @@ __global__ void k_finish(const FrPlanDev *plans, FrCtl *ctl, const int *slots, c
         return;
     C.last_temp = C.n_temp;
     C.iter += 1;
+    C.total_expansions += C.n_expansions;
     if (reseed) {
         dev_add(P, P.sx, P.sz, P.sh, -1, -1, 0.0f);
         C.reseeded = 1;
@@ -1549,7 +1678,7 @@ __global__ void k_finish(const FrPlanDev *plans, FrCtl *ctl, const int *slots, c
 /* ------------------------------------------------------------------------ */
 
 /* graph.cu:400-414: type <- NULL everywhere */
-__global__ void k_clear(const FrPlanDev *plans, int slot)
+__global__ void __launch_bounds__(FR_BLOCK) k_clear(const FrPlanDev *plans, int slot)
 {
     const FrPlanDev &P = plans[slot];
     for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x)
@@ -1557,7 +1686,7 @@ __global__ void k_clear(const FrPlanDev *plans, int slot)
 }
 
 /* graph_list.cu:8-66: out[0..5] = count by type */
-__global__ void k_count_types(const FrPlanDev *plans, int slot, int *out)
+__global__ void __launch_bounds__(FR_BLOCK) k_count_types(const FrPlanDev *plans, int slot, int *out)
 {
     const FrPlanDev &P = plans[slot];
     __shared__ int s_c[8];
@@ -1579,7 +1708,7 @@ __global__ void k_count_types(const FrPlanDev *plans, int slot, int *out)
 }
 
 /* graph_best_node.cu:182-222 over every NODE */
-__global__ void k_goal_scan(const FrPlanDev *plans, FrCtl *ctl, int slot, int gx, int gz, float gh, float tol,
+__global__ void __launch_bounds__(FR_BLOCK) k_goal_scan(const FrPlanDev *plans, FrCtl *ctl, int slot, int gx, int gz, float gh, float tol,
                             float herr)
 {
     const FrPlanDev &P = plans[slot];
@@ -1604,7 +1733,7 @@ __device__ __forceinline__ bool dev_gate(const FrCtl &C, int gate)
     return true;
 }
 
-__global__ void k_best_node_init(FrCtl *ctl, const int *slots, int n, int gate)
+__global__ void __launch_bounds__(FR_BLOCK) k_best_node_init(FrCtl *ctl, const int *slots, int n, int gate)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -1618,7 +1747,7 @@ __global__ void k_best_node_init(FrCtl *ctl, const int *slots, int n, int gate)
 
 /* __CUDA_KERNEL_findBestNodeWithHeading_bestCost (graph_best_node.cu:21-75).
  * Ties: lowest cell index (second kernel). */
-__global__ void k_best_node_cost(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int tx_override,
+__global__ void __launch_bounds__(FR_BLOCK) k_best_node_cost(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int tx_override,
                                  int tz_override, float h_override, float r_override, float herr_override,
                                  int use_override, int gate)
 {
@@ -1651,7 +1780,7 @@ __global__ void k_best_node_cost(const FrPlanDev *plans, FrCtl *ctl, const int *
     }
 }
 
-__global__ void k_best_node_pick(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int tx_override,
+__global__ void __launch_bounds__(FR_BLOCK) k_best_node_pick(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int tx_override,
                                  int tz_override, float h_override, float r_override, float herr_override,
                                  int use_override, int gate)
 {
@@ -1694,7 +1823,7 @@ __global__ void k_best_node_pick(const FrPlanDev *plans, FrCtl *ctl, const int *
  * (-1, -1) for an out-of-bounds or NULL cell, so such a cell is still pushed
  * and ends the walk.  One thread per plan; a path longer than the buffer is
  * reported as empty, as the reference does after 10^6 steps. */
-__global__ void k_walk(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n_slots, int gate)
+__global__ void __launch_bounds__(FR_BLOCK) k_walk(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n_slots, int gate)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_slots)
@@ -1733,7 +1862,7 @@ __global__ void k_walk(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int
 /* graph_optimization.cu:25-56 for all (i, j >= i + 2), one wave per pair:
  * each i keeps the last improving j with its connection cost, as one 64-bit
  * key (j << 32 | cost bits) in pd[i].xy, reduced with atomicMax (0: none). */
-__global__ void k_opt_pairs(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int gate)
+__global__ void __launch_bounds__(FR_BLOCK) k_opt_pairs(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int gate)
 {
     const int s = slots[blockIdx.y];
     const FrPlanDev &P = plans[s];
@@ -1762,7 +1891,7 @@ __global__ void k_opt_pairs(const FrPlanDev *plans, FrCtl *ctl, const int *slots
     }
 }
 
-__global__ void k_opt_reset(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int gate)
+__global__ void __launch_bounds__(FR_BLOCK) k_opt_reset(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int gate)
 {
     const int s = slots[blockIdx.y];
     const FrPlanDev &P = plans[s];
@@ -1777,7 +1906,7 @@ __global__ void k_opt_reset(const FrPlanDev *plans, FrCtl *ctl, const int *slots
  * the path is spliced (one thread per plan); k_opt_clear then empties the
  * grid and k_opt_rebuild re-adds the chain.  opt_changed = 2 marks the plans
  * this round changed until the rebuild. */
-__global__ void k_opt_apply(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n_slots, int gate)
+__global__ void __launch_bounds__(FR_BLOCK) k_opt_apply(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n_slots, int gate)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_slots)
@@ -1825,7 +1954,7 @@ __global__ void k_opt_apply(const FrPlanDev *plans, FrCtl *ctl, const int *slots
 }
 
 /* graph.cu:400-414 for the plans the round changed */
-__global__ void k_opt_clear(const FrPlanDev *plans, const FrCtl *ctl, const int *slots)
+__global__ void __launch_bounds__(FR_BLOCK) k_opt_clear(const FrPlanDev *plans, const FrCtl *ctl, const int *slots)
 {
     const int s = slots[blockIdx.y];
     if (ctl[s].opt_changed != 2)
@@ -1835,7 +1964,7 @@ __global__ void k_opt_clear(const FrPlanDev *plans, const FrCtl *ctl, const int 
         P.type[pos] = FR_TYPE_NULL;
 }
 
-__global__ void k_opt_rebuild(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n_slots)
+__global__ void __launch_bounds__(FR_BLOCK) k_opt_rebuild(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n_slots)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_slots)
@@ -1901,19 +2030,25 @@ void fr_launch_goal_complete(const FrLaunch &L, const int *d_slots, int n, int m
                        st, L.plans, d_slots, 1);
 }
 
-void fr_launch_iteration(const FrLaunch &L, hipStream_t st)
+/* One loop iteration of every active plan, in three parts so that a caller
+ * that captures graphs can time k_expand with events between them. */
+void fr_launch_iteration_pre(const FrLaunch &L, hipStream_t st)
 {
-    const int n_slots = L.max_slots;
     dim3 b(FR_BLOCK);
-    const int gs = L.grid_slots;
-    fr_prof_begin(FR_PROF_ITERATION, st);
-    hipLaunchKernelGGL(k_begin, dim3(fr_grid_for(n_slots, FR_BLOCK, 65535)), b, 0, st, L.plans, L.ctl, L.slots,
+    hipLaunchKernelGGL(k_begin, dim3(fr_grid_for(L.max_slots, FR_BLOCK, 65535)), b, 0, st, L.plans, L.ctl, L.slots,
                        L.n_slots, L.counters);
     hipLaunchKernelGGL(k_select, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles, L.exp_q,
                        L.counters);
-    fr_prof_begin(FR_PROF_EXPAND, st);
-    hipLaunchKernelGGL(k_expand, dim3(L.grid_waves), b, 0, st, L.plans, L.ctl, L.exp_q, L.counters);
-    fr_prof_end(FR_PROF_EXPAND, st);
+}
+
+void fr_launch_iteration_expand(const FrLaunch &L, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_expand, dim3(L.grid_waves), dim3(FR_BLOCK), 0, st, L.plans, L.ctl, L.exp_q, L.counters);
+}
+
+void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st)
+{
+    dim3 b(FR_BLOCK);
     hipLaunchKernelGGL(k_commit, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.exp_q, L.counters);
     hipLaunchKernelGGL(k_density, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles);
     hipLaunchKernelGGL(k_erase_mark, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles);
@@ -1935,10 +2070,19 @@ void fr_launch_iteration(const FrLaunch &L, hipStream_t st)
     hipLaunchKernelGGL(k_connect_inline, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);
     fr_prof_end(FR_PROF_CONNECT, st);
     hipLaunchKernelGGL(k_connect_pick, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);
-    hipLaunchKernelGGL(k_finish, dim3(fr_grid_for(n_slots, FR_BLOCK / 64, 65535)), b, 0, st, L.plans, L.ctl,
+    hipLaunchKernelGGL(k_finish, dim3(fr_grid_for(L.max_slots, FR_BLOCK / 64, 65535)), b, 0, st, L.plans, L.ctl,
                        L.slots, L.n_slots, L.fresh_q);
+}
+
+void fr_launch_iteration(const FrLaunch &L, hipStream_t st)
+{
+    fr_prof_begin(FR_PROF_ITERATION, st);
+    fr_launch_iteration_pre(L, st);
+    fr_prof_begin(FR_PROF_EXPAND, st);
+    fr_launch_iteration_expand(L, st);
+    fr_prof_end(FR_PROF_EXPAND, st);
+    fr_launch_iteration_post(L, st);
     fr_prof_end(FR_PROF_ITERATION, st);
-    (void)gs;
 }
 
 void fr_launch_clear(const FrLaunch &L, int slot, int N, hipStream_t st)
@@ -1995,7 +2139,7 @@ void fr_launch_opt_round(const FrLaunch &L, const int *d_slots, int n, int max_p
 
 /* graph_best_node.cu:182-222 for several plans (blockIdx.y), skipping plans
  * whose flag is already set (the flag is sticky, :229-230) */
-__global__ void k_goal_scan_multi(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int gate)
+__global__ void __launch_bounds__(FR_BLOCK) k_goal_scan_multi(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int gate)
 {
     const int s = slots[blockIdx.y];
     const FrPlanDev &P = plans[s];
@@ -2019,7 +2163,7 @@ void fr_launch_goal_scan_multi(const FrLaunch &L, const int *d_slots, int n, int
 
 /* search_init for a batch of freshly admitted plans (fastrrt.cpp:85-98): the
  * slab is zero, so only the start node is added. */
-__global__ void k_batch_start(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n)
+__global__ void __launch_bounds__(FR_BLOCK) k_batch_start(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -2039,7 +2183,7 @@ void fr_launch_batch_start(const FrLaunch &L, const int *d_slots, int n, hipStre
 /* ------------------------------------------------------------------------ */
 
 /* graph.cu:364-381 (add / addTemporary); count_parent=0 for addTemporary */
-__global__ void k_op_add(const FrPlanDev *plans, int slot, int x, int z, float h, int ppx, int ppz, float cost,
+__global__ void __launch_bounds__(FR_BLOCK) k_op_add(const FrPlanDev *plans, int slot, int x, int z, float h, int ppx, int ppz, float cost,
                          int type, int count_parent)
 {
     const FrPlanDev &P = plans[slot];
@@ -2059,7 +2203,7 @@ __global__ void k_op_add(const FrPlanDev *plans, int slot, int x, int z, float h
 }
 
 /* graph.cu:553-563 */
-__global__ void k_op_set_collision(const FrPlanDev *plans, int slot, int x, int z, int npx, int npz, float h,
+__global__ void __launch_bounds__(FR_BLOCK) k_op_set_collision(const FrPlanDev *plans, int slot, int x, int z, int npx, int npz, float h,
                                    float cost)
 {
     const FrPlanDev &P = plans[slot];
@@ -2080,7 +2224,7 @@ __global__ void k_op_set_collision(const FrPlanDev *plans, int slot, int x, int 
 
 /* CudaGraph::derivateNode -> expand_node, host sequential semantics
  * (graph_expansion.cu:103-145,224-244).  out = end cell or (-1, -1). */
-__global__ void k_op_derivate(const FrPlanDev *plans, int slot, float steer, float path_size, float v, int x, int z,
+__global__ void __launch_bounds__(FR_BLOCK) k_op_derivate(const FrPlanDev *plans, int slot, float steer, float path_size, float v, int x, int z,
                               int *out)
 {
     FrPlanDev P = plans[slot];
@@ -2122,7 +2266,7 @@ __global__ void k_op_derivate(const FrPlanDev *plans, int slot, float steer, flo
     out[1] = ez;
 }
 
-__global__ void k_op_accept(const FrPlanDev *plans, int slot)
+__global__ void __launch_bounds__(FR_BLOCK) k_op_accept(const FrPlanDev *plans, int slot)
 {
     const FrPlanDev &P = plans[slot];
     for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x)
@@ -2130,7 +2274,7 @@ __global__ void k_op_accept(const FrPlanDev *plans, int slot)
             P.type[pos] = FR_TYPE_NODE;
 }
 
-__global__ void k_op_solve_mark(const FrPlanDev *plans, int slot, int nodes)
+__global__ void __launch_bounds__(FR_BLOCK) k_op_solve_mark(const FrPlanDev *plans, int slot, int nodes)
 {
     const FrPlanDev &P = plans[slot];
     for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
@@ -2155,7 +2299,7 @@ __global__ void k_op_solve_mark(const FrPlanDev *plans, int slot, int nodes)
     }
 }
 
-__global__ void k_op_solve_apply(const FrPlanDev *plans, int slot)
+__global__ void __launch_bounds__(FR_BLOCK) k_op_solve_apply(const FrPlanDev *plans, int slot)
 {
     const FrPlanDev &P = plans[slot];
     for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
@@ -2170,7 +2314,7 @@ __global__ void k_op_solve_apply(const FrPlanDev *plans, int slot)
     }
 }
 
-__global__ void k_op_solve_parents(const FrPlanDev *plans, int slot)
+__global__ void __launch_bounds__(FR_BLOCK) k_op_solve_parents(const FrPlanDev *plans, int slot)
 {
     const FrPlanDev &P = plans[slot];
     for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
@@ -2183,7 +2327,7 @@ __global__ void k_op_solve_parents(const FrPlanDev *plans, int slot)
     }
 }
 
-__global__ void k_op_region_count(const FrPlanDev *plans, int slot, uint32_t *out)
+__global__ void __launch_bounds__(FR_BLOCK) k_op_region_count(const FrPlanDev *plans, int slot, uint32_t *out)
 {
     const FrPlanDev &P = plans[slot];
     for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {

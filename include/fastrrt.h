@@ -208,7 +208,7 @@ typedef struct fastrrt_plan_result {
     int status;               /* 2 = goal reached, 3 = failed (iteration cap or deadline) */
     int iterations;
     int path_len;             /* waypoints written (-1: the caller's buffer is too small) */
-    int pad0;
+    int expansions;           /* rollouts the search ran */
     double submit_ms, done_ms; /* batch clock */
 } fastrrt_plan_result;
 
