@@ -194,8 +194,10 @@ class FrBatch {
     void give_back(std::unique_ptr<FrSlab> s)
     {
         auto key = std::make_tuple(s->W, s->H, s->mx, s->mz);
+        /* keep every slab of the batch's size classes: freeing device memory
+         * synchronises the device, and the cell budget bounds the pool */
         auto &v = pool_[key];
-        if (v.size() < 8)
+        if ((long)v.size() * s->W * s->H < max_cells_)
             v.push_back(std::move(s));
     }
 
@@ -402,13 +404,20 @@ class FrBatch {
         FR_CHECK(hipGetLastError());
         FR_CHECK(hipMemcpyAsync(h_ctl_, L.ctl, sizeof(FrCtl) * slots_.size(), hipMemcpyDeviceToHost, st));
         FR_CHECK(hipStreamSynchronize(st));
-        for (int s : sl) {
+        std::vector<std::vector<float4>> paths(sl.size());
+        for (size_t q = 0; q < sl.size(); ++q) {
+            const int s = sl[q];
             const int len = h_ctl_[s].goal_reached ? std::max(0, h_ctl_[s].path_len) : 0;
-            std::vector<float4> p(len);
+            paths[q].resize(len);
             if (len)
-                FR_CHECK(hipMemcpyAsync(p.data(), slots_[s].slab->path, sizeof(float4) * len, hipMemcpyDeviceToHost,
-                                        st));
-            FR_CHECK(hipStreamSynchronize(st));
+                FR_CHECK(hipMemcpyAsync(paths[q].data(), slots_[s].slab->path, sizeof(float4) * len,
+                                        hipMemcpyDeviceToHost, st));
+        }
+        FR_CHECK(hipStreamSynchronize(st));
+        for (size_t q = 0; q < sl.size(); ++q) {
+            const int s = sl[q];
+            const std::vector<float4> &p = paths[q];
+            const int len = (int)p.size();
             std::vector<float> trip(3 * (size_t)len);
             for (int k = 0; k < len; ++k) {
                 trip[3 * k] = (float)(int)p[k].x;

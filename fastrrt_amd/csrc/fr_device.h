@@ -106,6 +106,7 @@ struct FrCtl {
     unsigned long long opt_key;
     int full_check;           /* next connect pass must scan all NODE cells */
     int total_expansions;     /* rollouts since the search started */
+    int chain_only;           /* the grid holds only the rebuilt path chain (after an optimisation round) */
     int opt_active;           /* batch finalisation: optimisation rounds pending */
 };
 

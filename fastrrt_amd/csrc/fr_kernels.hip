@@ -302,122 +302,64 @@ __device__ bool dev_wave_feasible(const FrPlanDev &P, int x, int z, float h)
     return true;
 }
 
-/* Footprint tests along a sequence of nearby poses (consecutive points of a
- * curve or of a rollout), one wave.  Each lane keeps, for its samples
- * k = lane + 64 r, the value A_at + D(p) where p is the sample's pixel at the
- * last pose it was computed for and A_at the accumulated displacement then.
- * A is a bound on how far any sample pixel can have moved since tracking
- * began: between two poses a sample (i, j) moves by at most
- * mx |dc| + mz |ds| + |dx| (resp. mx |ds| + mz |dc| + |dz|) before rounding,
- * and truncation adds at most one pixel.  By the triangle inequality of the
- * Chebyshev distance, a sample with A < A_at + D(p) is on a free pixel; only
- * the others are recomputed (the reference's expressions) and loaded.  The
- * result is the reference's boolean for every pose. */
-#define FR_TRACK_R 16
+/* path_feasible.cu:18-78 for one lane's own pose (lanes test different
+ * poses at once).  After the all-angle bit and the centre certificate, the
+ * (2 mz + 1) x (2 mx + 1) samples are taken in 7 x 7 groups: the group's
+ * middle sample is a probe; every other sample of the group lies within
+ * hypot(3, 3) of it before truncation, so within Chebyshev distance 5 after,
+ * and a probe at distance >= 6 from every obstacle certifies its group.  A
+ * probe at distance 0 is itself an obstacle sample.  Only the remaining
+ * groups are sampled (with the row skipping of dev_feasible).  The sample
+ * coordinates are the reference's expressions, so the result is its boolean. */
+#define FR_GROUP 7
+#define FR_GROUP_FREE 6
 
-/* per lane: lanemin = the smallest limit of its samples; the limits
- * themselves live in LDS (FR_TRACK_R words per lane, lane-contiguous), read
- * only when a sample's certificate has run out */
-struct FrTrack {
-    int A;
-    int lanemin;
-    int has_ref;
-    int x, z;
-    float s, c;
-    float margin;
-};
-
-__device__ __forceinline__ void dev_track_init(const FrPlanDev &P, FrTrack &T)
-{
-    T.A = 0;
-    T.lanemin = 0;
-    T.has_ref = 0;
-    T.x = T.z = 0;
-    T.s = T.c = 0;
-    /* float rounding of the sample coordinates (a few ulp of the largest) */
-    T.margin = 1e-3f + 1e-6f * (float)(P.W + P.H + 2 * P.pad);
-}
-
-__device__ __forceinline__ bool dev_track_usable(const FrPlanDev &P)
-{
-    return (2 * P.mx + 1) * (2 * P.mz + 1) <= 64 * FR_TRACK_R;
-}
-
-/* s, c = fr_sincosf(h) of a finite heading; all lanes, uniform arguments;
- * blocks of FR_BLOCK threads */
-__device__ __noinline__ bool dev_track_feasible(const FrPlanDev &P, FrTrack &T, int x, int z, float s, float c)
-{
-    __shared__ int s_lim[FR_BLOCK / 64][FR_TRACK_R][64];
-    const int mx = P.mx, mz = P.mz;
-    const bool first = !T.has_ref;
-    if (!first) {
-        const double dc = fabs((double)c - (double)T.c), dsn = fabs((double)s - (double)T.s);
-        const double bx = (double)mx * dc + (double)mz * dsn + fabs((double)(x - T.x));
-        const double bz = (double)mx * dsn + (double)mz * dc + fabs((double)(z - T.z));
-        const double b = (bx > bz ? bx : bz) + (double)T.margin;
-        T.A += (int)floor(b) + 1;
-    }
-    T.x = x;
-    T.z = z;
-    T.s = s;
-    T.c = c;
-    T.has_ref = 1;
-    const bool need = first || T.A >= T.lanemin;
-    if (!__ballot(need))
-        return true;
-    bool bad = false;
-    if (need) {
-        int *lim = &s_lim[threadIdx.x >> 6][0][0];
-        const uint8_t *D = P.dist;
-        const int PW = P.PW, pad = P.pad;
-        const int nx = 2 * mx + 1, ns = nx * (2 * mz + 1);
-        const int lane = dev_lane();
-        int i = lane / nx, j = lane - (lane / nx) * nx;
-        const int di = 64 / nx, dj = 64 - di * nx;
-        const float fxc = (float)x, fzc = (float)z;
-        int mn = 0x7FFFFFFF;
-        for (int r = 0; r < FR_TRACK_R; ++r) {
-            if (lane + 64 * r < ns) {
-                int l = first ? 0 : lim[r * 64 + lane];
-                if (first || T.A >= l) {
-                    const float is = (float)(i - mz) * s;
-                    const float ic = (float)(i - mz) * c;
-                    const float fj = (float)(j - mx);
-                    const float fx = fj * c - is + fxc;
-                    const float fz = fj * s + ic + fzc;
-                    const int xl = (int)fx;
-                    const int zl = (int)fz;
-                    const int d = D[(zl + pad) * PW + (xl + pad)];
-                    bad |= d == 0;
-                    l = T.A + d;
-                    lim[r * 64 + lane] = l;
-                }
-                mn = l < mn ? l : mn;
-            }
-            i += di;
-            j += dj;
-            if (j >= nx) {
-                j -= nx;
-                i += 1;
-            }
-        }
-        T.lanemin = mn;
-    }
-    return !__ballot(bad);
-}
-
-/* dev_wave_feasible through a tracker (falls back to the full test for
- * footprints above 64 * FR_TRACK_R samples) */
-__device__ __forceinline__ bool dev_track_check(const FrPlanDev &P, FrTrack &T, int x, int z, float h, float s,
-                                                float c)
+__device__ bool dev_lane_feasible(const FrPlanDev &P, int x, int z, float h)
 {
     if (P.trav[z * P.W + x] & 1)
         return true;
+    float s, c;
+    fr_sincosf(h, &s, &c);
+    const uint8_t *D = P.dist;
+    const int PW = P.PW, pad = P.pad;
     if (c != c || s != s)
-        return P.dist[P.pad * P.PW + P.pad] != 0;
-    if (!dev_track_usable(P))
-        return dev_wave_feasible(P, x, z, h);
-    return dev_track_feasible(P, T, x, z, s, c);
+        return D[pad * PW + pad] != 0;
+    if (D[(z + pad) * PW + x + pad] >= pad)
+        return true;
+    const int mx = P.mx, mz = P.mz;
+    const float fxc = (float)x, fzc = (float)z;
+    for (int i0 = -mz; i0 <= mz; i0 += FR_GROUP) {
+        const int i1 = i0 + FR_GROUP - 1 < mz ? i0 + FR_GROUP - 1 : mz;
+        const int ic = i0 + FR_GROUP / 2 < i1 ? i0 + FR_GROUP / 2 : i1;
+        const float ics = (float)ic * s, icc = (float)ic * c;
+        for (int j0 = -mx; j0 <= mx; j0 += FR_GROUP) {
+            const int j1 = j0 + FR_GROUP - 1 < mx ? j0 + FR_GROUP - 1 : mx;
+            const int jc = j0 + FR_GROUP / 2 < j1 ? j0 + FR_GROUP / 2 : j1;
+            const float fjc = (float)jc;
+            const int px = (int)(fjc * c - ics + fxc);
+            const int pz = (int)(fjc * s + icc + fzc);
+            const int dp = D[(pz + pad) * PW + (px + pad)];
+            if (dp == 0)
+                return false;
+            if (dp >= FR_GROUP_FREE)
+                continue;
+            for (int i = i0; i <= i1; ++i) {
+                const float is = (float)i * s;
+                const float ic2 = (float)i * c;
+                int j = j0;
+                while (j <= j1) {
+                    const float fj = (float)j;
+                    const int xl = (int)(fj * c - is + fxc);
+                    const int zl = (int)(fj * s + ic2 + fzc);
+                    const int d = D[(zl + pad) * PW + (xl + pad)];
+                    if (d == 0)
+                        return false;
+                    j += (d > 2) ? (d - 1) : 1;
+                }
+            }
+        }
+    }
+    return true;
 }
 
 /* One Hermite point of graph_direct_goal_connection.cu:16-167: position,
@@ -483,8 +425,6 @@ __device__ float dev_wave_check_direct(const FrPlanDev &P, float max_curv, int x
     const bool skip_fp = safe_zone && ((P.trav[pos] & 2) != 0);
     float node_cost = base;
     int last_x = -1, last_z = -1;
-    FrTrack trk;
-    dev_track_init(P, trk);
     for (int pass = 0; pass < 2; ++pass) {
         last_x = -1;
         last_z = -1;
@@ -543,25 +483,17 @@ __device__ float dev_wave_check_direct(const FrPlanDev &P, float max_curv, int x
                 if (__ballot(bad))
                     return -1;
             } else {
-                float add = 0, hd = 0, hs = 0, hc = 0;
-                bool need = false;
+                float add = 0;
+                bool bad = false;
                 if (acc) {
                     add = (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
                     if (!skip_fp) {
-                        hd = (float)((double)fr_atan2f((float)ddz, (float)ddx) + FR_HALF_PI_D);
-                        need = !dev_fp_certified(P, cx, cz, hd);
-                        if (need)
-                            fr_sincosf(hd, &hs, &hc);
+                        const float hd = (float)((double)fr_atan2f((float)ddz, (float)ddx) + FR_HALF_PI_D);
+                        bad = !dev_lane_feasible(P, cx, cz, hd);
                     }
                 }
-                unsigned long long nm = __ballot(need);
-                while (nm) {
-                    const int l = __ffsll((long long)nm) - 1;
-                    nm &= nm - 1;
-                    if (!dev_track_check(P, trk, dev_bcast_i(cx, l), dev_bcast_i(cz, l), dev_bcast_f(hd, l),
-                                         dev_bcast_f(hs, l), dev_bcast_f(hc, l)))
-                        return -1;
-                }
+                if (__ballot(bad))
+                    return -1;
                 unsigned long long cm = am;
                 while (cm) {
                     const int l = __ffsll((long long)cm) - 1;
@@ -671,52 +603,83 @@ __global__ void __launch_bounds__(FR_BLOCK) k_setup_prep(const FrPlanDev *plans,
     }
 }
 
-/* Row pass of the padded Chebyshev distance plane: horizontal distance to the
- * nearest obstacle pixel of the same row (capped), one lane per pixel. */
+/* Row pass of the padded Chebyshev distance plane: horizontal distance to
+ * the nearest obstacle pixel of the same row, capped at FR_DIST_CAP.  One
+ * wave per padded row: a forward sweep carries the last obstacle to the
+ * left, a backward sweep the next one to the right (ballots over 64-pixel
+ * chunks). */
 __global__ void __launch_bounds__(FR_BLOCK) k_setup_rowdist(const FrPlanDev *plans, const int *slots)
 {
     const FrPlanDev &P = plans[slots[blockIdx.y]];
-    const long total = (long)P.PW * P.PH;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-        const int zp = (int)(i / P.PW), xp = (int)(i - (long)zp * P.PW);
+    const int wpb = blockDim.x >> 6;
+    const int lane = dev_lane();
+    for (int zp = blockIdx.x * wpb + (int)(threadIdx.x >> 6); zp < P.PH; zp += gridDim.x * wpb) {
+        uint8_t *out = P.dist_tmp + (long)zp * P.PW;
         const int z = zp - P.pad;
-        int d = FR_DIST_CAP;
-        if (z >= 0 && z < P.H) {
-            const uint8_t *m = P.mark + (long)z * P.W;
-            const int x = xp - P.pad;
-            for (int k = 0; k < FR_DIST_CAP; ++k) {
-                const int xa = x - k, xb = x + k;
-                if ((xa >= 0 && xa < P.W && m[xa]) || (xb >= 0 && xb < P.W && m[xb])) {
-                    d = k;
-                    break;
-                }
-            }
+        if (z < 0 || z >= P.H) {
+            for (int xp = lane; xp < P.PW; xp += 64)
+                out[xp] = FR_DIST_CAP;
+            continue;
         }
-        P.dist_tmp[i] = (uint8_t)d;
+        const uint8_t *m = P.mark + (long)z * P.W;
+        int last = -100000; /* grid x of the last obstacle to the left */
+        for (int b = 0; b < P.PW; b += 64) {
+            const int xp = b + lane, x = xp - P.pad;
+            const bool ob = xp < P.PW && x >= 0 && x < P.W && m[x] != 0;
+            const unsigned long long msk = __ballot(ob);
+            const unsigned long long upto = msk & (lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull));
+            const int prev = upto ? b + 63 - __clzll((long long)upto) - P.pad : last;
+            if (xp < P.PW) {
+                const int d = x - prev;
+                out[xp] = (uint8_t)(d < FR_DIST_CAP ? d : FR_DIST_CAP);
+            }
+            if (msk)
+                last = b + 63 - __clzll((long long)msk) - P.pad;
+        }
+        int next = 100000; /* grid x of the next obstacle to the right */
+        const int nb = (P.PW + 63) / 64;
+        for (int c = nb - 1; c >= 0; --c) {
+            const int b = c * 64;
+            const int xp = b + lane, x = xp - P.pad;
+            const bool ob = xp < P.PW && x >= 0 && x < P.W && m[x] != 0;
+            const unsigned long long msk = __ballot(ob);
+            const unsigned long long from = msk & ~((1ull << lane) - 1ull);
+            const int nx = from ? b + __ffsll((long long)from) - 1 - P.pad : next;
+            if (xp < P.PW) {
+                const int d = nx - x;
+                const int cur = out[xp];
+                if (d < cur)
+                    out[xp] = (uint8_t)d;
+            }
+            if (msk)
+                next = b + __ffsll((long long)msk) - 1 - P.pad;
+        }
     }
 }
 
-/* Column pass: D(x, z) = min_dz max(|dz|, rowdist(x, z + dz)), capped. */
+/* Column pass: D(x, z) = min_dz max(|dz|, rowdist(x, z + dz)), capped.  A
+ * dz at or beyond the running minimum cannot lower it; rows are read eight
+ * offsets at a time so the loads overlap. */
 __global__ void __launch_bounds__(FR_BLOCK) k_setup_coldist(const FrPlanDev *plans, const int *slots)
 {
     const FrPlanDev &P = plans[slots[blockIdx.y]];
     const long total = (long)P.PW * P.PH;
+    const uint8_t *T = P.dist_tmp;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
         const int zp = (int)(i / P.PW), xp = (int)(i - (long)zp * P.PW);
-        int best = P.dist_tmp[i];
-        for (int dz = 1; dz < best; ++dz) {
-            if (zp - dz >= 0) {
-                int v = P.dist_tmp[(long)(zp - dz) * P.PW + xp];
+        int best = T[i];
+        for (int dz0 = 1; dz0 < best; dz0 += 8) {
+            int cand = best;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int dz = dz0 + k;
+                const int up = zp - dz >= 0 ? T[(long)(zp - dz) * P.PW + xp] : 255;
+                const int dn = zp + dz < P.PH ? T[(long)(zp + dz) * P.PW + xp] : 255;
+                int v = up < dn ? up : dn;
                 v = v > dz ? v : dz;
-                if (v < best)
-                    best = v;
+                cand = v < cand ? v : cand;
             }
-            if (zp + dz < P.PH) {
-                int v = P.dist_tmp[(long)(zp + dz) * P.PW + xp];
-                v = v > dz ? v : dz;
-                if (v < best)
-                    best = v;
-            }
+            best = cand;
         }
         P.dist[i] = (uint8_t)best;
     }
@@ -802,6 +765,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_begin(const FrPlanDev *plans, FrCt
     C.best_lcost = (long long)~0ull; /* holds fr_lcost_key() values */
     C.best_node_key = ~0ull;
     C.epoch += 1;
+    C.chain_only = 0;
     if (P.smart) {
         const int nr = P.dens_w * P.dens_h;
         uint32_t *wr = P.region + ((C.iter + 1) & 1) * nr;
@@ -935,8 +899,6 @@ __global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *plans, FrC
         int size = 0, last_x = sx, last_z = sz, prev_x = sx, prev_z = sz;
         float end_h = h;
         bool ok = true;
-        FrTrack trk;
-        dev_track_init(P, trk);
         for (int k0 = 0;; k0 += 64) {
             const int k = k0 + lane;
             /* heading before (hk) and after (hk1) this lane's step */
@@ -985,26 +947,18 @@ __global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *plans, FrC
             const int lim = ends_before ? t_before : (ends_at ? t_at + 1 : 64);
             const unsigned long long acc = m_in & (lim >= 64 ? ~0ull : ((1ull << lim) - 1ull));
             /* cost in step order, then the footprints (any failure rejects) */
-            float add = 0, hs = 0, hc = 0;
-            bool need_fp = false;
+            float add = 0;
+            bool bad = false;
             if ((acc >> lane) & 1ull) {
                 add = (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
-                need_fp = !dev_fp_certified(P, cx, cz, hk1);
-                if (need_fp)
-                    fr_sincosf(hk1, &hs, &hc);
+                bad = !dev_lane_feasible(P, cx, cz, hk1);
             }
             for (unsigned long long cm = acc; cm; cm &= cm - 1)
                 node_cost += dev_bcast_f(add, __ffsll((long long)cm) - 1);
-            for (unsigned long long fm = __ballot(need_fp); fm; fm &= fm - 1) {
-                const int l = __ffsll((long long)fm) - 1;
-                if (!dev_track_check(P, trk, dev_bcast_i(cx, l), dev_bcast_i(cz, l), dev_bcast_f(hk1, l),
-                                     dev_bcast_f(hs, l), dev_bcast_f(hc, l))) {
-                    ok = false;
-                    break;
-                }
-            }
-            if (!ok)
+            if (__ballot(bad)) {
+                ok = false;
                 break;
+            }
             size += __popcll(acc);
             if (acc) {
                 const int l = 63 - __clzll((long long)acc);
@@ -1724,6 +1678,50 @@ __global__ void __launch_bounds__(FR_BLOCK) k_goal_scan(const FrPlanDev *plans, 
     }
 }
 
+/* Cells a whole-grid scan of a plan in batch finalisation visits.  After an
+ * optimisation round has cleared the grid and re-added the path chain
+ * (k_opt_rebuild), the NODE cells are exactly the chain's cells, so the scan
+ * walks the chain (block 0) instead of the grid; a cell listed twice is
+ * evaluated twice with the same values.  Returns false when done. */
+struct FrScan {
+    bool chain;
+    int k, step, n;
+};
+
+__device__ __forceinline__ FrScan dev_scan_begin(const FrPlanDev &P, const FrCtl &C, int gate)
+{
+    FrScan sc;
+    sc.chain = gate != 0 && C.chain_only;
+    if (sc.chain) {
+        sc.k = blockIdx.x == 0 ? (int)threadIdx.x : 0x7FFFFFFF;
+        sc.step = blockDim.x;
+        sc.n = C.path_len;
+    } else {
+        sc.k = blockIdx.x * blockDim.x + threadIdx.x;
+        sc.step = gridDim.x * blockDim.x;
+        sc.n = P.N;
+    }
+    return sc;
+}
+
+__device__ __forceinline__ bool dev_scan_next(const FrPlanDev &P, FrScan &sc, int *pos)
+{
+    while (sc.k < sc.n) {
+        const int k = sc.k;
+        sc.k += sc.step;
+        if (!sc.chain) {
+            *pos = k;
+            return true;
+        }
+        const int x = fr_trunc_f(P.path[k].x), z = fr_trunc_f(P.path[k].y);
+        if (x < 0 || x >= P.W || z < 0 || z >= P.H)
+            continue;
+        *pos = z * P.W + x;
+        return true;
+    }
+    return false;
+}
+
 __device__ __forceinline__ bool dev_gate(const FrCtl &C, int gate)
 {
     if (gate == 1)
@@ -1761,7 +1759,9 @@ __global__ void __launch_bounds__(FR_BLOCK) k_best_node_cost(const FrPlanDev *pl
     const float th = use_override ? h_override : P.gh;
     const float radius = use_override ? r_override : P.dist_tol;
     const float herr = use_override ? herr_override : P.herr_tol;
-    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+    FrScan sc = dev_scan_begin(P, C, gate);
+    int pos;
+    while (dev_scan_next(P, sc, &pos)) {
         if (P.type[pos] != FR_TYPE_NODE)
             continue;
         const int z = pos / P.W, x = pos - z * P.W;
@@ -1797,7 +1797,9 @@ __global__ void __launch_bounds__(FR_BLOCK) k_best_node_pick(const FrPlanDev *pl
     const float th = use_override ? h_override : P.gh;
     const float radius = use_override ? r_override : P.dist_tol;
     const float herr = use_override ? herr_override : P.herr_tol;
-    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+    FrScan sc = dev_scan_begin(P, C, gate);
+    int pos;
+    while (dev_scan_next(P, sc, &pos)) {
         if (P.type[pos] != FR_TYPE_NODE)
             continue;
         const int z = pos / P.W, x = pos - z * P.W;
@@ -1984,6 +1986,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_opt_rebuild(const FrPlanDev *plans
         ppz = cz;
     }
     C.opt_changed = 1;
+    C.chain_only = 1;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2007,7 +2010,8 @@ void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, in
         return;
     dim3 b(FR_BLOCK);
     hipLaunchKernelGGL(k_setup_prep, dim3(fr_grid_for(max_N, FR_BLOCK, 1024), n), b, 0, st, L.plans, d_slots);
-    hipLaunchKernelGGL(k_setup_rowdist, dim3(fr_grid_for(max_PWH, FR_BLOCK, 1024), n), b, 0, st, L.plans, d_slots);
+    hipLaunchKernelGGL(k_setup_rowdist, dim3(fr_grid_for(max_PH, FR_BLOCK / 64, 1024), n), b, 0, st, L.plans,
+                       d_slots);
     hipLaunchKernelGGL(k_setup_coldist, dim3(fr_grid_for(max_PWH, FR_BLOCK, 1024), n), b, 0, st, L.plans,
                        d_slots);
     (void)max_PH;
@@ -2146,7 +2150,9 @@ __global__ void __launch_bounds__(FR_BLOCK) k_goal_scan_multi(const FrPlanDev *p
     FrCtl &C = ctl[s];
     if (C.status != FR_FOUND || (gate == 1 && !C.opt_active) || C.goal_reached)
         return;
-    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
+    FrScan sc = dev_scan_begin(P, C, gate);
+    int pos;
+    while (dev_scan_next(P, sc, &pos)) {
         if (P.type[pos] != FR_TYPE_NODE)
             continue;
         const int z = pos / P.W, x = pos - z * P.W;
