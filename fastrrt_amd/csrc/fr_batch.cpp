@@ -180,8 +180,7 @@ class FrBatch {
         std::unique_ptr<FrSlab> s;
         if (it != pool_.end() && !it->second.empty()) {
             s = std::move(it->second.back());
-            it->second.pop_back();
-            s->reset(eng_.stream());
+            it->second.pop_back(); /* reset on the device at admission (k_slab_reset) */
         } else {
             s.reset(new FrSlab());
             s->allocate(W, H, mx, mz, path_cap(W, H), false, eng_.stream());
@@ -222,6 +221,7 @@ class FrBatch {
             FrPlanParams prm;
             fill_params(prm, p.req, f);
             prm.gc_lazy = 1;
+            prm.opt_rounds = p.req.optimize_rounds;
             const int mx = fr_trunc_f((float)p.req.min_dist_x / 2.0f);
             const int mz = fr_trunc_f((float)p.req.min_dist_z / 2.0f);
             s.slab = take_slab(f->W, f->H, mx, mz);
@@ -229,7 +229,7 @@ class FrBatch {
             v.xyz = f->d_xyz;
             v.costs = f->d_costs;
             v.n_classes = f->n_classes;
-            eng_.bind(free_slot, *s.slab, prm, v);
+            eng_.bind(free_slot, *s.slab, prm, v, false);
             s.busy = true;
             s.ticket = p.ticket;
             s.p = p;
@@ -240,18 +240,18 @@ class FrBatch {
         }
         if (admitted.empty())
             return;
-        /* per-plan setup (goal-connection precompute) for the admitted set */
+        /* descriptors in one copy; slabs reset, planes prepared, control
+         * blocks and start nodes written on the device */
+        eng_.upload_plans();
+        eng_.upload_ints(d_list_, admitted.data(), (int)admitted.size());
+        fr_launch_slab_reset(eng_.launch(), d_list_, (int)admitted.size(), eng_.stream());
         eng_.setup(admitted, FR_GOAL_LAZY);
         for (int sl : admitted) {
             FrCtl c{};
             c.status = FR_SEARCHING;
             c.pending_start = 1;
             std::memcpy(&h_ctl_[sl], &c, sizeof(FrCtl));
-            FR_CHECK(hipMemcpyAsync(eng_.launch().ctl + sl, &h_ctl_[sl], sizeof(FrCtl), hipMemcpyHostToDevice,
-                                    eng_.stream()));
         }
-        FR_CHECK(hipMemcpyAsync(d_list_, admitted.data(), sizeof(int) * admitted.size(), hipMemcpyHostToDevice,
-                                eng_.stream()));
         fr_launch_batch_start(eng_.launch(), d_list_, (int)admitted.size(), eng_.stream());
         FR_CHECK(hipGetLastError());
         running_ += (int)admitted.size();

@@ -73,6 +73,10 @@ struct FrPlanDev {
     float4 *path;
     float4 *pd;
     int path_cap;
+    int opt_limit;            /* path_optimize rounds of batch finalisation */
+    /* planes a new plan starts from zero (batch admission resets them) */
+    uint4 *zero_base;
+    long zero_words;
 };
 
 /* Mutable per-plan control block. */

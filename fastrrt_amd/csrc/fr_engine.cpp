@@ -189,9 +189,9 @@ void FrSlab::allocate(int W_, int H_, int mx_, int mz_, int path_cap_, bool with
     const size_t o_mark = take(N);
     const size_t o_trav = take(N);
     const size_t o_region = take(4 * (size_t)2 * dens);
-    const size_t o_path = take(16 * (size_t)path_cap);
-    const size_t o_pd = take(16 * (size_t)path_cap);
     const size_t zero_end = off; /* everything above starts at zero */
+    const size_t o_path = take(16 * (size_t)path_cap); /* written before read */
+    const size_t o_pd = take(16 * (size_t)path_cap);
     const size_t o_claim = take(8 * N);
     bytes_ = off;
     FR_CHECK(hipMalloc(&base_, bytes_));
@@ -247,6 +247,12 @@ FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
         own_stream_ = true;
     }
     h_plans_.assign(max_slots, FrPlanDev{});
+    FR_CHECK(hipHostMalloc((void **)&h_plan_stage_, sizeof(FrPlanDev) * max_slots, hipHostMallocDefault));
+    FR_CHECK(hipHostMalloc((void **)&h_int_stage_, sizeof(int) * (max_slots + 1) * FR_STAGE_RING,
+                           hipHostMallocDefault));
+    FR_CHECK(hipEventCreateWithFlags(&plan_ev_, hipEventDisableTiming));
+    for (auto &e : int_ev_)
+        FR_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const long max_tiles = queue_cap / FR_TILE + max_slots + 1;
     FR_CHECK(hipMalloc(&L_.plans, sizeof(FrPlanDev) * max_slots));
     FR_CHECK(hipMalloc(&L_.ctl, sizeof(FrCtl) * max_slots));
@@ -290,6 +296,11 @@ FrEngine::~FrEngine()
     (void)hipFree(L_.exp_q);
     (void)hipFree(L_.fresh_q);
     (void)hipFree(d_scratch_);
+    (void)hipHostFree(h_plan_stage_);
+    (void)hipHostFree(h_int_stage_);
+    (void)hipEventDestroy(plan_ev_);
+    for (auto e : int_ev_)
+        (void)hipEventDestroy(e);
     (void)hipFree(L_.pairs);
     (void)hipFree(L_.chk);
     (void)hipFree(L_.demand);
@@ -297,7 +308,7 @@ FrEngine::~FrEngine()
         (void)hipStreamDestroy(st_);
 }
 
-void FrEngine::bind(int slot, FrSlab &s, const FrPlanParams &p, const FrFrameView &frame)
+void FrEngine::bind(int slot, FrSlab &s, const FrPlanParams &p, const FrFrameView &frame, bool upload)
 {
     FrPlanDev d{};
     d.W = s.W;
@@ -359,8 +370,33 @@ void FrEngine::bind(int slot, FrSlab &s, const FrPlanParams &p, const FrFrameVie
     d.path = s.path;
     d.pd = s.pd;
     d.path_cap = s.path_cap;
+    d.zero_base = (uint4 *)s.zero_base();
+    d.zero_words = (long)(s.zero_bytes() / 16);
+    d.opt_limit = p.opt_rounds;
     h_plans_[slot] = d;
-    upload_plan(slot);
+    if (upload)
+        upload_plan(slot);
+}
+
+void FrEngine::upload_plans()
+{
+    FR_CHECK(hipEventSynchronize(plan_ev_)); /* the previous copy has read the staging */
+    std::memcpy(h_plan_stage_, h_plans_.data(), sizeof(FrPlanDev) * max_slots_);
+    FR_CHECK(hipMemcpyAsync(L_.plans, h_plan_stage_, sizeof(FrPlanDev) * max_slots_, hipMemcpyHostToDevice, st_));
+    FR_CHECK(hipEventRecord(plan_ev_, st_));
+}
+
+void FrEngine::upload_ints(int *d_dst, const int *src, int n)
+{
+    if (n > max_slots_ + 1)
+        throw std::runtime_error("upload_ints: too many entries");
+    const int k = int_next_;
+    int_next_ = (int_next_ + 1) % FR_STAGE_RING;
+    int *stage = h_int_stage_ + (size_t)k * (max_slots_ + 1);
+    FR_CHECK(hipEventSynchronize(int_ev_[k])); /* that ring entry's copy has read it */
+    std::memcpy(stage, src, sizeof(int) * n);
+    FR_CHECK(hipMemcpyAsync(d_dst, stage, sizeof(int) * n, hipMemcpyHostToDevice, st_));
+    FR_CHECK(hipEventRecord(int_ev_[k], st_));
 }
 
 void FrEngine::upload_plan(int slot)
@@ -426,7 +462,7 @@ void FrEngine::setup(const std::vector<int> &slots, int goal_mode)
         max_PH = std::max(max_PH, d.PH);
         max_PWH = std::max(max_PWH, (long)d.PW * d.PH);
     }
-    FR_CHECK(hipMemcpyAsync(d_setup_slots_, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, st_));
+    upload_ints(d_setup_slots_, slots.data(), (int)slots.size());
     for (int s : slots) {
         if (goal_mode != FR_GOAL_NONE && h_plans_[s].gc_lazy != (goal_mode == FR_GOAL_LAZY)) {
             h_plans_[s].gc_lazy = goal_mode == FR_GOAL_LAZY;
@@ -435,7 +471,6 @@ void FrEngine::setup(const std::vector<int> &slots, int goal_mode)
     }
     fr_launch_setup(L_, d_setup_slots_, (int)slots.size(), max_N, max_PH, max_PWH, goal_mode, st_);
     FR_CHECK(hipGetLastError());
-    FR_CHECK(hipStreamSynchronize(st_));
 }
 
 void FrEngine::goal_complete(int slot)

@@ -71,6 +71,7 @@ void fr_launch_opt_round(const FrLaunch &L, const int *d_slots, int n, int max_p
                          int gate = FR_GATE_NONE);
 void fr_launch_goal_scan_multi(const FrLaunch &L, const int *d_slots, int n, int max_N, int gate, hipStream_t st);
 void fr_launch_batch_start(const FrLaunch &L, const int *d_slots, int n, hipStream_t st);
+void fr_launch_slab_reset(const FrLaunch &L, const int *d_slots, int n, hipStream_t st);
 void fr_launch_graph_add(const FrLaunch &L, int slot, int x, int z, float h, int px, int pz, float cost, int type,
                          int count_parent, hipStream_t st);
 void fr_launch_set_collision(const FrLaunch &L, int slot, int x, int z, int npx, int npz, float h, float cost,
@@ -125,6 +126,7 @@ struct FrPlanParams {
     int path_cap = 0;             /* 0: W*H + 1 */
     bool with_intrinsic = false;
     int gc_lazy = 0;              /* goal-connection planes evaluated on demand */
+    int opt_rounds = 0;           /* batch: path_optimize rounds after the goal is reached */
 };
 
 /* Device planes of one plan (one allocation, sub-allocated). */
@@ -149,6 +151,9 @@ class FrSlab {
     uint8_t *gc_flag = nullptr;
     float *gc_heading = nullptr, *gc_cost = nullptr;
     uint32_t *gc_done = nullptr;
+    /* planes a new plan needs zero: [zero_base, zero_base + zero_bytes) */
+    void *zero_base() const { return base_; }
+    size_t zero_bytes() const { return zero_end_; }
     unsigned long long *claim = nullptr;
     uint8_t *mark = nullptr, *dist = nullptr, *dist_tmp = nullptr, *trav = nullptr;
     uint32_t *region = nullptr;
@@ -189,9 +194,12 @@ class FrEngine {
     int max_slots() const { return max_slots_; }
 
     /* describe plan `slot` (slab + params + frame) and upload its descriptor */
-    void bind(int slot, FrSlab &slab, const FrPlanParams &p, const FrFrameView &frame);
+    void bind(int slot, FrSlab &slab, const FrPlanParams &p, const FrFrameView &frame, bool upload = true);
     FrPlanDev &host_plan(int slot) { return h_plans_[slot]; }
     void upload_plan(int slot);
+    void upload_plans();                           /* every descriptor, one copy */
+    /* copy n ints to device memory through pinned staging (asynchronous) */
+    void upload_ints(int *d_dst, const int *src, int n);
 
     /* control blocks */
     void reset_ctl(int slot, int status);
@@ -218,6 +226,13 @@ class FrEngine {
     bool own_stream_;
     FrLaunch L_{};
     std::vector<FrPlanDev> h_plans_;
+    FrPlanDev *h_plan_stage_ = nullptr;   /* pinned */
+    hipEvent_t plan_ev_ = nullptr;
+    /* pinned ring of FR_STAGE_RING int arrays of max_slots + 1 entries */
+    static const int FR_STAGE_RING = 8;
+    int *h_int_stage_ = nullptr;
+    hipEvent_t int_ev_[FR_STAGE_RING] = {};
+    int int_next_ = 0;
     std::vector<int> active_;
     std::vector<int2> h_tiles_;
     int *d_setup_slots_ = nullptr;

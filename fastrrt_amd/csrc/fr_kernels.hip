@@ -2167,8 +2167,21 @@ void fr_launch_goal_scan_multi(const FrLaunch &L, const int *d_slots, int n, int
                        L.plans, L.ctl, d_slots, gate);
 }
 
-/* search_init for a batch of freshly admitted plans (fastrrt.cpp:85-98): the
- * slab is zero, so only the start node is added. */
+/* A new plan's slab back to the state of a fresh allocation: its planes
+ * zero, its claim keys all-ones (one block row per plan). */
+__global__ void __launch_bounds__(FR_BLOCK) k_slab_reset(const FrPlanDev *plans, const int *slots)
+{
+    const FrPlanDev &P = plans[slots[blockIdx.y]];
+    const long stride = (long)gridDim.x * blockDim.x;
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    for (long w = t; w < P.zero_words; w += stride)
+        P.zero_base[w] = make_uint4(0u, 0u, 0u, 0u);
+    for (long c = t; c < P.N; c += stride)
+        P.claim[c] = ~0ull;
+}
+
+/* search_init for a batch of freshly admitted plans (fastrrt.cpp:85-98): a
+ * fresh control block, and the start node on the zeroed slab. */
 __global__ void __launch_bounds__(FR_BLOCK) k_batch_start(const FrPlanDev *plans, FrCtl *ctl, const int *slots, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2176,7 +2189,17 @@ __global__ void __launch_bounds__(FR_BLOCK) k_batch_start(const FrPlanDev *plans
         return;
     const int s = slots[i];
     const FrPlanDev &P = plans[s];
+    FrCtl c = {};
+    c.status = FR_SEARCHING;
+    c.pending_start = 1;
+    ctl[s] = c;
     dev_add(P, P.sx, P.sz, P.sh, -1, -1, 0.0f);
+}
+
+void fr_launch_slab_reset(const FrLaunch &L, const int *d_slots, int n, hipStream_t st)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_slab_reset, dim3(64, n), dim3(FR_BLOCK), 0, st, L.plans, d_slots);
 }
 
 void fr_launch_batch_start(const FrLaunch &L, const int *d_slots, int n, hipStream_t st)
