@@ -6,11 +6,12 @@
  * fr_launch_iteration() over the engine's flattened (plan, tile) table,
  * captured once as a hipGraph and replayed.  Requests are admitted as slots
  * and the cell budget (HBM) free up, so plans that arrive while others run
- * join at the next iteration (continuous batching); finished plans leave
- * after their path_optimize rounds and path extraction, which also run
- * batched.  Each plan draws its random numbers from Philox keyed by its own
- * (seed, stream), so a plan's result does not depend on what else is in the
- * batch.
+ * join at the next iteration (continuous batching).  A plan that reaches its
+ * goal is finalised (path_optimize rounds, path extraction) by a second
+ * captured graph on a second stream, over a small list of plans, while the
+ * others keep searching; it leaves when that graph's event has completed.
+ * Each plan draws its random numbers from Philox keyed by its own (seed,
+ * stream), so a plan's result does not depend on what else is in the batch.
  */
 #include <hip/hip_runtime.h>
 
@@ -51,6 +52,7 @@ struct Finished {
 
 struct Slot {
     bool busy = false;
+    bool finalizing = false;
     int ticket = -1;
     Pending p;
     std::unique_ptr<FrSlab> slab;
@@ -61,24 +63,46 @@ struct Slot {
 
 class FrBatch {
   public:
+    /* The engine has one slot more than the batch: a plan that never runs,
+     * whose control block stays IDLE, pads the finalisation list. */
     FrBatch(int max_plans, long max_cells)
-        : eng_(max_plans, std::max<long>(max_cells, 1) + 1), max_cells_(max_cells), slots_(max_plans)
+        : eng_(max_plans + 1, std::max<long>(max_cells, 1) + 1), max_cells_(max_cells), slots_(max_plans),
+          dummy_(max_plans), fin_max_(std::min(max_plans, 64))
     {
         if (max_plans <= 0 || max_cells <= 0)
             throw std::runtime_error("invalid batch capacity");
         FR_CHECK(hipHostMalloc((void **)&h_ctl_, sizeof(FrCtl) * max_plans, hipHostMallocDefault));
+        FR_CHECK(hipHostMalloc((void **)&h_fin_ctl_, sizeof(FrCtl) * fin_max_, hipHostMallocDefault));
+        FR_CHECK(hipHostMalloc((void **)&h_fin_list_, sizeof(int) * fin_max_, hipHostMallocDefault));
+        FR_CHECK(hipHostMalloc((void **)&h_idle_, sizeof(FrCtl), hipHostMallocDefault));
+        std::memset(h_idle_, 0, sizeof(FrCtl));
+        h_idle_->status = FR_IDLE;
         FR_CHECK(hipMalloc(&d_list_, sizeof(int) * max_plans));
+        FR_CHECK(hipMalloc(&fin_list_, sizeof(int) * fin_max_));
+        FR_CHECK(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
+        FR_CHECK(hipEventCreateWithFlags(&fin_ev_, hipEventDisableTiming));
+        eng_.upload_plans(); /* all-zero descriptors, including the padding plan's */
+        FR_CHECK(hipStreamSynchronize(eng_.stream()));
         t0_ = now_ms();
     }
 
     ~FrBatch()
     {
         (void)hipStreamSynchronize(eng_.stream());
+        (void)hipStreamSynchronize(st2_);
         drop_graphs();
-        if (h_ctl_)
-            (void)hipHostFree(h_ctl_);
-        if (d_list_)
-            (void)hipFree(d_list_);
+        if (fin_exec_)
+            (void)hipGraphExecDestroy(fin_exec_);
+        if (fin_graph_)
+            (void)hipGraphDestroy(fin_graph_);
+        (void)hipEventDestroy(fin_ev_);
+        (void)hipStreamDestroy(st2_);
+        (void)hipHostFree(h_ctl_);
+        (void)hipHostFree(h_fin_ctl_);
+        (void)hipHostFree(h_fin_list_);
+        (void)hipHostFree(h_idle_);
+        (void)hipFree(d_list_);
+        (void)hipFree(fin_list_);
     }
 
     int submit(const fastrrt_plan_request &r)
@@ -103,8 +127,12 @@ class FrBatch {
     {
         admit();
         if (running_ > 0) {
-            for (int k = 0; k < iterations; ++k)
-                launch_iteration();
+            if (running_ > finalizing_) {
+                for (int k = 0; k < iterations; ++k)
+                    launch_iteration();
+            } else {
+                FR_CHECK(hipEventSynchronize(fin_ev_)); /* only finalisation left */
+            }
             poll_and_retire();
         }
         return in_flight();
@@ -343,79 +371,105 @@ class FrBatch {
             poll_ema_ = poll_ema_ > 0 ? 0.75 * poll_ema_ + 0.25 * d : d;
         }
         last_poll_ = t;
+        bool changed = false;
+        if (fin_busy_ && hipEventQuery(fin_ev_) == hipSuccess) {
+            collect_finalized();
+            changed = true;
+        }
         for (int i = 0; i < ns; ++i) {
-            if (!slots_[i].busy)
+            if (!slots_[i].busy || slots_[i].finalizing)
                 continue;
             const double dl = slots_[i].p.req.deadline_ms;
             if (h_ctl_[i].status == FR_FOUND)
                 found.push_back(i);
             else if (h_ctl_[i].status == FR_FAILED)
                 failed.push_back(i);
-            else if (dl > 0 && t - slots_[i].p.t_submit + poll_ema_ > dl)
-                failed.push_back(i);
+            else if (dl > 0 && t - slots_[i].p.t_submit + 1.5 * poll_ema_ + 0.1 > dl)
+                failed.push_back(i); /* with a margin for a slower next iteration */
         }
-        if (found.empty() && failed.empty())
-            return;
-        if (!found.empty())
-            finalize(found);
         for (int sl : failed)
-            retire(sl, FR_FAILED, nullptr, 0);
-        rebuild_active();
+            retire(sl, FR_FAILED, nullptr, 0, h_ctl_[sl]);
+        if (!found.empty() && !fin_busy_)
+            launch_finalize(found);
+        if (changed || !failed.empty())
+            rebuild_active();
     }
 
     /* path_optimize rounds (fastrrt.cpp:183-196) and getPlannedPath
-     * (:204-232) for every plan that reached its goal */
-    void finalize(const std::vector<int> &sl)
+     * (:204-232) of up to fin_max_ plans that reached their goal, on the
+     * second stream.  The control blocks were prepared by k_finish (rounds
+     * from the request); the list is padded with the idle plan. */
+    void launch_finalize(const std::vector<int> &found)
+    {
+        int rounds = 0;
+        fin_slots_.clear();
+        for (int s : found) {
+            if ((int)fin_slots_.size() == fin_max_)
+                break;
+            fin_slots_.push_back(s);
+            rounds = std::max(rounds, slots_[s].p.req.optimize_rounds);
+        }
+        if (!fin_exec_ || rounds > fin_rounds_)
+            capture_finalize(std::max(rounds, fin_rounds_));
+        for (int q = 0; q < fin_max_; ++q)
+            h_fin_list_[q] = q < (int)fin_slots_.size() ? fin_slots_[q] : dummy_;
+        FR_CHECK(hipMemcpyAsync(fin_list_, h_fin_list_, sizeof(int) * fin_max_, hipMemcpyHostToDevice, st2_));
+        FR_CHECK(hipGraphLaunch(fin_exec_, st2_));
+        FR_CHECK(hipEventRecord(fin_ev_, st2_));
+        for (int s : fin_slots_)
+            slots_[s].finalizing = true;
+        finalizing_ += (int)fin_slots_.size();
+        fin_busy_ = true;
+    }
+
+    /* Grid sizes are small: the kernels stride over cells, path pairs and
+     * list entries, and most entries of most launches are gated out. */
+    void capture_finalize(int rounds)
+    {
+        if (fin_exec_)
+            (void)hipGraphExecDestroy(fin_exec_);
+        if (fin_graph_)
+            (void)hipGraphDestroy(fin_graph_);
+        fin_exec_ = nullptr;
+        fin_graph_ = nullptr;
+        const FrLaunch &L = eng_.launch();
+        const int n = fin_max_, grid_cells = 16 * FR_BLOCK, grid_path = 16;
+        const bool prof = fr_prof_on();
+        fr_prof_enable(false);
+        FR_CHECK(hipStreamBeginCapture(st2_, hipStreamCaptureModeThreadLocal));
+        for (int r = 0; r < rounds; ++r) {
+            fr_launch_goal_scan_multi(L, fin_list_, n, grid_cells, FR_GATE_OPT, st2_);
+            fr_launch_best_node(L, fin_list_, n, grid_cells, nullptr, st2_, FR_GATE_OPT);
+            fr_launch_walk(L, fin_list_, n, st2_, FR_GATE_OPT);
+            fr_launch_opt_round(L, fin_list_, n, grid_path, grid_cells, st2_, FR_GATE_OPT);
+        }
+        fr_launch_goal_scan_multi(L, fin_list_, n, grid_cells, FR_GATE_FOUND, st2_);
+        fr_launch_best_node(L, fin_list_, n, grid_cells, nullptr, st2_, FR_GATE_FOUND);
+        fr_launch_walk(L, fin_list_, n, st2_, FR_GATE_FOUND);
+        FR_CHECK(hipStreamEndCapture(st2_, &fin_graph_));
+        fr_prof_enable(prof);
+        FR_CHECK(hipGraphInstantiate(&fin_exec_, fin_graph_, nullptr, nullptr, 0));
+        fin_rounds_ = rounds;
+    }
+
+    void collect_finalized()
     {
         const FrLaunch &L = eng_.launch();
-        hipStream_t st = eng_.stream();
-        int max_N = 0, max_rounds = 0;
-        for (int s : sl) {
-            max_N = std::max(max_N, (int)(slots_[s].slab->W * slots_[s].slab->H));
-            max_rounds = std::max(max_rounds, slots_[s].p.req.optimize_rounds);
-        }
-        for (int s : sl) {
-            h_ctl_[s].opt_active = slots_[s].p.req.optimize_rounds > 0 ? 1 : 0;
-            h_ctl_[s].opt_limit = slots_[s].p.req.optimize_rounds;
-            h_ctl_[s].opt_rounds = 0;
-            FR_CHECK(hipMemcpyAsync(L.ctl + s, &h_ctl_[s], sizeof(FrCtl), hipMemcpyHostToDevice, st));
-        }
-        FR_CHECK(hipMemcpyAsync(d_list_, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice, st));
-        const int n = (int)sl.size();
-        for (int r = 0; r < max_rounds; ++r) {
-            fr_launch_goal_scan_multi(L, d_list_, n, max_N, FR_GATE_OPT, st);
-            fr_launch_best_node(L, d_list_, n, max_N, nullptr, st, FR_GATE_OPT);
-            fr_launch_walk(L, d_list_, n, st, FR_GATE_OPT);
-            fr_launch_opt_round(L, d_list_, n, max_path_len_, max_N, st, FR_GATE_OPT);
-            if ((r & 3) == 3) {
-                /* stop early when every plan has converged */
-                FR_CHECK(hipMemcpyAsync(h_ctl_, L.ctl, sizeof(FrCtl) * slots_.size(), hipMemcpyDeviceToHost, st));
-                FR_CHECK(hipStreamSynchronize(st));
-                bool any = false;
-                for (int s : sl)
-                    any |= h_ctl_[s].opt_active != 0;
-                if (!any)
-                    break;
-            }
-        }
-        fr_launch_goal_scan_multi(L, d_list_, n, max_N, FR_GATE_FOUND, st);
-        fr_launch_best_node(L, d_list_, n, max_N, nullptr, st, FR_GATE_FOUND);
-        fr_launch_walk(L, d_list_, n, st, FR_GATE_FOUND);
-        FR_CHECK(hipGetLastError());
-        FR_CHECK(hipMemcpyAsync(h_ctl_, L.ctl, sizeof(FrCtl) * slots_.size(), hipMemcpyDeviceToHost, st));
-        FR_CHECK(hipStreamSynchronize(st));
-        std::vector<std::vector<float4>> paths(sl.size());
-        for (size_t q = 0; q < sl.size(); ++q) {
-            const int s = sl[q];
-            const int len = h_ctl_[s].goal_reached ? std::max(0, h_ctl_[s].path_len) : 0;
+        const int nf = (int)fin_slots_.size();
+        for (int q = 0; q < nf; ++q)
+            FR_CHECK(hipMemcpyAsync(&h_fin_ctl_[q], L.ctl + fin_slots_[q], sizeof(FrCtl), hipMemcpyDeviceToHost,
+                                    st2_));
+        FR_CHECK(hipStreamSynchronize(st2_));
+        std::vector<std::vector<float4>> paths(nf);
+        for (int q = 0; q < nf; ++q) {
+            const int len = h_fin_ctl_[q].goal_reached ? std::max(0, h_fin_ctl_[q].path_len) : 0;
             paths[q].resize(len);
             if (len)
-                FR_CHECK(hipMemcpyAsync(paths[q].data(), slots_[s].slab->path, sizeof(float4) * len,
-                                        hipMemcpyDeviceToHost, st));
+                FR_CHECK(hipMemcpyAsync(paths[q].data(), slots_[fin_slots_[q]].slab->path, sizeof(float4) * len,
+                                        hipMemcpyDeviceToHost, st2_));
         }
-        FR_CHECK(hipStreamSynchronize(st));
-        for (size_t q = 0; q < sl.size(); ++q) {
-            const int s = sl[q];
+        FR_CHECK(hipStreamSynchronize(st2_));
+        for (int q = 0; q < nf; ++q) {
             const std::vector<float4> &p = paths[q];
             const int len = (int)p.size();
             std::vector<float> trip(3 * (size_t)len);
@@ -424,29 +478,31 @@ class FrBatch {
                 trip[3 * k + 1] = (float)(int)p[k].y;
                 trip[3 * k + 2] = p[k].z;
             }
-            retire(s, FR_FOUND, trip.data(), len);
+            slots_[fin_slots_[q]].finalizing = false;
+            retire(fin_slots_[q], FR_FOUND, trip.data(), len, h_fin_ctl_[q]);
         }
+        finalizing_ -= nf;
+        fin_slots_.clear();
+        fin_busy_ = false;
     }
 
-    void retire(int s, int status, const float *path, int len)
+    /* The slot's control block goes back to IDLE on the iteration stream
+     * (from a constant pinned block), ahead of any later admission. */
+    void retire(int s, int status, const float *path, int len, const FrCtl &c)
     {
         Slot &sl = slots_[s];
         Finished f;
         f.res.user_tag = sl.p.req.user_tag;
         f.res.status = status;
-        f.res.iterations = h_ctl_[s].iter;
-        f.res.expansions = h_ctl_[s].total_expansions;
+        f.res.iterations = c.iter;
+        f.res.expansions = c.total_expansions;
         f.res.path_len = len;
         f.res.submit_ms = sl.p.t_submit;
         f.res.done_ms = now_ms() - t0_;
         if (len > 0)
             f.path.assign(path, path + 3 * (size_t)len);
         done_.push_back(std::move(f));
-        FrCtl idle{};
-        idle.status = FR_IDLE;
-        std::memcpy(&h_ctl_[s], &idle, sizeof(FrCtl));
-        /* the stream is synchronised by rebuild_active() before h_ctl_ is reused */
-        FR_CHECK(hipMemcpyAsync(eng_.launch().ctl + s, &h_ctl_[s], sizeof(FrCtl), hipMemcpyHostToDevice,
+        FR_CHECK(hipMemcpyAsync(eng_.launch().ctl + s, h_idle_, sizeof(FrCtl), hipMemcpyHostToDevice,
                                 eng_.stream()));
         give_back(std::move(sl.slab));
         cells_in_use_ -= sl.cells;
@@ -464,9 +520,23 @@ class FrBatch {
     std::map<std::tuple<int, int, int, int>, std::vector<std::unique_ptr<FrSlab>>> pool_;
     FrCtl *h_ctl_ = nullptr;
     int *d_list_ = nullptr;
+    /* finalisation on the second stream */
+    int dummy_;
+    int fin_max_;
+    hipStream_t st2_ = nullptr;
+    hipEvent_t fin_ev_ = nullptr;
+    hipGraph_t fin_graph_ = nullptr;
+    hipGraphExec_t fin_exec_ = nullptr;
+    int fin_rounds_ = 0;
+    bool fin_busy_ = false;
+    int finalizing_ = 0;
+    std::vector<int> fin_slots_;
+    int *fin_list_ = nullptr;
+    int *h_fin_list_ = nullptr;
+    FrCtl *h_fin_ctl_ = nullptr;
+    FrCtl *h_idle_ = nullptr;
     int running_ = 0;
     int next_ticket_ = 0;
-    int max_path_len_ = 1024;
     double t0_ = 0;
     hipGraph_t graph_[3] = {nullptr, nullptr, nullptr};
     hipGraphExec_t exec_[3] = {nullptr, nullptr, nullptr};

@@ -57,6 +57,7 @@ struct FrPlanDev {
     uint8_t *gc_flag;
     float *gc_heading, *gc_cost;
     uint32_t *gc_done;        /* bit per cell: goal-connection planes evaluated */
+    uint32_t *gc_bits;        /* bit per cell: gc_flag, for window scans by word */
     int gc_lazy;              /* evaluate a cell when a node's window first covers it */
     /* scratch planes */
     unsigned long long *claim;

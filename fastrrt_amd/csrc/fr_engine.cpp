@@ -186,6 +186,7 @@ void FrSlab::allocate(int W_, int H_, int mx_, int mz_, int path_cap_, bool with
     const size_t o_gch = take(4 * N);
     const size_t o_gcc = take(4 * N);
     const size_t o_done = take(4 * ((N + 31) / 32));
+    const size_t o_bits = take(4 * ((N + 31) / 32));
     const size_t o_mark = take(N);
     const size_t o_trav = take(N);
     const size_t o_region = take(4 * (size_t)2 * dens);
@@ -208,6 +209,7 @@ void FrSlab::allocate(int W_, int H_, int mx_, int mz_, int path_cap_, bool with
     gc_heading = (float *)(b + o_gch);
     gc_cost = (float *)(b + o_gcc);
     gc_done = (uint32_t *)(b + o_done);
+    gc_bits = (uint32_t *)(b + o_bits);
     mark = (uint8_t *)(b + o_mark);
     trav = (uint8_t *)(b + o_trav);
     region = (uint32_t *)(b + o_region);
@@ -357,6 +359,7 @@ void FrEngine::bind(int slot, FrSlab &s, const FrPlanParams &p, const FrFrameVie
     d.gc_heading = s.gc_heading;
     d.gc_cost = s.gc_cost;
     d.gc_done = s.gc_done;
+    d.gc_bits = s.gc_bits;
     d.gc_lazy = p.gc_lazy;
     d.claim = s.claim;
     d.mark = s.mark;

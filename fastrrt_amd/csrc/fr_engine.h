@@ -150,7 +150,7 @@ class FrSlab {
     float *heading = nullptr, *cost = nullptr, *intrinsic = nullptr, *direct = nullptr;
     uint8_t *gc_flag = nullptr;
     float *gc_heading = nullptr, *gc_cost = nullptr;
-    uint32_t *gc_done = nullptr;
+    uint32_t *gc_done = nullptr, *gc_bits = nullptr;
     /* planes a new plan needs zero: [zero_base, zero_base + zero_bytes) */
     void *zero_base() const { return base_; }
     size_t zero_bytes() const { return zero_end_; }

@@ -700,6 +700,9 @@ __device__ void dev_gc_cell(const FrPlanDev &P, int pos)
         if (c >= 0) {
             P.gc_heading[pos] = lh;
             P.gc_cost[pos] = c;
+            atomicOr(&P.gc_bits[pos >> 5], 1u << (pos & 31));
+        } else {
+            atomicAnd(&P.gc_bits[pos >> 5], ~(1u << (pos & 31)));
         }
         atomicOr(&P.gc_done[pos >> 5], 1u << (pos & 31));
     }
@@ -712,8 +715,10 @@ __global__ void __launch_bounds__(FR_BLOCK) k_setup_gc_reset(const FrPlanDev *pl
     for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
         P.gc_cost[pos] = P.cost[pos];
         P.gc_flag[pos] = 0;
-        if ((pos & 31) == 0)
+        if ((pos & 31) == 0) {
             P.gc_done[pos >> 5] = 0;
+            P.gc_bits[pos >> 5] = 0;
+        }
     }
 }
 
@@ -1306,6 +1311,60 @@ __device__ __forceinline__ int dev_window_at(const FrPlanDev &P, int z0, int x0,
     return (z0 + r) * P.W + x0 + (q - r * wx);
 }
 
+/* A window by 32-cell words of a bit plane: row r of the window covers the
+ * cells [(z0 + r) W + x0, (z0 + r) W + x0 + wx), i.e. at most wpr words; pair
+ * p = r * wpr + k is the k-th of them, with the mask of the window's bits.
+ * Pairs in increasing order visit the window's cells in row-major order. */
+struct FrWin {
+    int z0, x0, wx, wz, wpr, npairs;
+};
+
+__device__ __forceinline__ bool dev_win_make(const FrPlanDev &P, int x, int z, FrWin *w)
+{
+    int z0, z1, x0, x1;
+    dev_window(P, x, z, &z0, &z1, &x0, &x1);
+    w->z0 = z0;
+    w->x0 = x0;
+    w->wx = x1 - x0;
+    w->wz = z1 - z0;
+    if (w->wx <= 0 || w->wz <= 0)
+        return false;
+    w->wpr = (w->wx + 62) / 32;
+    w->npairs = w->wz * w->wpr;
+    return true;
+}
+
+__device__ __forceinline__ uint32_t dev_win_pair(const FrPlanDev &P, const FrWin &w, int p, long *word)
+{
+    *word = 0;
+    if (p >= w.npairs)
+        return 0u;
+    const int r = p / w.wpr, k = p - r * w.wpr;
+    const long qa = (long)(w.z0 + r) * P.W + w.x0, qb = qa + w.wx - 1;
+    const long wa = qa >> 5, wb = qb >> 5, wi = wa + k;
+    if (wi > wb)
+        return 0u;
+    uint32_t m = 0xFFFFFFFFu;
+    if (wi == wa)
+        m &= 0xFFFFFFFFu << (int)(qa & 31);
+    if (wi == wb)
+        m &= 0xFFFFFFFFu >> (31 - (int)(qb & 31));
+    *word = wi;
+    return m;
+}
+
+/* inclusive prefix sum over the wave */
+__device__ __forceinline__ int dev_wave_incl_scan(int v)
+{
+    const int lane = dev_lane();
+    for (int off = 1; off < 64; off <<= 1) {
+        const int n = __shfl_up(v, off, 64);
+        if (lane >= off)
+            v += n;
+    }
+    return v;
+}
+
 /* node -> flagged cell check; returns the lcost or FR_BIG_COST.  `wave`:
  * every lane of the wave calls with the same node and cell. */
 __device__ __forceinline__ long long dev_pair_lcost(const FrPlanDev &P, int node, int cell, float *c_out,
@@ -1326,8 +1385,10 @@ __device__ __forceinline__ long long dev_pair_lcost(const FrPlanDev &P, int node
 }
 
 /* Cells of the new nodes' windows whose goal-connection planes are not yet
- * evaluated (lazy plans): claimed with one atomicOr per cell, queued. */
-__global__ void __launch_bounds__(FR_BLOCK) k_gc_demand(const FrPlanDev *plans, const FrFresh *fq, int2 *dq, long dq_cap, int *counters)
+ * evaluated (lazy plans): claimed a word at a time with atomicOr on the
+ * evaluated-bit plane, queued. */
+__global__ void __launch_bounds__(FR_BLOCK) k_gc_demand(const FrPlanDev *plans, const FrFresh *fq, int2 *dq,
+                                                         long dq_cap, int *counters)
 {
     const int n = counters[FR_CNT_FRESH];
     const int wpb = blockDim.x >> 6;
@@ -1339,32 +1400,33 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_demand(const FrPlanDev *plans, 
         if (!P.gc_lazy || P.type[f.pos] != FR_TYPE_NODE)
             continue;
         const int z = f.pos / P.W, x = f.pos - z * P.W;
-        int z0, z1, x0, x1;
-        dev_window(P, x, z, &z0, &z1, &x0, &x1);
-        const int wx = x1 - x0, wz = z1 - z0;
-        if (wx <= 0 || wz <= 0)
+        FrWin w;
+        if (!dev_win_make(P, x, z, &w))
             continue;
-        const int total = wx * wz;
-        for (int b = 0; b < total; b += 64) {
-            const int k = b + lane;
-            bool mine = false;
-            int q = 0;
-            if (k < total) {
-                q = dev_window_at(P, z0, x0, wx, k);
-                const uint32_t bit = 1u << (q & 31);
-                if (!(P.gc_done[q >> 5] & bit))
-                    mine = !(atomicOr(&P.gc_done[q >> 5], bit) & bit);
+        for (int b = 0; b < w.npairs; b += 64) {
+            long word;
+            const uint32_t m = dev_win_pair(P, w, b + lane, &word);
+            uint32_t mine = 0;
+            if (m) {
+                const uint32_t need = m & ~P.gc_done[word];
+                if (need)
+                    mine = need & ~atomicOr(&P.gc_done[word], need);
             }
-            const unsigned long long m = __ballot(mine);
-            if (!m)
+            const int c = __popc(mine);
+            const int incl = dev_wave_incl_scan(c);
+            const int total = dev_bcast_i(incl, 63);
+            if (!total)
                 continue;
             int base = 0;
             if (lane == 0)
-                base = atomicAdd(&counters[FR_CNT_DEMAND], __popcll(m));
-            base = dev_bcast_i(base, 0);
-            const long o = (long)base + __popcll(m & ((1ull << lane) - 1ull));
-            if (mine && o < dq_cap) /* each cell is queued once per plan: sum N <= dq_cap */
-                dq[o] = make_int2(f.slot, q);
+                base = atomicAdd(&counters[FR_CNT_DEMAND], total);
+            long o = (long)dev_bcast_i(base, 0) + (incl - c);
+            for (uint32_t mm = mine; mm; mm &= mm - 1) {
+                const int bit = __ffs((int)mm) - 1;
+                if (o < dq_cap) /* each cell is queued once per plan: sum N <= dq_cap */
+                    dq[o] = make_int2(f.slot, (int)(word * 32 + bit));
+                ++o;
+            }
         }
     }
 }
@@ -1383,9 +1445,10 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *plans, co
 }
 
 /* Goal test of the new nodes (graph_best_node.cu:182-222), window counts and
- * the pair queue; one wave per node, pairs in the window's row-major order. */
-__global__ void __launch_bounds__(FR_BLOCK) k_connect_emit(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, FrPair *pairs, long pair_cap,
-                               int *counters)
+ * the pair queue; one wave per node, the window read by words of the flag
+ * bit plane, pairs in the window's row-major order. */
+__global__ void __launch_bounds__(FR_BLOCK) k_connect_emit(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq,
+                                                            FrPair *pairs, long pair_cap, int *counters)
 {
     const int n = counters[FR_CNT_FRESH];
     const int wpb = blockDim.x >> 6;
@@ -1409,17 +1472,15 @@ __global__ void __launch_bounds__(FR_BLOCK) k_connect_emit(const FrPlanDev *plan
             if (dev_goal_hit(P, x, z, P.heading[fpos]))
                 C.goal_reached = 1;
         }
-        int z0, z1, x0, x1;
-        dev_window(P, x, z, &z0, &z1, &x0, &x1);
-        const int wx = x1 - x0, wz = z1 - z0;
-        if (wx <= 0 || wz <= 0)
+        FrWin w;
+        if (!dev_win_make(P, x, z, &w))
             continue;
-        const int total = wx * wz;
         int cnt = 0;
-        for (int b = 0; b < total; b += 64) {
-            const int k = b + lane;
-            const bool on = k < total && P.gc_flag[dev_window_at(P, z0, x0, wx, k)] != 0;
-            cnt += __popcll(__ballot(on));
+        for (int b = 0; b < w.npairs; b += 64) {
+            long word;
+            const uint32_t m = dev_win_pair(P, w, b + lane, &word);
+            const int c = m ? __popc(m & P.gc_bits[word]) : 0;
+            cnt += dev_bcast_i(dev_wave_incl_scan(c), 63);
         }
         if (cnt == 0)
             continue;
@@ -1433,18 +1494,20 @@ __global__ void __launch_bounds__(FR_BLOCK) k_connect_emit(const FrPlanDev *plan
             continue;
         }
         long o = base;
-        for (int b = 0; b < total; b += 64) {
-            const int k = b + lane;
-            const int q = k < total ? dev_window_at(P, z0, x0, wx, k) : 0;
-            const bool on = k < total && P.gc_flag[q] != 0;
-            const unsigned long long m = __ballot(on);
-            if (on) {
+        for (int b = 0; b < w.npairs; b += 64) {
+            long word;
+            const uint32_t m = dev_win_pair(P, w, b + lane, &word);
+            const uint32_t on = m ? (m & P.gc_bits[word]) : 0u;
+            const int c = __popc(on);
+            const int incl = dev_wave_incl_scan(c);
+            long q = o + (incl - c);
+            for (uint32_t mm = on; mm; mm &= mm - 1) {
                 FrPair pr;
                 pr.fresh = i;
-                pr.cell = q;
-                pairs[o + __popcll(m & ((1ull << lane) - 1ull))] = pr;
+                pr.cell = (int)(word * 32 + __ffs((int)mm) - 1);
+                pairs[q++] = pr;
             }
-            o += __popcll(m);
+            o += dev_bcast_i(incl, 63);
         }
         if (lane == 0) {
             f.pair_base = base;
@@ -1534,30 +1597,37 @@ __global__ void __launch_bounds__(FR_BLOCK) k_connect_pick(const FrPlanDev *plan
 }
 
 /* the rank-th goal-flagged cell of a node's window, in row-major order
- * (one wave) */
+ * (one wave, by words of the flag bit plane) */
 __device__ int dev_wave_window_cell(const FrPlanDev &P, int node, int rank)
 {
     const int z = node / P.W, x = node - z * P.W;
-    int z0, z1, x0, x1;
-    dev_window(P, x, z, &z0, &z1, &x0, &x1);
-    const int wx = x1 - x0, wz = z1 - z0;
-    if (wx <= 0 || wz <= 0)
+    FrWin w;
+    if (!dev_win_make(P, x, z, &w))
         return -1;
-    const int total = wx * wz;
     const int lane = dev_lane();
     int seen = 0;
-    for (int b = 0; b < total; b += 64) {
-        const int k = b + lane;
-        const int q = k < total ? dev_window_at(P, z0, x0, wx, k) : 0;
-        const bool on = k < total && P.gc_flag[q] != 0;
-        unsigned long long m = __ballot(on);
-        const int c = __popcll(m);
-        if (rank < seen + c) {
-            for (int t = rank - seen; t > 0; --t)
-                m &= m - 1;
-            return dev_bcast_i(q, __ffsll((long long)m) - 1);
+    for (int b = 0; b < w.npairs; b += 64) {
+        long word;
+        const uint32_t m = dev_win_pair(P, w, b + lane, &word);
+        const uint32_t on = m ? (m & P.gc_bits[word]) : 0u;
+        const int c = __popc(on);
+        const int incl = dev_wave_incl_scan(c);
+        const int total = dev_bcast_i(incl, 63);
+        if (rank < seen + total) {
+            const int r = rank - seen;
+            const bool here = r >= incl - c && r < incl;
+            const unsigned long long hm = __ballot(here);
+            const int l = __ffsll((long long)hm) - 1;
+            int cell = -1;
+            if (lane == l) {
+                uint32_t mm = on;
+                for (int t = r - (incl - c); t > 0; --t)
+                    mm &= mm - 1;
+                cell = (int)(word * 32 + __ffs((int)mm) - 1);
+            }
+            return dev_bcast_i(cell, l);
         }
-        seen += c;
+        seen += total;
     }
     return -1;
 }
@@ -1621,10 +1691,15 @@ __global__ void __launch_bounds__(FR_BLOCK) k_finish(const FrPlanDev *plans, FrC
                 C.full_check = 1;
         }
     }
-    if (C.goal_reached)
+    if (C.goal_reached) {
         C.status = FR_FOUND;
-    else if (P.max_iter > 0 && C.iter >= P.max_iter)
+        /* batch finalisation: path_optimize rounds of the request */
+        C.opt_limit = P.opt_limit;
+        C.opt_active = P.opt_limit > 0;
+        C.opt_rounds = 0;
+    } else if (P.max_iter > 0 && C.iter >= P.max_iter) {
         C.status = FR_FAILED;
+    }
 }
 
 /* ------------------------------------------------------------------------ */
