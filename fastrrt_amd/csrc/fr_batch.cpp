@@ -33,6 +33,10 @@
 
 namespace {
 
+/* batch result status beyond FrStatus: goal reached, but no feasible node
+ * near the goal to extract a path from (get_planned_path returns nothing) */
+const int FR_RESULT_NO_PATH = 4;
+
 double now_ms()
 {
     using namespace std::chrono;
@@ -479,7 +483,9 @@ class FrBatch {
                 trip[3 * k + 2] = p[k].z;
             }
             slots_[fin_slots_[q]].finalizing = false;
-            retire(fin_slots_[q], FR_FOUND, trip.data(), len, h_fin_ctl_[q]);
+            /* the goal test (graph_best_node.cu:182-222) does not test the
+             * footprint, findBestNode does: a reached goal can leave no path */
+            retire(fin_slots_[q], len > 0 ? FR_FOUND : FR_RESULT_NO_PATH, trip.data(), len, h_fin_ctl_[q]);
         }
         finalizing_ -= nf;
         fin_slots_.clear();

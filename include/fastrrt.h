@@ -205,7 +205,8 @@ typedef struct fastrrt_plan_request {
 
 typedef struct fastrrt_plan_result {
     int64_t user_tag;
-    int status;               /* 2 = goal reached, 3 = failed (iteration cap or deadline) */
+    int status;               /* 2 = path found, 3 = failed (iteration cap or deadline),
+                                 4 = goal reached but no feasible node to extract a path from */
     int iterations;
     int path_len;             /* waypoints written (-1: the caller's buffer is too small) */
     int expansions;           /* rollouts the search ran */

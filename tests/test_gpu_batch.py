@@ -79,7 +79,7 @@ def test_batch_equals_plain_handle_and_oracle():
             while k < 3 and o.path_optimize():
                 k += 1
             po = o.path()
-            assert r.status == 2
+            assert r.status == (2 if po.shape[0] else 4)
             assert r.path.shape == po.shape and (r.path.view(np.int32) == po.view(np.int32)).all(), i
         else:
             assert r.status == 3
