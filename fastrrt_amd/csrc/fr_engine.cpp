@@ -280,7 +280,7 @@ FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
     L_.grid_tiles = (int)std::min<long>(max_tiles, 2048);
     L_.grid_items = (int)std::max<long>(1, std::min<long>((queue_cap + FR_BLOCK - 1) / FR_BLOCK, 2048));
     L_.grid_slots = 1;
-    L_.grid_waves = 4096; /* blocks of 4 one-item waves */
+    L_.grid_waves = 4096; /* blocks of 4 one-item waves; k_expand needs a multiple of 8 blocks */
     L_.max_slots = max_slots;
 }
 

@@ -313,51 +313,86 @@ __device__ bool dev_wave_feasible(const FrPlanDev &P, int x, int z, float h)
  * coordinates are the reference's expressions, so the result is its boolean. */
 #define FR_GROUP 7
 #define FR_GROUP_FREE 6
+#define FR_PROBE_BATCH 24
 
 __device__ bool dev_lane_feasible(const FrPlanDev &P, int x, int z, float h)
 {
-    if (P.trav[z * P.W + x] & 1)
+    const uint8_t *D = P.dist;
+    const int PW = P.PW, pad = P.pad;
+    /* the three bytes every test starts from, loaded together */
+    const int tv = P.trav[z * P.W + x];
+    const int dc = D[(z + pad) * PW + x + pad];
+    const int d00 = D[pad * PW + pad];
+    if (tv & 1)
         return true;
     float s, c;
     fr_sincosf(h, &s, &c);
-    const uint8_t *D = P.dist;
-    const int PW = P.PW, pad = P.pad;
     if (c != c || s != s)
-        return D[pad * PW + pad] != 0;
-    if (D[(z + pad) * PW + x + pad] >= pad)
+        return d00 != 0;
+    if (dc >= pad)
         return true;
     const int mx = P.mx, mz = P.mz;
+    const int gr = (2 * mz + FR_GROUP) / FR_GROUP, gc = (2 * mx + FR_GROUP) / FR_GROUP, ng = gr * gc;
+    if (ng > 64)
+        return dev_feasible(P, x, z, h);
     const float fxc = (float)x, fzc = (float)z;
-    for (int i0 = -mz; i0 <= mz; i0 += FR_GROUP) {
-        const int i1 = i0 + FR_GROUP - 1 < mz ? i0 + FR_GROUP - 1 : mz;
-        const int ic = i0 + FR_GROUP / 2 < i1 ? i0 + FR_GROUP / 2 : i1;
-        const float ics = (float)ic * s, icc = (float)ic * c;
-        for (int j0 = -mx; j0 <= mx; j0 += FR_GROUP) {
-            const int j1 = j0 + FR_GROUP - 1 < mx ? j0 + FR_GROUP - 1 : mx;
-            const int jc = j0 + FR_GROUP / 2 < j1 ? j0 + FR_GROUP / 2 : j1;
-            const float fjc = (float)jc;
-            const int px = (int)(fjc * c - ics + fxc);
-            const int pz = (int)(fjc * s + icc + fzc);
-            const int dp = D[(pz + pad) * PW + (px + pad)];
-            if (dp == 0)
+    /* probes, loaded FR_PROBE_BATCH at a time so the loads overlap */
+    unsigned long long open = 0;
+    for (int g0 = 0; g0 < ng; g0 += FR_PROBE_BATCH) {
+        int dp[FR_PROBE_BATCH];
+#pragma unroll
+        for (int k = 0; k < FR_PROBE_BATCH; ++k) {
+            const int g = g0 + k;
+            dp[k] = 255;
+            if (g < ng) {
+                const int a = g / gc, bcol = g - (g / gc) * gc;
+                const int i0 = -mz + a * FR_GROUP, j0 = -mx + bcol * FR_GROUP;
+                const int i1 = i0 + FR_GROUP - 1 < mz ? i0 + FR_GROUP - 1 : mz;
+                const int j1 = j0 + FR_GROUP - 1 < mx ? j0 + FR_GROUP - 1 : mx;
+                const int ic = i0 + FR_GROUP / 2 < i1 ? i0 + FR_GROUP / 2 : i1;
+                const int jc = j0 + FR_GROUP / 2 < j1 ? j0 + FR_GROUP / 2 : j1;
+                const float is = (float)ic * s, icc = (float)ic * c, fj = (float)jc;
+                const int px = (int)(fj * c - is + fxc);
+                const int pz = (int)(fj * s + icc + fzc);
+                dp[k] = D[(pz + pad) * PW + (px + pad)];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < FR_PROBE_BATCH; ++k) {
+            if (dp[k] == 0)
                 return false;
-            if (dp >= FR_GROUP_FREE)
-                continue;
-            for (int i = i0; i <= i1; ++i) {
-                const float is = (float)i * s;
-                const float ic2 = (float)i * c;
-                int j = j0;
-                while (j <= j1) {
+            if (dp[k] < FR_GROUP_FREE)
+                open |= 1ull << (g0 + k);
+        }
+    }
+    /* every sample of each group its probe could not certify, all loads of a
+     * group issued before any is tested */
+    while (open) {
+        const int g = __ffsll((long long)open) - 1;
+        open &= open - 1;
+        const int a = g / gc, bcol = g - (g / gc) * gc;
+        const int i0 = -mz + a * FR_GROUP, j0 = -mx + bcol * FR_GROUP;
+        const int i1 = i0 + FR_GROUP - 1 < mz ? i0 + FR_GROUP - 1 : mz;
+        const int j1 = j0 + FR_GROUP - 1 < mx ? j0 + FR_GROUP - 1 : mx;
+        int hit = 0;
+#pragma unroll
+        for (int di = 0; di < FR_GROUP; ++di) {
+            const int i = i0 + di;
+            const float is = (float)i * s;
+            const float ic2 = (float)i * c;
+#pragma unroll
+            for (int dj = 0; dj < FR_GROUP; ++dj) {
+                const int j = j0 + dj;
+                if (i <= i1 && j <= j1) {
                     const float fj = (float)j;
                     const int xl = (int)(fj * c - is + fxc);
                     const int zl = (int)(fj * s + ic2 + fzc);
-                    const int d = D[(zl + pad) * PW + (xl + pad)];
-                    if (d == 0)
-                        return false;
-                    j += (d > 2) ? (d - 1) : 1;
+                    hit |= D[(zl + pad) * PW + (xl + pad)] == 0;
                 }
             }
         }
+        if (hit)
+            return false;
     }
     return true;
 }
@@ -864,20 +899,65 @@ __global__ void __launch_bounds__(FR_BLOCK) k_select(const FrPlanDev *plans, FrC
  * grid or that completes max_size cells; the end heading is the chain value
  * after the last step executed.  Every rollout reads the pre-expansion grid;
  * the end cell is claimed here and written by k_commit. */
-__global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *plans, FrCtl *ctl, FrExp *q, const int *counters)
+#define FR_ROLL_BUF 128
+
+/* Footprint tests and cost of the first m (<= 64) buffered cells of a
+ * rollout, one cell per lane, all lanes' loads in flight together; the cost
+ * is added cell by cell in walk order.  False when any footprint collides. */
+__device__ bool dev_roll_flush(const FrPlanDev &P, const int *bx, const int *bz, const float *bh, int m,
+                               float &node_cost)
 {
+    const int lane = dev_lane();
+    float add = 0;
+    bool bad = false;
+    if (lane < m) {
+        const int cx = bx[lane], cz = bz[lane];
+        add = (P.intrinsic ? P.intrinsic[cz * P.W + cx] : 0.0f) + 1.0f;
+        bad = !dev_lane_feasible(P, cx, cz, bh[lane]);
+    }
+    if (__ballot(bad))
+        return false;
+    for (int r = 0; r < m; ++r)
+        node_cost += dev_bcast_f(add, r);
+    return true;
+}
+
+__global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *__restrict__ plans, const FrCtl *__restrict__ ctl,
+                                                     FrExp *__restrict__ q, const int *__restrict__ counters)
+{
+    __shared__ int s_bx[FR_BLOCK / 64][FR_ROLL_BUF];
+    __shared__ int s_bz[FR_BLOCK / 64][FR_ROLL_BUF];
+    __shared__ float s_bh[FR_BLOCK / 64][FR_ROLL_BUF];
     const int n = counters[FR_CNT_EXP];
     const int wpb = blockDim.x >> 6;
-    const int nw = gridDim.x * wpb;
+    /* wave index made uniform, so a rollout's descriptor and its plan's
+     * fields are read through scalar loads */
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = dev_lane();
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (int i = blockIdx.x * wpb + (int)(threadIdx.x >> 6); i < n; i += nw) {
+    int *bx = s_bx[wv], *bz = s_bz[wv];
+    float *bh = s_bh[wv];
+    /* Workgroups are dealt to the eight XCDs round robin (block b runs on
+     * XCD b % 8); each XCD takes one contiguous eighth of the queue, which
+     * holds a plan's rollouts together, so a plan's planes are read through
+     * one XCD's L2.  The grid is a multiple of 8 blocks. */
+    const int xcd = (int)(blockIdx.x & 7u), jb = (int)(blockIdx.x >> 3);
+    const int nbx = (int)(gridDim.x >> 3);
+    if (jb >= nbx)
+        return;
+    const int lo = (int)((long long)n * xcd / 8), hi = (int)((long long)n * (xcd + 1) / 8);
+    for (int i = lo + jb * wpb + wv; i < hi; i += nbx * wpb) {
         const FrExp e = q[i];
-        const FrPlanDev &P = plans[e.slot];
-        const FrCtl &C = ctl[e.slot];
-        const int src = e.src;
+        const int slot = __builtin_amdgcn_readfirstlane(e.slot);
+        const FrPlanDev &P = plans[slot];
+        const FrCtl &C = ctl[slot];
+        const int src = __builtin_amdgcn_readfirstlane(e.src);
         const int W = P.W, H = P.H;
         const int sz = src / W, sx = src - sz * W;
+        /* the node's heading and cost are fetched before the draws and the
+         * trigonometry, so their latency overlaps that arithmetic */
+        float h = P.heading[src];
+        float node_cost = P.cost[src];
         float us, ul;
         fr_expansion_draws(P.seed, P.stream, (uint32_t)C.iter, (uint32_t)src, &us, &ul);
         float steer = (2.0f * us - 1.0f) * (float)P.max_steer_rad;
@@ -899,11 +979,10 @@ __global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *plans, FrC
         const int max_size = fr_trunc_f(psize) + 1;
         const int loop_max = fr_trunc_f((float)(100 * max_size) / dt);
         const double ogx = (double)P.og_x, ogy = (double)P.og_y;
-        float h = P.heading[src];
-        float node_cost = P.cost[src];
         int size = 0, last_x = sx, last_z = sz, prev_x = sx, prev_z = sz;
         float end_h = h;
         bool ok = true;
+        int nb = 0; /* cells buffered, footprints not yet tested */
         for (int k0 = 0;; k0 += 64) {
             const int k = k0 + lane;
             /* heading before (hk) and after (hk1) this lane's step */
@@ -951,38 +1030,50 @@ __global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *plans, FrC
             const bool ends_at = !ends_before && t_at < 64;
             const int lim = ends_before ? t_before : (ends_at ? t_at + 1 : 64);
             const unsigned long long acc = m_in & (lim >= 64 ? ~0ull : ((1ull << lim) - 1ull));
-            /* cost in step order, then the footprints (any failure rejects) */
-            float add = 0;
-            bool bad = false;
+            /* the new cells join the buffer in walk order; their footprints
+             * and cost are taken 64 at a time (any collision rejects, so the
+             * tests can wait until the walk's geometry is known) */
             if ((acc >> lane) & 1ull) {
-                add = (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
-                bad = !dev_lane_feasible(P, cx, cz, hk1);
+                const int r = nb + __popcll(acc & below);
+                bx[r] = cx;
+                bz[r] = cz;
+                bh[r] = hk1;
             }
-            for (unsigned long long cm = acc; cm; cm &= cm - 1)
-                node_cost += dev_bcast_f(add, __ffsll((long long)cm) - 1);
-            if (__ballot(bad)) {
-                ok = false;
-                break;
-            }
+            nb += __popcll(acc);
             size += __popcll(acc);
             if (acc) {
                 const int l = 63 - __clzll((long long)acc);
                 last_x = dev_bcast_i(cx, l);
                 last_z = dev_bcast_i(cz, l);
             }
+            bool done = true;
             if (ends_before) {
                 end_h = dev_bcast_f(hk, t_before);
-                break;
-            }
-            if (ends_at) {
+            } else if (ends_at) {
                 end_h = dev_bcast_f(hk1, t_at);
-                break;
+            } else {
+                done = false;
+                h = hh;
+                xf = xx;
+                yf = yy;
+                prev_x = dev_bcast_i(cx, 63);
+                prev_z = dev_bcast_i(cz, 63);
             }
-            h = hh;
-            xf = xx;
-            yf = yy;
-            prev_x = dev_bcast_i(cx, 63);
-            prev_z = dev_bcast_i(cz, 63);
+            while (nb >= 64 || (done && nb > 0)) {
+                const int m = nb < 64 ? nb : 64;
+                if (!dev_roll_flush(P, bx, bz, bh, m, node_cost)) {
+                    ok = false;
+                    break;
+                }
+                nb -= m;
+                if (lane < nb) {
+                    bx[lane] = bx[64 + lane];
+                    bz[lane] = bz[64 + lane];
+                    bh[lane] = bh[64 + lane];
+                }
+            }
+            if (done || !ok)
+                break;
         }
         if (lane != 0)
             continue;
