@@ -257,65 +257,50 @@ __device__ __forceinline__ float dev_bcast_f(float v, int lane)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-/* path_feasible.cu:13-78 with the 64 lanes sharing the (2 mx + 1) x (2 mz + 1)
- * samples; the sample coordinates are the reference's expressions. */
-__device__ bool dev_wave_feasible(const FrPlanDev &P, int x, int z, float h)
-{
-    if (dev_fp_certified(P, x, z, h))
-        return true;
-    float s, c;
-    fr_sincosf(h, &s, &c);
-    const uint8_t *D = P.dist;
-    const int PW = P.PW, pad = P.pad;
-    if (c != c || s != s)
-        return D[pad * PW + pad] != 0;
-    const int mx = P.mx, mz = P.mz;
-    const int nx = 2 * mx + 1, ns = nx * (2 * mz + 1);
-    const int lane = dev_lane();
-    int i = lane / nx, j = lane - (lane / nx) * nx;
-    const int di = 64 / nx, dj = 64 - di * nx;
-    const float fxc = (float)x, fzc = (float)z;
-    for (int k0 = 0; k0 < ns; k0 += 8 * 64) {
-        bool bad = false;
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            if (k0 + r * 64 + lane < ns) {
-                const float is = (float)(i - mz) * s;
-                const float ic = (float)(i - mz) * c;
-                const float fj = (float)(j - mx);
-                const float fx = fj * c - is + fxc;
-                const float fz = fj * s + ic + fzc;
-                const int xl = (int)fx;
-                const int zl = (int)fz;
-                bad |= D[(zl + pad) * PW + (xl + pad)] == 0;
-            }
-            i += di;
-            j += dj;
-            if (j >= nx) {
-                j -= nx;
-                i += 1;
-            }
-        }
-        if (__ballot(bad))
-            return false;
-    }
-    return true;
-}
-
 /* path_feasible.cu:18-78 for one lane's own pose (lanes test different
- * poses at once).  After the all-angle bit and the centre certificate, the
- * (2 mz + 1) x (2 mx + 1) samples are taken in 7 x 7 groups: the group's
- * middle sample is a probe; every other sample of the group lies within
- * hypot(3, 3) of it before truncation, so within Chebyshev distance 5 after,
- * and a probe at distance >= 6 from every obstacle certifies its group.  A
- * probe at distance 0 is itself an obstacle sample.  Only the remaining
- * groups are sampled (with the row skipping of dev_feasible).  The sample
- * coordinates are the reference's expressions, so the result is its boolean. */
+ * poses at once), in two phases.  After the all-angle bit and the centre
+ * certificate, the (2 mz + 1) x (2 mx + 1) samples are taken in 7 x 7 groups:
+ * the group's middle sample is a probe; every other sample of the group lies
+ * within hypot(3, 3) of it before truncation, so within Chebyshev distance 5
+ * after, and a probe at distance >= 6 from every obstacle certifies its
+ * group.  A probe at distance 0 is itself an obstacle sample.  The probe
+ * phase reads every probe; the group phase samples the groups their probes
+ * left open, skipping the samples whose cell lies within the probe's own
+ * distance of the probe's cell (those cells are free).  The sample
+ * coordinates are the reference's expressions, so the result is its boolean;
+ * the two phases may run at different times since any collision rejects. */
 #define FR_GROUP 7
 #define FR_GROUP_FREE 6
-#define FR_PROBE_BATCH 24
+#define FR_MAX_GROUPS 32
+#define FR_PROBE_BATCH 12
 
-__device__ bool dev_lane_feasible(const FrPlanDev &P, int x, int z, float h)
+struct FrOpen {
+    float s, c;              /* sin, cos of the heading */
+    uint32_t open;           /* groups left open by their probes */
+    unsigned long long code; /* per open group: min(probe distance, 4) - 1, 2 bits */
+};
+
+__device__ __forceinline__ void dev_fp_cell(int j, int i, float s, float c, float fxc, float fzc, int *xl, int *zl)
+{
+    const float is = (float)i * s, ic = (float)i * c, fj = (float)j;
+    *xl = (int)(fj * c - is + fxc);
+    *zl = (int)(fj * s + ic + fzc);
+}
+
+/* group in row a, column b of the group grid */
+__device__ __forceinline__ void dev_group_box(const FrPlanDev &P, int a, int b, int *i0, int *i1, int *j0, int *j1,
+                                              int *ic, int *jc)
+{
+    *i0 = -P.mz + a * FR_GROUP;
+    *j0 = -P.mx + b * FR_GROUP;
+    *i1 = *i0 + FR_GROUP - 1 < P.mz ? *i0 + FR_GROUP - 1 : P.mz;
+    *j1 = *j0 + FR_GROUP - 1 < P.mx ? *j0 + FR_GROUP - 1 : P.mx;
+    *ic = *i0 + FR_GROUP / 2 < *i1 ? *i0 + FR_GROUP / 2 : *i1;
+    *jc = *j0 + FR_GROUP / 2 < *j1 ? *j0 + FR_GROUP / 2 : *j1;
+}
+
+/* Probe phase: -1 collision, 1 free, 0 open (groups to sample in o). */
+__device__ int dev_lane_probe(const FrPlanDev &P, int x, int z, float h, FrOpen *o)
 {
     const uint8_t *D = P.dist;
     const int PW = P.PW, pad = P.pad;
@@ -324,70 +309,83 @@ __device__ bool dev_lane_feasible(const FrPlanDev &P, int x, int z, float h)
     const int dc = D[(z + pad) * PW + x + pad];
     const int d00 = D[pad * PW + pad];
     if (tv & 1)
-        return true;
+        return 1;
     float s, c;
     fr_sincosf(h, &s, &c);
     if (c != c || s != s)
-        return d00 != 0;
+        return d00 != 0 ? 1 : -1;
     if (dc >= pad)
-        return true;
-    const int mx = P.mx, mz = P.mz;
-    const int gr = (2 * mz + FR_GROUP) / FR_GROUP, gc = (2 * mx + FR_GROUP) / FR_GROUP, ng = gr * gc;
-    if (ng > 64)
-        return dev_feasible(P, x, z, h);
+        return 1;
+    const int gr = (2 * P.mz + FR_GROUP) / FR_GROUP, gc = (2 * P.mx + FR_GROUP) / FR_GROUP, ng = gr * gc;
+    if (ng > FR_MAX_GROUPS)
+        return dev_feasible(P, x, z, h) ? 1 : -1;
     const float fxc = (float)x, fzc = (float)z;
-    /* probes, loaded FR_PROBE_BATCH at a time so the loads overlap */
-    unsigned long long open = 0;
+    /* probes in batches whose loads are all in flight together */
+    uint32_t open = 0;
+    unsigned long long code = 0;
+    int a = 0, b = 0; /* group g = a * gc + b */
     for (int g0 = 0; g0 < ng; g0 += FR_PROBE_BATCH) {
         int dp[FR_PROBE_BATCH];
 #pragma unroll
         for (int k = 0; k < FR_PROBE_BATCH; ++k) {
-            const int g = g0 + k;
             dp[k] = 255;
-            if (g < ng) {
-                const int a = g / gc, bcol = g - (g / gc) * gc;
-                const int i0 = -mz + a * FR_GROUP, j0 = -mx + bcol * FR_GROUP;
-                const int i1 = i0 + FR_GROUP - 1 < mz ? i0 + FR_GROUP - 1 : mz;
-                const int j1 = j0 + FR_GROUP - 1 < mx ? j0 + FR_GROUP - 1 : mx;
-                const int ic = i0 + FR_GROUP / 2 < i1 ? i0 + FR_GROUP / 2 : i1;
-                const int jc = j0 + FR_GROUP / 2 < j1 ? j0 + FR_GROUP / 2 : j1;
-                const float is = (float)ic * s, icc = (float)ic * c, fj = (float)jc;
-                const int px = (int)(fj * c - is + fxc);
-                const int pz = (int)(fj * s + icc + fzc);
-                dp[k] = D[(pz + pad) * PW + (px + pad)];
+            if (g0 + k < ng) {
+                int i0, i1, j0, j1, ic, jc, px, pz;
+                dev_group_box(P, a, b, &i0, &i1, &j0, &j1, &ic, &jc);
+                dev_fp_cell(jc, ic, s, c, fxc, fzc, &px, &pz);
+                dp[k] = D[(unsigned)((pz + pad) * PW + (px + pad))];
+            }
+            if (++b == gc) {
+                b = 0;
+                ++a;
             }
         }
 #pragma unroll
         for (int k = 0; k < FR_PROBE_BATCH; ++k) {
             if (dp[k] == 0)
-                return false;
-            if (dp[k] < FR_GROUP_FREE)
-                open |= 1ull << (g0 + k);
+                return -1;
+            if (dp[k] < FR_GROUP_FREE) {
+                const int g = g0 + k;
+                open |= 1u << g;
+                code |= (unsigned long long)((dp[k] < 4 ? dp[k] : 4) - 1) << (2 * g);
+            }
         }
     }
-    /* every sample of each group its probe could not certify, all loads of a
-     * group issued before any is tested */
+    o->s = s;
+    o->c = c;
+    o->open = open;
+    o->code = code;
+    return open ? 0 : 1;
+}
+
+/* Group phase: false when a sample of an open group is an obstacle. */
+__device__ bool dev_lane_groups(const FrPlanDev &P, int x, int z, const FrOpen &o)
+{
+    const uint8_t *D = P.dist;
+    const int PW = P.PW, pad = P.pad;
+    const int gc = (2 * P.mx + FR_GROUP) / FR_GROUP;
+    const float fxc = (float)x, fzc = (float)z;
+    uint32_t open = o.open;
     while (open) {
-        const int g = __ffsll((long long)open) - 1;
+        const int g = __ffs((int)open) - 1;
         open &= open - 1;
-        const int a = g / gc, bcol = g - (g / gc) * gc;
-        const int i0 = -mz + a * FR_GROUP, j0 = -mx + bcol * FR_GROUP;
-        const int i1 = i0 + FR_GROUP - 1 < mz ? i0 + FR_GROUP - 1 : mz;
-        const int j1 = j0 + FR_GROUP - 1 < mx ? j0 + FR_GROUP - 1 : mx;
+        int i0, i1, j0, j1, ic, jc, pcx, pcz;
+        const int a = g / gc;
+        dev_group_box(P, a, g - a * gc, &i0, &i1, &j0, &j1, &ic, &jc);
+        dev_fp_cell(jc, ic, o.s, o.c, fxc, fzc, &pcx, &pcz);
+        const int pd = (int)((o.code >> (2 * g)) & 3ull) + 1;
         int hit = 0;
 #pragma unroll
         for (int di = 0; di < FR_GROUP; ++di) {
-            const int i = i0 + di;
-            const float is = (float)i * s;
-            const float ic2 = (float)i * c;
 #pragma unroll
             for (int dj = 0; dj < FR_GROUP; ++dj) {
-                const int j = j0 + dj;
+                const int i = i0 + di, j = j0 + dj;
                 if (i <= i1 && j <= j1) {
-                    const float fj = (float)j;
-                    const int xl = (int)(fj * c - is + fxc);
-                    const int zl = (int)(fj * s + ic2 + fzc);
-                    hit |= D[(zl + pad) * PW + (xl + pad)] == 0;
+                    int xl, zl;
+                    dev_fp_cell(j, i, o.s, o.c, fxc, fzc, &xl, &zl);
+                    const int ax = xl > pcx ? xl - pcx : pcx - xl, az = zl > pcz ? zl - pcz : pcz - zl;
+                    if (ax >= pd || az >= pd)
+                        hit |= D[(unsigned)((zl + pad) * PW + (xl + pad))] == 0;
                 }
             }
         }
@@ -395,6 +393,15 @@ __device__ bool dev_lane_feasible(const FrPlanDev &P, int x, int z, float h)
             return false;
     }
     return true;
+}
+
+__device__ bool dev_lane_feasible(const FrPlanDev &P, int x, int z, float h)
+{
+    FrOpen o;
+    const int r = dev_lane_probe(P, x, z, h, &o);
+    if (r != 0)
+        return r > 0;
+    return dev_lane_groups(P, x, z, o);
 }
 
 /* One Hermite point of graph_direct_goal_connection.cu:16-167: position,
@@ -426,12 +433,54 @@ __device__ __forceinline__ bool dev_hermite_point(const FrPlanDev &P, const FrHe
     return x >= 0 && x < P.W && z >= 0 && z < P.H;
 }
 
+#define FR_OPEN_BUF 128
+
+/* Per-wave buffer of footprints whose probes left groups open. */
+struct FrOpenBuf {
+    int x[FR_OPEN_BUF], z[FR_OPEN_BUF];
+    float s[FR_OPEN_BUF], c[FR_OPEN_BUF];
+    uint32_t open[FR_OPEN_BUF];
+    unsigned long long code[FR_OPEN_BUF];
+};
+
+/* Group phase of the first m (<= 64) buffered footprints, one per lane; the
+ * rest move to the front.  False when any collides. */
+__device__ bool dev_open_flush(const FrPlanDev &P, FrOpenBuf &b, int m, int *nb)
+{
+    const int lane = dev_lane();
+    bool bad = false;
+    if (lane < m) {
+        FrOpen o;
+        o.s = b.s[lane];
+        o.c = b.c[lane];
+        o.open = b.open[lane];
+        o.code = b.code[lane];
+        bad = !dev_lane_groups(P, b.x[lane], b.z[lane], o);
+    }
+    if (__ballot(bad))
+        return false;
+    *nb -= m;
+    if (lane < *nb) {
+        b.x[lane] = b.x[m + lane];
+        b.z[lane] = b.z[m + lane];
+        b.s[lane] = b.s[m + lane];
+        b.c[lane] = b.c[m + lane];
+        b.open[lane] = b.open[m + lane];
+        b.code[lane] = b.code[m + lane];
+    }
+    return true;
+}
+
 /* dev_check_direct with the curve's points spread over the lanes.  A point
  * contributes iff its cell is in the grid and differs from the cell of the
  * previous such point (the reference's last_x / last_z test: duplicates of
  * the last cell never change it, so "last accepted" = "last in-grid").  The
- * cost is summed lane by lane in point order, as the reference adds it. */
-__device__ float dev_wave_check_direct(const FrPlanDev &P, float max_curv, int x, int z, float lh, int gx, int gz,
+ * cost is summed lane by lane in point order, as the reference adds it.
+ * Pass 0 tests curvature and the centre cells; pass 1 the footprints, whose
+ * probe phase runs chunk by chunk while the groups left open wait in a
+ * per-wave buffer and are sampled 64 at a time (a collision anywhere rejects,
+ * so a curve that some probe rejects never has its groups sampled). */
+__device__ __forceinline__ float dev_wave_check_direct(const FrPlanDev &P, float max_curv, int x, int z, float lh, int gx, int gz,
                                        float gh, bool safe_zone, bool host_nan, float base)
 {
     const int W = P.W;
@@ -458,6 +507,10 @@ __device__ float dev_wave_check_direct(const FrPlanDev &P, float max_curv, int x
     hm.dgz = (double)gz;
     hm.denom = (double)(num_points - 1);
     const bool skip_fp = safe_zone && ((P.trav[pos] & 2) != 0);
+    __shared__ FrOpenBuf s_open[FR_BLOCK / 64];
+    FrOpenBuf &ob = s_open[__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))];
+    int nb = 0;
+    const unsigned long long below = (1ull << lane) - 1ull;
     float node_cost = base;
     int last_x = -1, last_z = -1;
     for (int pass = 0; pass < 2; ++pass) {
@@ -519,15 +572,16 @@ __device__ float dev_wave_check_direct(const FrPlanDev &P, float max_curv, int x
                     return -1;
             } else {
                 float add = 0;
-                bool bad = false;
+                int pr = 1;
+                FrOpen o;
                 if (acc) {
                     add = (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
                     if (!skip_fp) {
                         const float hd = (float)((double)fr_atan2f((float)ddz, (float)ddx) + FR_HALF_PI_D);
-                        bad = !dev_lane_feasible(P, cx, cz, hd);
+                        pr = dev_lane_probe(P, cx, cz, hd, &o);
                     }
                 }
-                if (__ballot(bad))
+                if (__ballot(pr < 0))
                     return -1;
                 unsigned long long cm = am;
                 while (cm) {
@@ -535,9 +589,25 @@ __device__ float dev_wave_check_direct(const FrPlanDev &P, float max_curv, int x
                     cm &= cm - 1;
                     node_cost += dev_bcast_f(add, l);
                 }
+                const unsigned long long om = __ballot(pr == 0);
+                if (pr == 0) {
+                    const int r = nb + __popcll(om & below);
+                    ob.x[r] = cx;
+                    ob.z[r] = cz;
+                    ob.s[r] = o.s;
+                    ob.c[r] = o.c;
+                    ob.open[r] = o.open;
+                    ob.code[r] = o.code;
+                }
+                nb += __popcll(om);
+                if (nb >= 64 && !dev_open_flush(P, ob, 64, &nb))
+                    return -1;
             }
         }
     }
+    while (nb > 0)
+        if (!dev_open_flush(P, ob, nb < 64 ? nb : 64, &nb))
+            return -1;
     if (last_x != gx && last_z != gz)
         return -1;
     return node_cost;
@@ -1522,16 +1592,18 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_demand(const FrPlanDev *plans, 
     }
 }
 
-__global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *plans, const int2 *dq, long dq_cap, const int *counters)
+__global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *__restrict__ plans, const int2 *__restrict__ dq,
+                                                      long dq_cap, const int *__restrict__ counters)
 {
     long n = counters[FR_CNT_DEMAND];
     if (n > dq_cap)
         n = dq_cap;
     const int wpb = blockDim.x >> 6;
     const long nw = (long)gridDim.x * wpb;
-    for (long i = blockIdx.x * (long)wpb + (threadIdx.x >> 6); i < n; i += nw) {
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (long i = blockIdx.x * (long)wpb + wv; i < n; i += nw) {
         const int2 d = dq[i];
-        dev_gc_cell(plans[d.x], d.y);
+        dev_gc_cell(plans[__builtin_amdgcn_readfirstlane(d.x)], __builtin_amdgcn_readfirstlane(d.y));
     }
 }
 
