@@ -35,6 +35,7 @@ enum FrStatus {
 struct FrPlanDev {
     int W, H, N;
     int pad, PW, PH;          /* padded distance plane (W + 2 pad) x (H + 2 pad) */
+    int TW;                   /* 8 x 8 tiles per tile row of the padded plane */
     int mx, mz;               /* half footprint (min_dist / 2, truncated) */
     int lbx, lbz, ubx, ubz;   /* ego box, pixels inside are ignored */
     float og_x, og_y;         /* map-frame origin in grid pixels */
@@ -62,7 +63,8 @@ struct FrPlanDev {
     /* scratch planes */
     unsigned long long *claim;
     uint8_t *mark;
-    uint8_t *dist;            /* padded Chebyshev distance to the nearest obstacle pixel */
+    uint8_t *dist;            /* padded Chebyshev distance to the nearest obstacle pixel, in 8 x 8
+                                 tiles of 64 bytes (fr_dist_at) */
     uint8_t *dist_tmp;
     uint8_t *trav;            /* bit0: feasible for all headings; bit1: safe-zone bit 0x100 */
     uint32_t *region;
@@ -79,6 +81,20 @@ struct FrPlanDev {
     uint4 *zero_base;
     long zero_words;
 };
+
+/* Byte offset of padded cell (xp, zp) in the tiled distance plane: tiles of
+ * 8 x 8 cells (64 bytes) in row-major tile order, cells row-major inside a
+ * tile, so a footprint's samples and a curve's consecutive points fall in a
+ * few cache lines whatever their direction. */
+#if defined(__HIPCC__)
+__host__ __device__ __forceinline__
+#else
+inline
+#endif
+unsigned fr_dist_at(int TW, int xp, int zp)
+{
+    return ((unsigned)((zp >> 3) * TW + (xp >> 3)) << 6) | ((unsigned)(zp & 7) << 3) | (unsigned)(xp & 7);
+}
 
 /* Mutable per-plan control block. */
 struct FrCtl {

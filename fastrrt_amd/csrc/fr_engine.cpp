@@ -150,11 +150,13 @@ void FrSlab::set_footprint(int mx_, int mz_, hipStream_t st)
     pad = (int)std::ceil(hyp) + 2;
     PW = W + 2 * pad;
     PH = H + 2 * pad;
+    TW = (PW + 7) / 8;
     const size_t PWH = (size_t)PW * PH;
-    FR_CHECK(hipMalloc(&dist_base_, 2 * fr_align(PWH)));
+    const size_t tiled = (size_t)TW * ((PH + 7) / 8) * 64; /* whole 8 x 8 tiles */
+    FR_CHECK(hipMalloc(&dist_base_, fr_align(tiled) + fr_align(PWH)));
     dist = (uint8_t *)dist_base_;
-    dist_tmp = (uint8_t *)dist_base_ + fr_align(PWH);
-    FR_CHECK(hipMemsetAsync(dist_base_, 0, 2 * fr_align(PWH), st));
+    dist_tmp = (uint8_t *)dist_base_ + fr_align(tiled);
+    FR_CHECK(hipMemsetAsync(dist_base_, 0, fr_align(tiled) + fr_align(PWH), st));
 }
 
 void FrSlab::allocate(int W_, int H_, int mx_, int mz_, int path_cap_, bool with_intrinsic_, hipStream_t st)
@@ -274,6 +276,7 @@ FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
     FR_CHECK(hipMalloc(&L_.chk, sizeof(int2) * L_.chk_cap));
     L_.demand_cap = queue_cap + 1;
     FR_CHECK(hipMalloc(&L_.demand, sizeof(int2) * L_.demand_cap));
+    FR_CHECK(hipMalloc(&L_.demand2, sizeof(int2) * L_.demand_cap));
     FR_CHECK(hipMemsetAsync(L_.ctl, 0, sizeof(FrCtl) * max_slots, st_));
     FR_CHECK(hipMemsetAsync(L_.n_tiles, 0, sizeof(int), st_));
     FR_CHECK(hipMemsetAsync(L_.counters, 0, sizeof(int) * FR_CNT_N, st_));
@@ -306,6 +309,7 @@ FrEngine::~FrEngine()
     (void)hipFree(L_.pairs);
     (void)hipFree(L_.chk);
     (void)hipFree(L_.demand);
+    (void)hipFree(L_.demand2);
     if (own_stream_)
         (void)hipStreamDestroy(st_);
 }
@@ -319,6 +323,7 @@ void FrEngine::bind(int slot, FrSlab &s, const FrPlanParams &p, const FrFrameVie
     d.pad = s.pad;
     d.PW = s.PW;
     d.PH = s.PH;
+    d.TW = s.TW;
     d.mx = s.mx;
     d.mz = s.mz;
     d.lbx = p.lbx;

@@ -16,7 +16,7 @@
 
 #include "fr_device.h"
 
-enum { FR_CNT_EXP = 0, FR_CNT_FRESH = 1, FR_CNT_PAIRS = 2, FR_CNT_DEMAND = 3, FR_CNT_N = 8 };
+enum { FR_CNT_EXP = 0, FR_CNT_FRESH = 1, FR_CNT_PAIRS = 2, FR_CNT_DEMAND = 3, FR_CNT_DEMAND2 = 4, FR_CNT_N = 8 };
 
 /* goal-connection planes at setup: untouched, evaluated when first needed
  * (lazy), or evaluated for every cell (eager) */
@@ -36,7 +36,8 @@ struct FrLaunch {
     int2 *chk;                /* per rollout: (cell, heading bits) footprint tests, FR_CHK_STRIDE each */
     long chk_cap;
     int2 *demand;             /* (slot, cell) goal-connection evaluations queued this iteration */
-    long demand_cap;
+    int2 *demand2;            /* the queued cells whose goal curves passed curvature and centres */
+    long demand_cap;          /* of both */
     int *n_slots;             /* device: entries of `slots` */
     int grid_tiles, grid_items, grid_slots, grid_waves;
     int max_slots;
@@ -143,7 +144,7 @@ class FrSlab {
     void reset(hipStream_t st);
     size_t bytes() const { return bytes_; }
     bool allocated() const { return base_ != nullptr; }
-    int W = 0, H = 0, pad = 0, PW = 0, PH = 0, mx = -1, mz = -1, path_cap = 0;
+    int W = 0, H = 0, pad = 0, PW = 0, PH = 0, TW = 0, mx = -1, mz = -1, path_cap = 0;
     bool with_intrinsic = false;
     uint8_t *type = nullptr;
     int32_t *derive = nullptr, *px = nullptr, *pz = nullptr;

@@ -56,9 +56,9 @@ __device__ bool dev_feasible(const FrPlanDev &P, int x, int z, float h)
     float s, c;
     fr_sincosf(h, &s, &c);
     const uint8_t *D = P.dist;
-    const int PW = P.PW, pad = P.pad;
+    const int TW = P.TW, pad = P.pad;
     if (c != c || s != s) /* every sample truncates to pixel (0, 0) */
-        return D[pad * PW + pad] != 0;
+        return D[fr_dist_at(TW, pad, pad)] != 0;
     const float fxc = (float)x, fzc = (float)z;
     const int mx = P.mx, mz = P.mz;
     for (int i = -mz; i <= mz; ++i) {
@@ -71,7 +71,7 @@ __device__ bool dev_feasible(const FrPlanDev &P, int x, int z, float h)
             const float fz = fj * s + ic + fzc;
             const int xl = (int)fx;
             const int zl = (int)fz;
-            const int d = D[(zl + pad) * PW + (xl + pad)];
+            const int d = D[fr_dist_at(TW, xl + pad, zl + pad)];
             if (d == 0)
                 return false;
             j += (d > 2) ? (d - 1) : 1;
@@ -87,7 +87,7 @@ __device__ __forceinline__ bool dev_fp_certified(const FrPlanDev &P, int x, int 
 {
     if (P.trav[z * P.W + x] & 1)
         return true;
-    return (h - h) == 0.0f && P.dist[(z + P.pad) * P.PW + x + P.pad] >= P.pad;
+    return (h - h) == 0.0f && P.dist[fr_dist_at(P.TW, x + P.pad, z + P.pad)] >= P.pad;
 }
 
 /* kinematics.cu:123-231.  Returns 0 when rejected. */
@@ -228,7 +228,7 @@ __device__ float dev_check_direct(const FrPlanDev &P, float max_curv, int x, int
                     if (k > max_curv)
                         return -1;
                 }
-                if (!skip_fp && !(P.trav[cz * W + cx] & 1) && P.dist[(cz + P.pad) * P.PW + cx + P.pad] == 0)
+                if (!skip_fp && !(P.trav[cz * W + cx] & 1) && P.dist[fr_dist_at(P.TW, cx + P.pad, cz + P.pad)] == 0)
                     return -1;
             } else if (!skip_fp) {
                 const float heading = (float)((double)fr_atan2f((float)ddz, (float)ddx) + FR_HALF_PI_D);
@@ -272,7 +272,7 @@ __device__ __forceinline__ float dev_bcast_f(float v, int lane)
 #define FR_GROUP 7
 #define FR_GROUP_FREE 6
 #define FR_MAX_GROUPS 32
-#define FR_PROBE_BATCH 12
+#define FR_PROBE_BATCH 24
 
 struct FrOpen {
     float s, c;              /* sin, cos of the heading */
@@ -303,43 +303,48 @@ __device__ __forceinline__ void dev_group_box(const FrPlanDev &P, int a, int b, 
 __device__ int dev_lane_probe(const FrPlanDev &P, int x, int z, float h, FrOpen *o)
 {
     const uint8_t *D = P.dist;
-    const int PW = P.PW, pad = P.pad;
-    /* the three bytes every test starts from, loaded together */
-    const int tv = P.trav[z * P.W + x];
-    const int dc = D[(z + pad) * PW + x + pad];
-    const int d00 = D[pad * PW + pad];
-    if (tv & 1)
-        return 1;
+    const int TW = P.TW, pad = P.pad;
     float s, c;
     fr_sincosf(h, &s, &c);
-    if (c != c || s != s)
+    const bool nan_h = c != c || s != s;
+    const int gr = (2 * P.mz + FR_GROUP) / FR_GROUP, gc = (2 * P.mx + FR_GROUP) / FR_GROUP, ng = gr * gc;
+    const bool grouped = ng <= FR_MAX_GROUPS;
+    const float fxc = (float)x, fzc = (float)z;
+    /* Every byte the test may need first - the cell's bits, the centre and
+     * origin distances, the first batch of probes - so that one round trip
+     * to memory serves the common cases.  A NaN heading probes the centre
+     * (its probes are not used then). */
+    const float sp = nan_h ? 0.0f : s, cp = nan_h ? 0.0f : c;
+    const int tv = P.trav[z * P.W + x];
+    const int dc = D[fr_dist_at(TW, x + pad, z + pad)];
+    const int d00 = D[fr_dist_at(TW, pad, pad)];
+    int dp[FR_PROBE_BATCH];
+    int a = 0, b = 0; /* group g = a * gc + b */
+#pragma unroll
+    for (int k = 0; k < FR_PROBE_BATCH; ++k) {
+        dp[k] = 255;
+        if (grouped && k < ng) {
+            int i0, i1, j0, j1, ic, jc, px, pz;
+            dev_group_box(P, a, b, &i0, &i1, &j0, &j1, &ic, &jc);
+            dev_fp_cell(jc, ic, sp, cp, fxc, fzc, &px, &pz);
+            dp[k] = D[fr_dist_at(TW, px + pad, pz + pad)];
+        }
+        if (++b == gc) {
+            b = 0;
+            ++a;
+        }
+    }
+    if (tv & 1)
+        return 1;
+    if (nan_h)
         return d00 != 0 ? 1 : -1;
     if (dc >= pad)
         return 1;
-    const int gr = (2 * P.mz + FR_GROUP) / FR_GROUP, gc = (2 * P.mx + FR_GROUP) / FR_GROUP, ng = gr * gc;
-    if (ng > FR_MAX_GROUPS)
+    if (!grouped)
         return dev_feasible(P, x, z, h) ? 1 : -1;
-    const float fxc = (float)x, fzc = (float)z;
-    /* probes in batches whose loads are all in flight together */
     uint32_t open = 0;
     unsigned long long code = 0;
-    int a = 0, b = 0; /* group g = a * gc + b */
-    for (int g0 = 0; g0 < ng; g0 += FR_PROBE_BATCH) {
-        int dp[FR_PROBE_BATCH];
-#pragma unroll
-        for (int k = 0; k < FR_PROBE_BATCH; ++k) {
-            dp[k] = 255;
-            if (g0 + k < ng) {
-                int i0, i1, j0, j1, ic, jc, px, pz;
-                dev_group_box(P, a, b, &i0, &i1, &j0, &j1, &ic, &jc);
-                dev_fp_cell(jc, ic, s, c, fxc, fzc, &px, &pz);
-                dp[k] = D[(unsigned)((pz + pad) * PW + (px + pad))];
-            }
-            if (++b == gc) {
-                b = 0;
-                ++a;
-            }
-        }
+    for (int g0 = 0;;) {
 #pragma unroll
         for (int k = 0; k < FR_PROBE_BATCH; ++k) {
             if (dp[k] == 0)
@@ -348,6 +353,24 @@ __device__ int dev_lane_probe(const FrPlanDev &P, int x, int z, float h, FrOpen 
                 const int g = g0 + k;
                 open |= 1u << g;
                 code |= (unsigned long long)((dp[k] < 4 ? dp[k] : 4) - 1) << (2 * g);
+            }
+        }
+        g0 += FR_PROBE_BATCH;
+        if (g0 >= ng)
+            break;
+        /* the next batch (footprints of more than FR_PROBE_BATCH groups) */
+#pragma unroll
+        for (int k = 0; k < FR_PROBE_BATCH; ++k) {
+            dp[k] = 255;
+            if (g0 + k < ng) {
+                int i0, i1, j0, j1, ic, jc, px, pz;
+                dev_group_box(P, a, b, &i0, &i1, &j0, &j1, &ic, &jc);
+                dev_fp_cell(jc, ic, s, c, fxc, fzc, &px, &pz);
+                dp[k] = D[fr_dist_at(TW, px + pad, pz + pad)];
+            }
+            if (++b == gc) {
+                b = 0;
+                ++a;
             }
         }
     }
@@ -362,7 +385,7 @@ __device__ int dev_lane_probe(const FrPlanDev &P, int x, int z, float h, FrOpen 
 __device__ bool dev_lane_groups(const FrPlanDev &P, int x, int z, const FrOpen &o)
 {
     const uint8_t *D = P.dist;
-    const int PW = P.PW, pad = P.pad;
+    const int TW = P.TW, pad = P.pad;
     const int gc = (2 * P.mx + FR_GROUP) / FR_GROUP;
     const float fxc = (float)x, fzc = (float)z;
     uint32_t open = o.open;
@@ -385,7 +408,7 @@ __device__ bool dev_lane_groups(const FrPlanDev &P, int x, int z, const FrOpen &
                     dev_fp_cell(j, i, o.s, o.c, fxc, fzc, &xl, &zl);
                     const int ax = xl > pcx ? xl - pcx : pcx - xl, az = zl > pcz ? zl - pcz : pcz - zl;
                     if (ax >= pd || az >= pd)
-                        hit |= D[(unsigned)((zl + pad) * PW + (xl + pad))] == 0;
+                        hit |= D[fr_dist_at(TW, xl + pad, zl + pad)] == 0;
                 }
             }
         }
@@ -479,135 +502,250 @@ __device__ bool dev_open_flush(const FrPlanDev &P, FrOpenBuf &b, int m, int *nb)
  * Pass 0 tests curvature and the centre cells; pass 1 the footprints, whose
  * probe phase runs chunk by chunk while the groups left open wait in a
  * per-wave buffer and are sampled 64 at a time (a collision anywhere rejects,
- * so a curve that some probe rejects never has its groups sampled). */
-__device__ __forceinline__ float dev_wave_check_direct(const FrPlanDev &P, float max_curv, int x, int z, float lh, int gx, int gz,
-                                       float gh, bool safe_zone, bool host_nan, float base)
+ * so a curve that some probe rejects never has its groups sampled).  `passes`
+ * selects the passes run (bit 0, bit 1): a caller may run pass 0 alone and
+ * pass 1 later on the curves it kept; the cost is summed in pass 1. */
+/* Hermite set-up of graph_direct_goal_connection.cu:16-60 (per lane);
+ * false when the curve has no point. */
+__device__ __forceinline__ bool dev_curve_setup(int x, int z, float lh, int gx, int gz, float gh, FrHermite *hm,
+                                                int *np)
 {
-    const int W = P.W;
-    const int lane = dev_lane();
-    const int pos = z * W + x;
     const int dxi = gx - x, dzi = gz - z;
     const float distance = sqrtf((float)(dxi * dxi + dzi * dzi));
     const int num_points = fr_trunc_f(distance);
+    *np = num_points;
     if (num_points <= 0)
-        return -1;
+        return false;
     const float a1 = (float)((double)lh - FR_HALF_PI_D);
     const float a2 = (float)((double)gh - FR_HALF_PI_D);
     float s1, c1, s2, c2;
     fr_sincosf(a1, &s1, &c1);
     fr_sincosf(a2, &s2, &c2);
+    hm->t1x = (double)(distance * c1);
+    hm->t1y = (double)(distance * s1);
+    hm->t2x = (double)(distance * c2);
+    hm->t2y = (double)(distance * s2);
+    hm->dx0 = (double)x;
+    hm->dz0 = (double)z;
+    hm->dgx = (double)gx;
+    hm->dgz = (double)gz;
+    hm->denom = (double)(num_points - 1);
+    return true;
+}
+
+__device__ __forceinline__ double dev_bcast_d(double v, int lane)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ FrHermite dev_bcast_hermite(const FrHermite &h, int lane)
+{
+    FrHermite o;
+    o.dx0 = dev_bcast_d(h.dx0, lane);
+    o.dz0 = dev_bcast_d(h.dz0, lane);
+    o.dgx = dev_bcast_d(h.dgx, lane);
+    o.dgz = dev_bcast_d(h.dgz, lane);
+    o.t1x = dev_bcast_d(h.t1x, lane);
+    o.t1y = dev_bcast_d(h.t1y, lane);
+    o.t2x = dev_bcast_d(h.t2x, lane);
+    o.t2y = dev_bcast_d(h.t2y, lane);
+    o.denom = dev_bcast_d(h.denom, lane);
+    return o;
+}
+
+__device__ __forceinline__ float dev_wave_curve(const FrPlanDev &P, const FrHermite &hm, int num_points, float max_curv,
+                                                int gx, int gz, bool skip_fp, bool host_nan, float base, int passes);
+
+__device__ __forceinline__ float dev_wave_check_direct(const FrPlanDev &P, float max_curv, int x, int z, float lh, int gx, int gz,
+                                       float gh, bool safe_zone, bool host_nan, float base, int passes = 3)
+{
     FrHermite hm;
-    hm.t1x = (double)(distance * c1);
-    hm.t1y = (double)(distance * s1);
-    hm.t2x = (double)(distance * c2);
-    hm.t2y = (double)(distance * s2);
-    hm.dx0 = (double)x;
-    hm.dz0 = (double)z;
-    hm.dgx = (double)gx;
-    hm.dgz = (double)gz;
-    hm.denom = (double)(num_points - 1);
-    const bool skip_fp = safe_zone && ((P.trav[pos] & 2) != 0);
-    __shared__ FrOpenBuf s_open[FR_BLOCK / 64];
-    FrOpenBuf &ob = s_open[__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))];
-    int nb = 0;
-    const unsigned long long below = (1ull << lane) - 1ull;
-    float node_cost = base;
+    int num_points;
+    if (!dev_curve_setup(x, z, lh, gx, gz, gh, &hm, &num_points))
+        return -1;
+    const bool skip_fp = safe_zone && ((P.trav[z * P.W + x] & 2) != 0);
+    return dev_wave_curve(P, hm, num_points, max_curv, gx, gz, skip_fp, host_nan, base, passes);
+}
+
+/* Point b + lane of a curve: its cell, its parameter t, and whether it is
+ * accepted (in the grid and in another cell than the previous in-grid point,
+ * whose cell last_x / last_z carries from chunk to chunk).  *am: the
+ * accepted lanes. */
+__device__ __forceinline__ bool dev_curve_point(const FrPlanDev &P, const FrHermite &hm, int b, int num_points,
+                                                bool host_nan, int &last_x, int &last_z, int *cx, int *cz, double *t,
+                                                unsigned long long *am)
+{
+    const int lane = dev_lane();
+    const int i = b + lane;
+    *cx = 0;
+    *cz = 0;
+    *t = 0;
+    const bool cand = i < num_points && dev_hermite_point(P, hm, i, host_nan, cx, cz, t);
+    const unsigned long long m = __ballot(cand);
+    const unsigned long long lower = m & ((1ull << lane) - 1ull);
+    const int src = lower ? 63 - __clzll((long long)lower) : lane;
+    int pcx = __shfl(*cx, src, 64), pcz = __shfl(*cz, src, 64);
+    if (!lower) {
+        pcx = last_x;
+        pcz = last_z;
+    }
+    const bool acc = cand && !(*cx == pcx && *cz == pcz);
+    if (m) {
+        const int hi = 63 - __clzll((long long)m);
+        last_x = dev_bcast_i(*cx, hi);
+        last_z = dev_bcast_i(*cz, hi);
+    }
+    *am = __ballot(acc);
+    return acc;
+}
+
+__device__ __forceinline__ void dev_curve_d1(const FrHermite &hm, double t, double *ddx, double *ddz)
+{
+    const double t2 = t * t;
+    const double t00 = 6 * t2 - 6 * t;
+    const double t10 = 3 * t2 - 4 * t + 1;
+    const double t01 = -6 * t2 + 6 * t;
+    const double t11 = 3 * t2 - 2 * t;
+    *ddx = t00 * hm.dx0 + t10 * hm.t1x + t01 * hm.dgx + t11 * hm.t2x;
+    *ddz = t00 * hm.dz0 + t10 * hm.t1y + t01 * hm.dgz + t11 * hm.t2y;
+}
+
+/* curvature test of graph_direct_goal_connection.cu (float of the double
+ * curvature against max_curv) */
+__device__ __forceinline__ bool dev_curve_too_sharp(const FrHermite &hm, double t, float max_curv)
+{
+    double ddx, ddz;
+    dev_curve_d1(hm, t, &ddx, &ddz);
+    const double d00 = 12 * t - 6;
+    const double d10 = 6 * t - 4;
+    const double d01 = -12 * t + 6;
+    const double d11 = 6 * t - 2;
+    const double dd2x = d00 * hm.dx0 + d10 * hm.t1x + d01 * hm.dgx + d11 * hm.t2x;
+    const double dd2z = d00 * hm.dz0 + d10 * hm.t1y + d01 * hm.dgz + d11 * hm.t2y;
+    const double v = ddx * ddx + ddz * ddz;
+    const double cr = ddx * dd2z - ddz * dd2x;
+    const float k = (float)((cr < 0 ? -cr : cr) / (v * __builtin_sqrt(v)));
+    return k > max_curv;
+}
+
+/* Pass 0 over FR_P0_CHUNKS chunks of 64 points at a time: the geometry and
+ * curvature of every point first, then the centre cells' bytes, all loads
+ * of the chunks in flight together (the pass is bound by their latency). */
+#define FR_P0_CHUNKS 4
+
+__device__ __forceinline__ bool dev_wave_curve_pass0(const FrPlanDev &P, const FrHermite &hm, int num_points,
+                                                     float max_curv, bool skip_fp, bool host_nan, int &last_x,
+                                                     int &last_z)
+{
+    last_x = -1;
+    last_z = -1;
+    for (int b0 = 0; b0 < num_points; b0 += 64 * FR_P0_CHUNKS) {
+        int cxs[FR_P0_CHUNKS], czs[FR_P0_CHUNKS];
+        bool accs[FR_P0_CHUNKS];
+        bool bad = false;
+#pragma unroll
+        for (int u = 0; u < FR_P0_CHUNKS; ++u) {
+            const int b = b0 + 64 * u;
+            accs[u] = false;
+            cxs[u] = 0;
+            czs[u] = 0;
+            if (b < num_points) {
+                double t;
+                unsigned long long am;
+                accs[u] = dev_curve_point(P, hm, b, num_points, host_nan, last_x, last_z, &cxs[u], &czs[u], &t, &am);
+                if (accs[u] && max_curv > 0 && dev_curve_too_sharp(hm, t, max_curv))
+                    bad = true;
+            }
+        }
+        if (__ballot(bad))
+            return false;
+        if (skip_fp)
+            continue;
+        int tv[FR_P0_CHUNKS], dv[FR_P0_CHUNKS];
+#pragma unroll
+        for (int u = 0; u < FR_P0_CHUNKS; ++u) {
+            tv[u] = 1;
+            dv[u] = 1;
+            if (accs[u]) {
+                tv[u] = P.trav[czs[u] * P.W + cxs[u]];
+                dv[u] = P.dist[fr_dist_at(P.TW, cxs[u] + P.pad, czs[u] + P.pad)];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < FR_P0_CHUNKS; ++u)
+            if (!(tv[u] & 1) && dv[u] == 0)
+                bad = true;
+        if (__ballot(bad))
+            return false;
+    }
+    return true;
+}
+
+/* The passes of dev_wave_check_direct over a set-up curve. */
+__device__ __forceinline__ float dev_wave_curve(const FrPlanDev &P, const FrHermite &hm, int num_points, float max_curv,
+                                                int gx, int gz, bool skip_fp, bool host_nan, float base, int passes)
+{
+    const int W = P.W;
+    const int lane = dev_lane();
     int last_x = -1, last_z = -1;
-    for (int pass = 0; pass < 2; ++pass) {
+    if ((passes & 1) && !dev_wave_curve_pass0(P, hm, num_points, max_curv, skip_fp, host_nan, last_x, last_z))
+        return -1;
+    float node_cost = base;
+    if (passes & 2) {
+        __shared__ FrOpenBuf s_open[FR_BLOCK / 64];
+        FrOpenBuf &ob = s_open[__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))];
+        int nb = 0;
+        const unsigned long long below = (1ull << lane) - 1ull;
         last_x = -1;
         last_z = -1;
         for (int b = 0; b < num_points; b += 64) {
-            const int i = b + lane;
-            int cx = 0, cz = 0;
-            double t = 0;
-            const bool cand = i < num_points && dev_hermite_point(P, hm, i, host_nan, &cx, &cz, &t);
-            const unsigned long long m = __ballot(cand);
-            const unsigned long long lower = m & ((1ull << lane) - 1ull);
-            const int src = lower ? 63 - __clzll((long long)lower) : lane;
-            int pcx = __shfl(cx, src, 64), pcz = __shfl(cz, src, 64);
-            if (!lower) {
-                pcx = last_x;
-                pcz = last_z;
-            }
-            const bool acc = cand && !(cx == pcx && cz == pcz);
-            if (m) {
-                const int hi = 63 - __clzll((long long)m);
-                last_x = dev_bcast_i(cx, hi);
-                last_z = dev_bcast_i(cz, hi);
-            }
-            const unsigned long long am = __ballot(acc);
+            int cx, cz;
+            double t;
+            unsigned long long am;
+            const bool acc = dev_curve_point(P, hm, b, num_points, host_nan, last_x, last_z, &cx, &cz, &t, &am);
             if (!am)
                 continue;
-            double ddx = 0, ddz = 0;
+            float add = 0;
+            int pr = 1;
+            FrOpen o;
             if (acc) {
-                const double t2 = t * t;
-                const double t00 = 6 * t2 - 6 * t;
-                const double t10 = 3 * t2 - 4 * t + 1;
-                const double t01 = -6 * t2 + 6 * t;
-                const double t11 = 3 * t2 - 2 * t;
-                ddx = t00 * hm.dx0 + t10 * hm.t1x + t01 * hm.dgx + t11 * hm.t2x;
-                ddz = t00 * hm.dz0 + t10 * hm.t1y + t01 * hm.dgz + t11 * hm.t2y;
+                add = (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
+                if (!skip_fp) {
+                    double ddx, ddz;
+                    dev_curve_d1(hm, t, &ddx, &ddz);
+                    const float hd = (float)((double)fr_atan2f((float)ddz, (float)ddx) + FR_HALF_PI_D);
+                    pr = dev_lane_probe(P, cx, cz, hd, &o);
+                }
             }
-            if (pass == 0) {
-                bool bad = false;
-                if (acc) {
-                    if (max_curv > 0) {
-                        const double d00 = 12 * t - 6;
-                        const double d10 = 6 * t - 4;
-                        const double d01 = -12 * t + 6;
-                        const double d11 = 6 * t - 2;
-                        const double dd2x = d00 * hm.dx0 + d10 * hm.t1x + d01 * hm.dgx + d11 * hm.t2x;
-                        const double dd2z = d00 * hm.dz0 + d10 * hm.t1y + d01 * hm.dgz + d11 * hm.t2y;
-                        const double v = ddx * ddx + ddz * ddz;
-                        const double cr = ddx * dd2z - ddz * dd2x;
-                        const float k = (float)((cr < 0 ? -cr : cr) / (v * __builtin_sqrt(v)));
-                        if (k > max_curv)
-                            bad = true;
-                    }
-                    if (!skip_fp && !(P.trav[cz * W + cx] & 1) &&
-                        P.dist[(cz + P.pad) * P.PW + cx + P.pad] == 0)
-                        bad = true;
-                }
-                if (__ballot(bad))
-                    return -1;
-            } else {
-                float add = 0;
-                int pr = 1;
-                FrOpen o;
-                if (acc) {
-                    add = (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
-                    if (!skip_fp) {
-                        const float hd = (float)((double)fr_atan2f((float)ddz, (float)ddx) + FR_HALF_PI_D);
-                        pr = dev_lane_probe(P, cx, cz, hd, &o);
-                    }
-                }
-                if (__ballot(pr < 0))
-                    return -1;
-                unsigned long long cm = am;
-                while (cm) {
-                    const int l = __ffsll((long long)cm) - 1;
-                    cm &= cm - 1;
-                    node_cost += dev_bcast_f(add, l);
-                }
-                const unsigned long long om = __ballot(pr == 0);
-                if (pr == 0) {
-                    const int r = nb + __popcll(om & below);
-                    ob.x[r] = cx;
-                    ob.z[r] = cz;
-                    ob.s[r] = o.s;
-                    ob.c[r] = o.c;
-                    ob.open[r] = o.open;
-                    ob.code[r] = o.code;
-                }
-                nb += __popcll(om);
-                if (nb >= 64 && !dev_open_flush(P, ob, 64, &nb))
-                    return -1;
+            if (__ballot(pr < 0))
+                return -1;
+            unsigned long long cm = am;
+            while (cm) {
+                const int l = __ffsll((long long)cm) - 1;
+                cm &= cm - 1;
+                node_cost += dev_bcast_f(add, l);
             }
+            const unsigned long long om = __ballot(pr == 0);
+            if (pr == 0) {
+                const int r = nb + __popcll(om & below);
+                ob.x[r] = cx;
+                ob.z[r] = cz;
+                ob.s[r] = o.s;
+                ob.c[r] = o.c;
+                ob.open[r] = o.open;
+                ob.code[r] = o.code;
+            }
+            nb += __popcll(om);
+            if (nb >= 64 && !dev_open_flush(P, ob, 64, &nb))
+                return -1;
         }
+        while (nb > 0)
+            if (!dev_open_flush(P, ob, nb < 64 ? nb : 64, &nb))
+                return -1;
     }
-    while (nb > 0)
-        if (!dev_open_flush(P, ob, nb < 64 ? nb : 64, &nb))
-            return -1;
     if (last_x != gx && last_z != gz)
         return -1;
     return node_cost;
@@ -786,7 +924,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_setup_coldist(const FrPlanDev *pla
             }
             best = cand;
         }
-        P.dist[i] = (uint8_t)best;
+        P.dist[fr_dist_at(P.TW, xp, zp)] = (uint8_t)best;
     }
 }
 
@@ -794,12 +932,12 @@ __global__ void __launch_bounds__(FR_BLOCK) k_setup_coldist(const FrPlanDev *pla
  * (one wave).  The start cost is the cost plane as it was at setup, kept in
  * gc_cost by k_setup_gc_reset, so an evaluation made later gives the same
  * planes as the reference's pass at set-up time. */
-__device__ void dev_gc_cell(const FrPlanDev &P, int pos)
+/* Result of one cell's evaluation in every plane.  (The lazy kernels store
+ * less: k_gc_demand has set the evaluated bit of every cell it queues, and a
+ * queued cell's flag bit is still clear, each cell being evaluated once
+ * after the reset, so there only a flagged cell touches the bit planes.) */
+__device__ void dev_gc_store(const FrPlanDev &P, int pos, float lh, float c)
 {
-    const int z = pos / P.W, x = pos - z * P.W;
-    const float lh = (float)dev_compute_heading(x, z, P.gx, P.gz);
-    const float c =
-        dev_wave_check_direct(P, P.max_curv_goal, x, z, lh, P.gx, P.gz, P.gh, false, false, P.gc_cost[pos]);
     if (dev_lane() == 0) {
         P.gc_flag[pos] = c < 0 ? 0 : 1;
         if (c >= 0) {
@@ -811,6 +949,15 @@ __device__ void dev_gc_cell(const FrPlanDev &P, int pos)
         }
         atomicOr(&P.gc_done[pos >> 5], 1u << (pos & 31));
     }
+}
+
+__device__ __forceinline__ void dev_gc_cell(const FrPlanDev &P, int pos)
+{
+    const int z = pos / P.W, x = pos - z * P.W;
+    const float lh = (float)dev_compute_heading(x, z, P.gx, P.gz);
+    const float c = dev_wave_check_direct(P, P.max_curv_goal, x, z, lh, P.gx, P.gz, P.gh, false, false,
+                                          P.gc_cost[pos]);
+    dev_gc_store(P, pos, lh, c);
 }
 
 /* snapshot of the cost plane, flags cleared, nothing evaluated */
@@ -853,6 +1000,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_begin(const FrPlanDev *plans, FrCt
         counters[FR_CNT_FRESH] = 0;
         counters[FR_CNT_PAIRS] = 0;
         counters[FR_CNT_DEMAND] = 0;
+        counters[FR_CNT_DEMAND2] = 0;
     }
     if (i >= n_slots)
         return;
@@ -1592,18 +1740,123 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_demand(const FrPlanDev *plans, 
     }
 }
 
+/* The lazily evaluated cells of one iteration are split in blocks of up to
+ * 64 consecutive queue entries, one block per wave (more cells per wave as
+ * the queue grows): the lanes set up the block's goal curves together, one
+ * cell per lane, then the wave walks the block's curves one by one. */
+__device__ __forceinline__ int dev_gc_block(long n, long nwaves)
+{
+    const long c = (n + nwaves - 1) / nwaves;
+    return c < 1 ? 1 : (c > 64 ? 64 : (int)c);
+}
+
+/* Per lane: set-up of the goal curve of queue entry d (base: the cost the
+ * curve starts from, read when `with_base`). */
+__device__ __forceinline__ bool dev_gc_lane_setup(const FrPlanDev *plans, int2 d, bool with_base, FrHermite *hm, int *np,
+                                                  float *lh, float *base)
+{
+    const FrPlanDev &P = plans[d.x];
+    const int z = d.y / P.W, x = d.y - z * P.W;
+    *lh = (float)dev_compute_heading(x, z, P.gx, P.gz);
+    *base = with_base ? P.gc_cost[d.y] : 0.0f;
+    return dev_curve_setup(x, z, *lh, P.gx, P.gz, P.gh, hm, np);
+}
+
+/* Curvature and centre cells of the queued cells' goal curves; a rejection
+ * is stored, the other cells are queued for k_gc_foot (at most as many as
+ * k_gc_eval reads, so dq2 has dq's capacity). */
 __global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *__restrict__ plans, const int2 *__restrict__ dq,
-                                                      long dq_cap, const int *__restrict__ counters)
+                                                      long dq_cap, int2 *__restrict__ dq2, int *__restrict__ counters)
 {
     long n = counters[FR_CNT_DEMAND];
     if (n > dq_cap)
         n = dq_cap;
     const int wpb = blockDim.x >> 6;
-    const long nw = (long)gridDim.x * wpb;
-    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    for (long i = blockIdx.x * (long)wpb + wv; i < n; i += nw) {
-        const int2 d = dq[i];
-        dev_gc_cell(plans[__builtin_amdgcn_readfirstlane(d.x)], __builtin_amdgcn_readfirstlane(d.y));
+    const long nwaves = (long)gridDim.x * wpb;
+    const int cpw = dev_gc_block(n, nwaves);
+    const int lane = dev_lane();
+    const long gw = (long)blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (long b0 = gw * cpw; b0 < n; b0 += nwaves * cpw) {
+        const int cnt = (int)(n - b0 < cpw ? n - b0 : cpw);
+        int2 d = make_int2(0, 0);
+        FrHermite hm = {};
+        int np = 0;
+        float lh = 0, base = 0;
+        bool ok = false;
+        if (lane < cnt) {
+            d = dq[b0 + lane];
+            ok = dev_gc_lane_setup(plans, d, false, &hm, &np, &lh, &base);
+        }
+        const unsigned long long okm = __ballot(ok);
+        unsigned long long kept = 0; /* cells of the block queued for k_gc_foot */
+        for (int k = 0; k < cnt; ++k) {
+            const int slot = dev_bcast_i(d.x, k), pos = dev_bcast_i(d.y, k);
+            const FrPlanDev &P = plans[slot];
+            bool keep = false;
+            if ((okm >> k) & 1ull) {
+                const FrHermite hk = dev_bcast_hermite(hm, k);
+                keep = dev_wave_curve(P, hk, dev_bcast_i(np, k), P.max_curv_goal, P.gx, P.gz, false, false, 0.0f, 1) >=
+                       0;
+            }
+            if (keep)
+                kept |= 1ull << k;
+        }
+        /* the block's rejections stored together, one lane per cell (their
+         * bits stay clear, see dev_gc_store) */
+        if (lane < cnt && !((kept >> lane) & 1ull))
+            plans[d.x].gc_flag[d.y] = 0;
+        /* one queue reservation per block */
+        if (kept) {
+            int at = 0;
+            if (lane == 0)
+                at = atomicAdd(&counters[FR_CNT_DEMAND2], __popcll(kept));
+            at = dev_bcast_i(at, 0);
+            if ((kept >> lane) & 1ull)
+                dq2[at + __popcll(kept & ((1ull << lane) - 1ull))] = d;
+        }
+    }
+}
+
+/* Footprints and cost of the cells k_gc_eval kept. */
+__global__ void __launch_bounds__(FR_BLOCK) k_gc_foot(const FrPlanDev *__restrict__ plans, const int2 *__restrict__ dq2,
+                                                      const int *__restrict__ counters)
+{
+    const long n = counters[FR_CNT_DEMAND2];
+    const int wpb = blockDim.x >> 6;
+    const long nwaves = (long)gridDim.x * wpb;
+    const int cpw = dev_gc_block(n, nwaves);
+    const int lane = dev_lane();
+    const long gw = (long)blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (long b0 = gw * cpw; b0 < n; b0 += nwaves * cpw) {
+        const int cnt = (int)(n - b0 < cpw ? n - b0 : cpw);
+        int2 d = make_int2(0, 0);
+        FrHermite hm = {};
+        int np = 0;
+        float lh = 0, base = 0;
+        if (lane < cnt) {
+            d = dq2[b0 + lane];
+            (void)dev_gc_lane_setup(plans, d, true, &hm, &np, &lh, &base);
+        }
+        float res = -1;
+        for (int k = 0; k < cnt; ++k) {
+            const int slot = dev_bcast_i(d.x, k);
+            const FrPlanDev &P = plans[slot];
+            const FrHermite hk = dev_bcast_hermite(hm, k);
+            const float c = dev_wave_curve(P, hk, dev_bcast_i(np, k), P.max_curv_goal, P.gx, P.gz, false, false,
+                                           dev_bcast_f(base, k), 2);
+            res = lane == k ? c : res;
+        }
+        /* the block's results stored together, one lane per cell */
+        if (lane < cnt) {
+            const FrPlanDev &P = plans[d.x];
+            const int pos = d.y;
+            P.gc_flag[pos] = res < 0 ? 0 : 1;
+            if (res >= 0) {
+                P.gc_heading[pos] = lh;
+                P.gc_cost[pos] = res;
+                atomicOr(&P.gc_bits[pos >> 5], 1u << (pos & 31));
+            }
+        }
     }
 }
 
@@ -2303,7 +2556,9 @@ void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st)
     fr_prof_begin(FR_PROF_GOAL_CONNECTION, st);
     hipLaunchKernelGGL(k_gc_demand, dim3(L.grid_waves), b, 0, st, L.plans, L.fresh_q, L.demand, L.demand_cap,
                        L.counters);
-    hipLaunchKernelGGL(k_gc_eval, dim3(L.grid_waves), b, 0, st, L.plans, L.demand, L.demand_cap, L.counters);
+    hipLaunchKernelGGL(k_gc_eval, dim3(L.grid_waves), b, 0, st, L.plans, L.demand, L.demand_cap, L.demand2,
+                       L.counters);
+    hipLaunchKernelGGL(k_gc_foot, dim3(L.grid_waves), b, 0, st, L.plans, L.demand2, L.counters);
     fr_prof_end(FR_PROF_GOAL_CONNECTION, st);
     hipLaunchKernelGGL(k_connect_emit, dim3(L.grid_waves), b, 0, st, L.plans, L.ctl, L.fresh_q, L.pairs,
                        L.pair_cap, L.counters);
