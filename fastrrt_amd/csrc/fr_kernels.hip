@@ -257,6 +257,18 @@ __device__ __forceinline__ float dev_bcast_f(float v, int lane)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+/* inclusive prefix sum over the wave */
+__device__ __forceinline__ int dev_wave_incl_scan(int v)
+{
+    const int lane = dev_lane();
+    for (int off = 1; off < 64; off <<= 1) {
+        const int n = __shfl_up(v, off, 64);
+        if (lane >= off)
+            v += n;
+    }
+    return v;
+}
+
 /* path_feasible.cu:18-78 for one lane's own pose (lanes test different
  * poses at once), in two phases.  After the all-angle bit and the centre
  * certificate, the (2 mz + 1) x (2 mx + 1) samples are taken in 7 x 7 groups:
@@ -1032,6 +1044,69 @@ __global__ void __launch_bounds__(FR_BLOCK) k_begin(const FrPlanDev *plans, FrCt
     }
 }
 
+/* Tile scans: one wave per tile of FR_TILE = 1024 cells, lane l holding
+ * the type bytes of cells base + 16 l .. base + 16 l + 15 (one 16-byte
+ * load: the type plane is 256-byte aligned and padded to a multiple of 256
+ * bytes; cells at or past N read as NULL).  A tile without the cells a scan
+ * looks for costs one load and one ballot. */
+__device__ __forceinline__ uint4 dev_tile_types(const FrPlanDev &P, int base)
+{
+    const int p0 = base + 16 * dev_lane();
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (p0 < P.N) {
+        v = *reinterpret_cast<const uint4 *>(P.type + p0);
+        const int left = P.N - p0;
+        if (left < 16) {
+            uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                if (i >= left)
+                    w[i >> 2] &= ~(0xFFu << (8 * (i & 3)));
+            v = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    }
+    return v;
+}
+
+__device__ __forceinline__ int dev_byte(uint4 v, int i)
+{
+    const uint32_t w = i < 4 ? v.x : (i < 8 ? v.y : (i < 12 ? v.z : v.w));
+    return (int)((w >> (8 * (i & 3))) & 0xFFu);
+}
+
+/* 16-bit mask of the bytes of v equal to t */
+__device__ __forceinline__ uint32_t dev_bytes_eq(uint4 v, int t)
+{
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        m |= (uint32_t)(dev_byte(v, i) == t) << i;
+    return m;
+}
+
+/* 16-bit mask of the non-zero bytes of v */
+__device__ __forceinline__ uint32_t dev_bytes_nz(uint4 v)
+{
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        m |= (uint32_t)(dev_byte(v, i) != 0) << i;
+    return m;
+}
+
+/* Wave-wide reservation of `cnt` consecutive queue entries per lane in lane
+ * order; returns this lane's first index (one atomic per wave). */
+__device__ __forceinline__ int dev_wave_reserve(int *counter, int cnt, int *total_out)
+{
+    const int incl = dev_wave_incl_scan(cnt);
+    const int total = dev_bcast_i(incl, 63);
+    *total_out = total;
+    int at = 0;
+    if (total && dev_lane() == 0)
+        at = atomicAdd(counter, total);
+    return dev_bcast_i(at, 0) + incl - cnt;
+}
+
 /* Gate of __CUDA_random_node_expansion (graph_expansion.cu:157-164) and of
  * __CUDA_smart_node_expansion (graph_expansion_search_zones.cu:461-473):
  * queue one rollout per eligible NODE cell. */
@@ -1039,23 +1114,27 @@ __global__ void __launch_bounds__(FR_BLOCK) k_select(const FrPlanDev *plans, FrC
                          FrExp *q, int *counters)
 {
     const int n_tiles = *n_tiles_p;
-    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    const int lane = dev_lane();
+    for (int t = blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); t < n_tiles; t += nw) {
         const int2 tl = tiles[t];
-        const FrPlanDev &P = plans[tl.x];
-        FrCtl &C = ctl[tl.x];
+        const int slot = __builtin_amdgcn_readfirstlane(tl.x);
+        const FrPlanDev &P = plans[slot];
+        FrCtl &C = ctl[slot];
         if (C.status != FR_SEARCHING)
             continue;
-        const int base = tl.y * FR_TILE;
-        int mine[4];
-        int cnt = 0, nodes = 0;
+        const int base = __builtin_amdgcn_readfirstlane(tl.y) * FR_TILE;
+        const uint4 v = dev_tile_types(P, base);
+        const uint32_t nodes_m = dev_bytes_eq(v, FR_TYPE_NODE);
+        if (!__ballot(nodes_m != 0))
+            continue;
         const int nr = P.dens_w * P.dens_h;
         const uint32_t *rd = P.region + (C.iter & 1) * nr;
-        for (int k = 0; k < 4; ++k) {
-            const int pos = base + k * FR_BLOCK + threadIdx.x;
-            mine[k] = -1;
-            if (pos >= P.N || P.type[pos] != FR_TYPE_NODE)
-                continue;
-            nodes++;
+        uint32_t ok_m = 0;
+        for (uint32_t m = nodes_m; m; m &= m - 1) {
+            const int i = __ffs((int)m) - 1;
+            const int pos = base + 16 * lane + i;
             bool ok;
             if (P.smart) {
                 if (C.force)
@@ -1070,37 +1149,28 @@ __global__ void __launch_bounds__(FR_BLOCK) k_select(const FrPlanDev *plans, FrC
             } else {
                 ok = !(C.frontier && P.derive[pos] > 0);
             }
-            if (ok) {
-                mine[k] = pos;
-                cnt++;
-            }
+            if (ok)
+                ok_m |= 1u << i;
         }
-        int prefix;
-        const int total = block_exclusive_scan(cnt, &prefix);
-        int nodes_prefix;
-        const int node_total = block_exclusive_scan(nodes, &nodes_prefix);
-        __shared__ int s_base;
-        if (threadIdx.x == 0) {
-            s_base = total ? atomicAdd(&counters[FR_CNT_EXP], total) : 0;
+        int total;
+        int w = dev_wave_reserve(&counters[FR_CNT_EXP], __popc(ok_m), &total);
+        const int node_total = dev_bcast_i(dev_wave_incl_scan(__popc(nodes_m)), 63);
+        for (uint32_t m = ok_m; m; m &= m - 1) {
+            FrExp e;
+            e.slot = slot;
+            e.src = base + 16 * lane + __ffs((int)m) - 1;
+            e.end = -1;
+            e.heading = 0;
+            e.cost = 0;
+            e.n_chk = 0;
+            q[w++] = e;
+        }
+        if (lane == 0) {
             if (node_total)
                 atomicAdd(&C.n_node_pre, node_total);
             if (total)
                 atomicAdd(&C.n_expansions, total);
         }
-        __syncthreads();
-        int w = s_base + prefix;
-        for (int k = 0; k < 4; ++k)
-            if (mine[k] >= 0) {
-                FrExp e;
-                e.slot = tl.x;
-                e.src = mine[k];
-                e.end = -1;
-                e.heading = 0;
-                e.cost = 0;
-                e.n_chk = 0;
-                q[w++] = e;
-            }
-        __syncthreads();
     }
 }
 
@@ -1350,23 +1420,27 @@ __global__ void __launch_bounds__(FR_BLOCK) k_commit(const FrPlanDev *plans, FrC
 __global__ void __launch_bounds__(FR_BLOCK) k_density(const FrPlanDev *plans, const FrCtl *ctl, const int2 *tiles, const int *n_tiles_p)
 {
     const int n_tiles = *n_tiles_p;
-    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    const int lane = dev_lane();
+    for (int t = blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); t < n_tiles; t += nw) {
         const int2 tl = tiles[t];
-        const FrPlanDev &P = plans[tl.x];
-        const FrCtl &C = ctl[tl.x];
+        const int slot = __builtin_amdgcn_readfirstlane(tl.x);
+        const FrPlanDev &P = plans[slot];
+        const FrCtl &C = ctl[slot];
         if (C.status != FR_SEARCHING || !P.smart)
+            continue;
+        const int base = __builtin_amdgcn_readfirstlane(tl.y) * FR_TILE;
+        const uint4 v = dev_tile_types(P, base);
+        /* counted: every type but NULL and PROCESSING */
+        const uint32_t cm = dev_bytes_nz(v) & ~dev_bytes_eq(v, FR_TYPE_PROCESSING);
+        if (!__ballot(cm != 0))
             continue;
         const int nr = P.dens_w * P.dens_h;
         uint32_t *wr = P.region + ((C.iter + 1) & 1) * nr;
-        const int base = tl.y * FR_TILE;
         int cur = -1, acc = 0;
-        for (int k = 0; k < 4; ++k) {
-            const int pos = base + threadIdx.x * 4 + k;
-            if (pos >= P.N)
-                break;
-            const int ty = P.type[pos];
-            if (ty == FR_TYPE_NULL || ty == FR_TYPE_PROCESSING)
-                continue;
+        for (uint32_t m = cm; m; m &= m - 1) {
+            const int pos = base + 16 * lane + __ffs((int)m) - 1;
             const int z = pos / P.W, x = pos - z * P.W;
             const int r = (z / P.region_side) * P.dens_w + x / P.region_side;
             if (r != cur) {
@@ -1434,49 +1508,47 @@ __global__ void __launch_bounds__(FR_BLOCK) k_erase_apply(const FrPlanDev *plans
                               FrFresh *fq, int *counters)
 {
     const int n_tiles = *n_tiles_p;
-    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    const int lane = dev_lane();
+    for (int t = blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); t < n_tiles; t += nw) {
         const int2 tl = tiles[t];
-        const FrPlanDev &P = plans[tl.x];
-        FrCtl &C = ctl[tl.x];
+        const int slot = __builtin_amdgcn_readfirstlane(tl.x);
+        const FrPlanDev &P = plans[slot];
+        const FrCtl &C = ctl[slot];
         if (C.status != FR_SEARCHING || !dev_solve_runs(C))
             continue;
-        const int base = tl.y * FR_TILE;
-        int mine[4];
-        int cnt = 0;
-        for (int k = 0; k < 4; ++k) {
-            const int pos = base + k * FR_BLOCK + threadIdx.x;
-            mine[k] = -1;
-            if (pos >= P.N)
-                continue;
+        const int base = __builtin_amdgcn_readfirstlane(tl.y) * FR_TILE;
+        const uint4 v = dev_tile_types(P, base);
+        /* k_erase_mark marks non-NULL cells only: a tile of NULL cells is unchanged */
+        const uint32_t nz = dev_bytes_nz(v);
+        if (!__ballot(nz != 0))
+            continue;
+        uint32_t conv = 0;
+        for (uint32_t m = nz; m; m &= m - 1) {
+            const int i = __ffs((int)m) - 1;
+            const int pos = base + 16 * lane + i;
             if (P.mark[pos]) {
                 P.type[pos] = FR_TYPE_NULL;
-            } else if (P.type[pos] == FR_TYPE_COLLISION) {
+            } else if (dev_byte(v, i) == FR_TYPE_COLLISION) {
                 P.type[pos] = FR_TYPE_NODE;
                 P.derive[pos] = 0;
-                mine[k] = pos;
-                cnt++;
+                conv |= 1u << i;
             }
         }
-        int prefix;
-        const int total = block_exclusive_scan(cnt, &prefix);
-        __shared__ int s_base;
-        if (threadIdx.x == 0)
-            s_base = total ? atomicAdd(&counters[FR_CNT_FRESH], total) : 0;
-        __syncthreads();
-        int w = s_base + prefix;
-        for (int k = 0; k < 4; ++k)
-            if (mine[k] >= 0) {
-                FrFresh f;
-                f.slot = tl.x;
-                f.pos = mine[k];
-                f.flags = FR_FRESH_CONVERTED;
-                f.pair_base = 0;
-                f.pair_cnt = 0;
-                f.pad0 = 0;
-                f.key = ~0ull;
-                fq[w++] = f;
-            }
-        __syncthreads();
+        int total;
+        int w = dev_wave_reserve(&counters[FR_CNT_FRESH], __popc(conv), &total);
+        for (uint32_t m = conv; m; m &= m - 1) {
+            FrFresh f;
+            f.slot = slot;
+            f.pos = base + 16 * lane + __ffs((int)m) - 1;
+            f.flags = FR_FRESH_CONVERTED;
+            f.pair_base = 0;
+            f.pair_cnt = 0;
+            f.pad0 = 0;
+            f.key = ~0ull;
+            fq[w++] = f;
+        }
     }
 }
 
@@ -1503,61 +1575,52 @@ __global__ void __launch_bounds__(FR_BLOCK) k_accept(const FrPlanDev *plans, FrC
                          FrFresh *fq, int *counters)
 {
     const int n_tiles = *n_tiles_p;
-    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    const int lane = dev_lane();
+    for (int t = blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); t < n_tiles; t += nw) {
         const int2 tl = tiles[t];
-        const FrPlanDev &P = plans[tl.x];
-        FrCtl &C = ctl[tl.x];
+        const int slot = __builtin_amdgcn_readfirstlane(tl.x);
+        const FrPlanDev &P = plans[slot];
+        FrCtl &C = ctl[slot];
         if (C.status != FR_SEARCHING)
             continue;
-        const int base = tl.y * FR_TILE;
+        const int base = __builtin_amdgcn_readfirstlane(tl.y) * FR_TILE;
+        const uint4 v = dev_tile_types(P, base);
+        const uint32_t nz = dev_bytes_nz(v);
+        if (!__ballot(nz != 0))
+            continue;
         const int start_pos = (P.sx >= 0 && P.sx < P.W && P.sz >= 0 && P.sz < P.H) ? P.sz * P.W + P.sx : -1;
-        int mine[4];
-        int cnt = 0, temps = 0, alls = 0;
-        for (int k = 0; k < 4; ++k) {
-            const int pos = base + k * FR_BLOCK + threadIdx.x;
-            mine[k] = -1;
-            if (pos >= P.N)
-                continue;
-            const int ty = P.type[pos];
-            if (ty != FR_TYPE_NULL)
-                alls++;
-            if (ty == FR_TYPE_TEMP) {
-                temps++;
-                P.type[pos] = FR_TYPE_NODE;
-                mine[k] = pos;
-                cnt++;
-            } else if (ty == FR_TYPE_NODE && (C.full_check || (C.pending_start && pos == start_pos))) {
-                mine[k] = pos;
-                cnt++;
-            }
+        const uint32_t temps = dev_bytes_eq(v, FR_TYPE_TEMP);
+        uint32_t fresh = temps;
+        const uint32_t nodes = dev_bytes_eq(v, FR_TYPE_NODE);
+        if (C.full_check)
+            fresh |= nodes;
+        else if (C.pending_start && start_pos >= base + 16 * lane && start_pos < base + 16 * lane + 16)
+            fresh |= nodes & (1u << (start_pos - base - 16 * lane));
+        for (uint32_t m = temps; m; m &= m - 1)
+            P.type[base + 16 * lane + __ffs((int)m) - 1] = FR_TYPE_NODE;
+        int total;
+        int w = dev_wave_reserve(&counters[FR_CNT_FRESH], __popc(fresh), &total);
+        const int ttot = dev_bcast_i(dev_wave_incl_scan(__popc(temps)), 63);
+        const int atot = dev_bcast_i(dev_wave_incl_scan(__popc(nz)), 63);
+        for (uint32_t m = fresh; m; m &= m - 1) {
+            FrFresh f;
+            f.slot = slot;
+            f.pos = base + 16 * lane + __ffs((int)m) - 1;
+            f.flags = 0;
+            f.pair_base = 0;
+            f.pair_cnt = 0;
+            f.pad0 = 0;
+            f.key = ~0ull;
+            fq[w++] = f;
         }
-        int prefix, tp, ap;
-        const int total = block_exclusive_scan(cnt, &prefix);
-        const int ttot = block_exclusive_scan(temps, &tp);
-        const int atot = block_exclusive_scan(alls, &ap);
-        __shared__ int s_base;
-        if (threadIdx.x == 0) {
-            s_base = total ? atomicAdd(&counters[FR_CNT_FRESH], total) : 0;
+        if (lane == 0) {
             if (ttot)
                 atomicAdd(&C.n_temp, ttot);
             if (atot)
                 atomicAdd(&C.n_all, atot);
         }
-        __syncthreads();
-        int w = s_base + prefix;
-        for (int k = 0; k < 4; ++k)
-            if (mine[k] >= 0) {
-                FrFresh f;
-                f.slot = tl.x;
-                f.pos = mine[k];
-                f.flags = 0;
-                f.pair_base = 0;
-                f.pair_cnt = 0;
-                f.pad0 = 0;
-                f.key = ~0ull;
-                fq[w++] = f;
-            }
-        __syncthreads();
     }
 }
 
@@ -1662,17 +1725,6 @@ __device__ __forceinline__ uint32_t dev_win_pair(const FrPlanDev &P, const FrWin
     return m;
 }
 
-/* inclusive prefix sum over the wave */
-__device__ __forceinline__ int dev_wave_incl_scan(int v)
-{
-    const int lane = dev_lane();
-    for (int off = 1; off < 64; off <<= 1) {
-        const int n = __shfl_up(v, off, 64);
-        if (lane >= off)
-            v += n;
-    }
-    return v;
-}
 
 /* node -> flagged cell check; returns the lcost or FR_BIG_COST.  `wave`:
  * every lane of the wave calls with the same node and cell. */
