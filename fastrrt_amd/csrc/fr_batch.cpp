@@ -65,6 +65,8 @@ struct Slot {
 
 } // namespace
 
+static_assert(sizeof(FrCtl) <= FR_PACK_PATH_OFF, "control block must fit before the path of a pinned record");
+
 class FrBatch {
   public:
     /* The engine has one slot more than the batch: a plan that never runs,
@@ -76,11 +78,13 @@ class FrBatch {
         if (max_plans <= 0 || max_cells <= 0)
             throw std::runtime_error("invalid batch capacity");
         FR_CHECK(hipHostMalloc((void **)&h_ctl_, sizeof(FrCtl) * max_plans, hipHostMallocDefault));
-        FR_CHECK(hipHostMalloc((void **)&h_fin_ctl_, sizeof(FrCtl) * fin_max_, hipHostMallocDefault));
         FR_CHECK(hipHostMalloc((void **)&h_fin_list_, sizeof(int) * fin_max_, hipHostMallocDefault));
         FR_CHECK(hipHostMalloc((void **)&h_idle_, sizeof(FrCtl), hipHostMallocDefault));
         std::memset(h_idle_, 0, sizeof(FrCtl));
         h_idle_->status = FR_IDLE;
+        /* finished plans' records, written by the device (k_fin_pack) */
+        FR_CHECK(hipHostMalloc((void **)&h_pack_, (size_t)fin_max_ * kPackStride, hipHostMallocMapped));
+        FR_CHECK(hipHostGetDevicePointer((void **)&d_pack_, h_pack_, 0));
         FR_CHECK(hipMalloc(&d_list_, sizeof(int) * max_plans));
         FR_CHECK(hipMalloc(&fin_list_, sizeof(int) * fin_max_));
         FR_CHECK(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
@@ -102,9 +106,9 @@ class FrBatch {
         (void)hipEventDestroy(fin_ev_);
         (void)hipStreamDestroy(st2_);
         (void)hipHostFree(h_ctl_);
-        (void)hipHostFree(h_fin_ctl_);
         (void)hipHostFree(h_fin_list_);
         (void)hipHostFree(h_idle_);
+        (void)hipHostFree(h_pack_);
         (void)hipFree(d_list_);
         (void)hipFree(fin_list_);
     }
@@ -296,7 +300,7 @@ class FrBatch {
         for (int i = 0; i < (int)slots_.size(); ++i)
             if (slots_[i].busy)
                 act.push_back(i);
-        eng_.set_active(act); /* synchronises the stream */
+        eng_.set_active(act);
     }
 
     /* One loop iteration of every running plan: one replay of the captured
@@ -450,32 +454,32 @@ class FrBatch {
         fr_launch_goal_scan_multi(L, fin_list_, n, grid_cells, FR_GATE_FOUND, st2_);
         fr_launch_best_node(L, fin_list_, n, grid_cells, nullptr, st2_, FR_GATE_FOUND);
         fr_launch_walk(L, fin_list_, n, st2_, FR_GATE_FOUND);
+        fr_launch_fin_pack(L, fin_list_, n, d_pack_, kPackStride, kPackPath, st2_);
         FR_CHECK(hipStreamEndCapture(st2_, &fin_graph_));
         fr_prof_enable(prof);
         FR_CHECK(hipGraphInstantiate(&fin_exec_, fin_graph_, nullptr, nullptr, 0));
         fin_rounds_ = rounds;
     }
 
+    /* The finalisation graph has completed (its event was queried): every
+     * record is in pinned memory and every control block is IDLE again. */
     void collect_finalized()
     {
-        const FrLaunch &L = eng_.launch();
         const int nf = (int)fin_slots_.size();
-        for (int q = 0; q < nf; ++q)
-            FR_CHECK(hipMemcpyAsync(&h_fin_ctl_[q], L.ctl + fin_slots_[q], sizeof(FrCtl), hipMemcpyDeviceToHost,
-                                    st2_));
-        FR_CHECK(hipStreamSynchronize(st2_));
-        std::vector<std::vector<float4>> paths(nf);
         for (int q = 0; q < nf; ++q) {
-            const int len = h_fin_ctl_[q].goal_reached ? std::max(0, h_fin_ctl_[q].path_len) : 0;
-            paths[q].resize(len);
-            if (len)
-                FR_CHECK(hipMemcpyAsync(paths[q].data(), slots_[fin_slots_[q]].slab->path, sizeof(float4) * len,
+            const uint8_t *rec = h_pack_ + (size_t)q * kPackStride;
+            FrCtl c;
+            std::memcpy(&c, rec, sizeof(FrCtl));
+            const int len = c.goal_reached ? std::max(0, c.path_len) : 0;
+            std::vector<float4> p((size_t)len);
+            if (len <= kPackPath) {
+                if (len)
+                    std::memcpy(p.data(), rec + FR_PACK_PATH_OFF, sizeof(float4) * len);
+            } else { /* longer than a record holds: read from the plan's slab */
+                FR_CHECK(hipMemcpyAsync(p.data(), slots_[fin_slots_[q]].slab->path, sizeof(float4) * len,
                                         hipMemcpyDeviceToHost, st2_));
-        }
-        FR_CHECK(hipStreamSynchronize(st2_));
-        for (int q = 0; q < nf; ++q) {
-            const std::vector<float4> &p = paths[q];
-            const int len = (int)p.size();
+                FR_CHECK(hipStreamSynchronize(st2_));
+            }
             std::vector<float> trip(3 * (size_t)len);
             for (int k = 0; k < len; ++k) {
                 trip[3 * k] = (float)(int)p[k].x;
@@ -485,7 +489,7 @@ class FrBatch {
             slots_[fin_slots_[q]].finalizing = false;
             /* the goal test (graph_best_node.cu:182-222) does not test the
              * footprint, findBestNode does: a reached goal can leave no path */
-            retire(fin_slots_[q], len > 0 ? FR_FOUND : FR_RESULT_NO_PATH, trip.data(), len, h_fin_ctl_[q]);
+            retire(fin_slots_[q], len > 0 ? FR_FOUND : FR_RESULT_NO_PATH, trip.data(), len, c, false);
         }
         finalizing_ -= nf;
         fin_slots_.clear();
@@ -494,7 +498,7 @@ class FrBatch {
 
     /* The slot's control block goes back to IDLE on the iteration stream
      * (from a constant pinned block), ahead of any later admission. */
-    void retire(int s, int status, const float *path, int len, const FrCtl &c)
+    void retire(int s, int status, const float *path, int len, const FrCtl &c, bool reset_ctl = true)
     {
         Slot &sl = slots_[s];
         Finished f;
@@ -508,8 +512,9 @@ class FrBatch {
         if (len > 0)
             f.path.assign(path, path + 3 * (size_t)len);
         done_.push_back(std::move(f));
-        FR_CHECK(hipMemcpyAsync(eng_.launch().ctl + s, h_idle_, sizeof(FrCtl), hipMemcpyHostToDevice,
-                                eng_.stream()));
+        if (reset_ctl)
+            FR_CHECK(hipMemcpyAsync(eng_.launch().ctl + s, h_idle_, sizeof(FrCtl), hipMemcpyHostToDevice,
+                                    eng_.stream()));
         give_back(std::move(sl.slab));
         cells_in_use_ -= sl.cells;
         sl.busy = false;
@@ -539,8 +544,10 @@ class FrBatch {
     std::vector<int> fin_slots_;
     int *fin_list_ = nullptr;
     int *h_fin_list_ = nullptr;
-    FrCtl *h_fin_ctl_ = nullptr;
     FrCtl *h_idle_ = nullptr;
+    static constexpr int kPackPath = 2048;                         /* waypoints per pinned record */
+    static constexpr long kPackStride = FR_PACK_PATH_OFF + 16L * kPackPath;
+    uint8_t *h_pack_ = nullptr, *d_pack_ = nullptr;
     int running_ = 0;
     int next_ticket_ = 0;
     double t0_ = 0;

@@ -20,7 +20,7 @@
 
 #define FR_TILE 1024          /* cells per tile of the flattened tile table */
 #define FR_BLOCK 256          /* threads per block of the tile kernels */
-#define FR_DIST_CAP 63        /* cap of the Chebyshev obstacle-distance plane */
+#define FR_DIST_CAP 63        /* largest cap of the Chebyshev obstacle-distance plane (min(pad, 63) is kept) */
 #define FR_CHK_STRIDE 64      /* deferred footprint tests kept per rollout */
 #define FR_BIG_COST 99999999999LL
 

@@ -260,12 +260,19 @@ FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
     const long max_tiles = queue_cap / FR_TILE + max_slots + 1;
     FR_CHECK(hipMalloc(&L_.plans, sizeof(FrPlanDev) * max_slots));
     FR_CHECK(hipMalloc(&L_.ctl, sizeof(FrCtl) * max_slots));
-    FR_CHECK(hipMalloc(&L_.slots, sizeof(int) * max_slots));
+    act_max_tiles_ = max_tiles;
+    act_tiles_off_ = sizeof(int) * (4 + (size_t)(max_slots + 1) / 2 * 2);
+    act_bytes_ = act_tiles_off_ + sizeof(int2) * (size_t)max_tiles;
+    FR_CHECK(hipMalloc(&d_act_, act_bytes_));
+    FR_CHECK(hipHostMalloc((void **)&h_act_stage_, act_bytes_ * FR_ACT_RING, hipHostMallocDefault));
+    for (auto &e : act_ev_)
+        FR_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    L_.n_tiles = (int *)d_act_;
+    L_.n_slots = (int *)d_act_ + 1;
+    L_.slots = (int *)d_act_ + 4;
+    L_.tiles = (int2 *)((uint8_t *)d_act_ + act_tiles_off_);
     FR_CHECK(hipMalloc(&d_setup_slots_, sizeof(int) * max_slots));
-    FR_CHECK(hipMalloc(&L_.tiles, sizeof(int2) * max_tiles));
-    FR_CHECK(hipMalloc(&L_.n_tiles, sizeof(int)));
-    FR_CHECK(hipMalloc(&L_.n_slots, sizeof(int)));
-    FR_CHECK(hipMemsetAsync(L_.n_slots, 0, sizeof(int), st_));
+    FR_CHECK(hipMemsetAsync(d_act_, 0, act_bytes_, st_));
     FR_CHECK(hipMalloc(&L_.counters, sizeof(int) * FR_CNT_N));
     FR_CHECK(hipMalloc(&L_.exp_q, sizeof(FrExp) * (queue_cap + 1)));
     FR_CHECK(hipMalloc(&L_.fresh_q, sizeof(FrFresh) * (2 * queue_cap + max_slots + 1)));
@@ -278,7 +285,6 @@ FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
     FR_CHECK(hipMalloc(&L_.demand, sizeof(int2) * L_.demand_cap));
     FR_CHECK(hipMalloc(&L_.demand2, sizeof(int2) * L_.demand_cap));
     FR_CHECK(hipMemsetAsync(L_.ctl, 0, sizeof(FrCtl) * max_slots, st_));
-    FR_CHECK(hipMemsetAsync(L_.n_tiles, 0, sizeof(int), st_));
     FR_CHECK(hipMemsetAsync(L_.counters, 0, sizeof(int) * FR_CNT_N, st_));
     L_.grid_tiles = (int)std::min<long>(max_tiles, 2048);
     L_.grid_items = (int)std::max<long>(1, std::min<long>((queue_cap + FR_BLOCK - 1) / FR_BLOCK, 2048));
@@ -292,11 +298,11 @@ FrEngine::~FrEngine()
     (void)hipStreamSynchronize(st_);
     (void)hipFree(L_.plans);
     (void)hipFree(L_.ctl);
-    (void)hipFree(L_.slots);
+    (void)hipFree(d_act_);
+    (void)hipHostFree(h_act_stage_);
+    for (auto e : act_ev_)
+        (void)hipEventDestroy(e);
     (void)hipFree(d_setup_slots_);
-    (void)hipFree(L_.tiles);
-    (void)hipFree(L_.n_tiles);
-    (void)hipFree(L_.n_slots);
     (void)hipFree(L_.counters);
     (void)hipFree(L_.exp_q);
     (void)hipFree(L_.fresh_q);
@@ -437,6 +443,8 @@ void FrEngine::write_ctl(int slot, const FrCtl &c)
 
 void FrEngine::set_active(const std::vector<int> &slots)
 {
+    if ((int)slots.size() > max_slots_)
+        throw std::runtime_error("set_active: too many slots");
     active_ = slots;
     h_tiles_.clear();
     for (int s : slots) {
@@ -445,17 +453,24 @@ void FrEngine::set_active(const std::vector<int> &slots)
         for (int t = 0; t < nt; ++t)
             h_tiles_.push_back(make_int2(s, t));
     }
-    const int nt = (int)h_tiles_.size();
+    const long nt = (long)h_tiles_.size();
+    if (nt > act_max_tiles_)
+        throw std::runtime_error("set_active: tile table overflow");
+    /* one pinned record, one copy; the ring entry is reused once its copy has read it */
+    const int k = act_next_;
+    act_next_ = (act_next_ + 1) % FR_ACT_RING;
+    uint8_t *stage = h_act_stage_ + (size_t)k * act_bytes_;
+    FR_CHECK(hipEventSynchronize(act_ev_[k]));
+    int *hdr = (int *)stage;
+    hdr[0] = (int)nt;
+    hdr[1] = (int)slots.size();
     if (!slots.empty())
-        FR_CHECK(hipMemcpyAsync(L_.slots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, st_));
+        std::memcpy(hdr + 4, slots.data(), sizeof(int) * slots.size());
     if (nt)
-        FR_CHECK(hipMemcpyAsync(L_.tiles, h_tiles_.data(), sizeof(int2) * nt, hipMemcpyHostToDevice, st_));
-    static thread_local int staged[2];
-    staged[0] = nt;
-    staged[1] = (int)slots.size();
-    FR_CHECK(hipMemcpyAsync(L_.n_tiles, &staged[0], sizeof(int), hipMemcpyHostToDevice, st_));
-    FR_CHECK(hipMemcpyAsync(L_.n_slots, &staged[1], sizeof(int), hipMemcpyHostToDevice, st_));
-    FR_CHECK(hipStreamSynchronize(st_));
+        std::memcpy(stage + act_tiles_off_, h_tiles_.data(), sizeof(int2) * nt);
+    const size_t bytes = nt ? act_tiles_off_ + sizeof(int2) * nt : act_tiles_off_;
+    FR_CHECK(hipMemcpyAsync(d_act_, stage, bytes, hipMemcpyHostToDevice, st_));
+    FR_CHECK(hipEventRecord(act_ev_[k], st_));
 }
 
 void FrEngine::setup(const std::vector<int> &slots, int goal_mode)

@@ -68,6 +68,10 @@ enum { FR_GATE_NONE = 0, FR_GATE_OPT = 1, FR_GATE_FOUND = 2 };
 void fr_launch_best_node(const FrLaunch &L, const int *d_slots, int n, int max_N, const FrBestNodeArgs *ov,
                          hipStream_t st, int gate = FR_GATE_NONE);
 void fr_launch_walk(const FrLaunch &L, const int *d_slots, int n, hipStream_t st, int gate = FR_GATE_NONE);
+/* pinned-host records of finished plans (control block, then the path from byte FR_PACK_PATH_OFF) */
+#define FR_PACK_PATH_OFF 512
+void fr_launch_fin_pack(const FrLaunch &L, const int *d_list, int n, uint8_t *pack, long stride, int path_max,
+                        hipStream_t st);
 void fr_launch_opt_round(const FrLaunch &L, const int *d_slots, int n, int max_path, int max_N, hipStream_t st,
                          int gate = FR_GATE_NONE);
 void fr_launch_goal_scan_multi(const FrLaunch &L, const int *d_slots, int n, int max_N, int gate, hipStream_t st);
@@ -207,7 +211,7 @@ class FrEngine {
     void read_ctl(int slot, FrCtl *out);           /* synchronous */
     void write_ctl(int slot, const FrCtl &c);
 
-    /* the set of plans the iteration kernels walk */
+    /* the set of plans the iteration kernels walk (asynchronous, stream-ordered) */
     void set_active(const std::vector<int> &slots);
     int n_active() const { return (int)active_.size(); }
     const int *d_active() const { return L_.slots; }
@@ -236,6 +240,15 @@ class FrEngine {
     int int_next_ = 0;
     std::vector<int> active_;
     std::vector<int2> h_tiles_;
+    /* active lists in one device block [n_tiles, n_slots, 2 pad, slots (max_slots, even), tiles],
+     * uploaded in one copy from a pinned ring */
+    static const int FR_ACT_RING = 4;
+    void *d_act_ = nullptr;
+    uint8_t *h_act_stage_ = nullptr;
+    size_t act_bytes_ = 0, act_tiles_off_ = 0;
+    long act_max_tiles_ = 0;
+    hipEvent_t act_ev_[FR_ACT_RING] = {};
+    int act_next_ = 0;
     int *d_setup_slots_ = nullptr;
     int *d_scratch_ = nullptr;
 };

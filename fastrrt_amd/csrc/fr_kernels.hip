@@ -859,13 +859,20 @@ __global__ void __launch_bounds__(FR_BLOCK) k_setup_prep(const FrPlanDev *plans,
 }
 
 /* Row pass of the padded Chebyshev distance plane: horizontal distance to
- * the nearest obstacle pixel of the same row, capped at FR_DIST_CAP.  One
+ * the nearest obstacle pixel of the same row, capped at dev_dist_cap.  One
  * wave per padded row: a forward sweep carries the last obstacle to the
  * left, a backward sweep the next one to the right (ballots over 64-pixel
  * chunks). */
+/* Every reader of the distance plane compares it with values up to pad
+ * (the centre certificate: D >= pad), so distances are kept up to
+ * min(pad, FR_DIST_CAP) only; the column pass then scans at most that many
+ * rows either side. */
+__device__ __forceinline__ int dev_dist_cap(const FrPlanDev &P) { return P.pad < FR_DIST_CAP ? P.pad : FR_DIST_CAP; }
+
 __global__ void __launch_bounds__(FR_BLOCK) k_setup_rowdist(const FrPlanDev *plans, const int *slots)
 {
     const FrPlanDev &P = plans[slots[blockIdx.y]];
+    const int cap = dev_dist_cap(P);
     const int wpb = blockDim.x >> 6;
     const int lane = dev_lane();
     for (int zp = blockIdx.x * wpb + (int)(threadIdx.x >> 6); zp < P.PH; zp += gridDim.x * wpb) {
@@ -873,7 +880,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_setup_rowdist(const FrPlanDev *pla
         const int z = zp - P.pad;
         if (z < 0 || z >= P.H) {
             for (int xp = lane; xp < P.PW; xp += 64)
-                out[xp] = FR_DIST_CAP;
+                out[xp] = (uint8_t)cap;
             continue;
         }
         const uint8_t *m = P.mark + (long)z * P.W;
@@ -886,7 +893,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_setup_rowdist(const FrPlanDev *pla
             const int prev = upto ? b + 63 - __clzll((long long)upto) - P.pad : last;
             if (xp < P.PW) {
                 const int d = x - prev;
-                out[xp] = (uint8_t)(d < FR_DIST_CAP ? d : FR_DIST_CAP);
+                out[xp] = (uint8_t)(d < cap ? d : cap);
             }
             if (msk)
                 last = b + 63 - __clzll((long long)msk) - P.pad;
@@ -2664,6 +2671,46 @@ void fr_launch_best_node(const FrLaunch &L, const int *d_slots, int n, int max_N
                        gate);
     hipLaunchKernelGGL(k_best_node_pick, g, dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_slots, tx, tz, h, r, e, use,
                        gate);
+}
+
+/* End of a batch finalisation: each listed plan's control block and path
+ * (when it fits) written straight into pinned host memory, record q at
+ * pack + q * stride (the control block, then float4 waypoints from byte
+ * FR_PACK_PATH_OFF), and the control block returned to IDLE, so the host
+ * collects the results without copies.  One block per list entry. */
+__global__ void __launch_bounds__(FR_BLOCK) k_fin_pack(const FrPlanDev *plans, FrCtl *ctl, const int *list, int n,
+                                                       uint8_t *pack, long stride, int path_max)
+{
+    for (int q = blockIdx.x; q < n; q += gridDim.x) {
+        const int s = list[q];
+        uint8_t *rec = pack + (long)q * stride;
+        const FrCtl c = ctl[s];
+        const uint32_t *cw = reinterpret_cast<const uint32_t *>(&c);
+        uint32_t *hw = reinterpret_cast<uint32_t *>(rec);
+        for (int k = threadIdx.x; k < (int)(sizeof(FrCtl) / 4); k += blockDim.x)
+            hw[k] = cw[k];
+        const int len = c.goal_reached ? (c.path_len > 0 ? c.path_len : 0) : 0;
+        if (len <= path_max) {
+            const float4 *src = reinterpret_cast<const float4 *>(plans[s].path);
+            float4 *dst = reinterpret_cast<float4 *>(rec + FR_PACK_PATH_OFF);
+            for (int k = threadIdx.x; k < len; k += blockDim.x)
+                dst[k] = src[k];
+        }
+        __threadfence_system();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            FrCtl idle = {};
+            idle.status = FR_IDLE;
+            ctl[s] = idle;
+        }
+        __syncthreads();
+    }
+}
+
+void fr_launch_fin_pack(const FrLaunch &L, const int *d_list, int n, uint8_t *pack, long stride, int path_max,
+                        hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fin_pack, dim3(n), dim3(FR_BLOCK), 0, st, L.plans, L.ctl, d_list, n, pack, stride, path_max);
 }
 
 void fr_launch_walk(const FrLaunch &L, const int *d_slots, int n, hipStream_t st, int gate)
