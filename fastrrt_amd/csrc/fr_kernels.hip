@@ -393,51 +393,85 @@ __device__ int dev_lane_probe(const FrPlanDev &P, int x, int z, float h, FrOpen 
     return open ? 0 : 1;
 }
 
-/* Group phase: false when a sample of an open group is an obstacle. */
-__device__ bool dev_lane_groups(const FrPlanDev &P, int x, int z, const FrOpen &o)
+/* Sampling of open group g of the footprint at (x, z): true when one of its
+ * samples is an obstacle. */
+__device__ __forceinline__ bool dev_group_hit(const FrPlanDev &P, int x, int z, float s, float c,
+                                              unsigned long long code, int g)
 {
     const uint8_t *D = P.dist;
     const int TW = P.TW, pad = P.pad;
     const int gc = (2 * P.mx + FR_GROUP) / FR_GROUP;
     const float fxc = (float)x, fzc = (float)z;
-    uint32_t open = o.open;
-    while (open) {
-        const int g = __ffs((int)open) - 1;
-        open &= open - 1;
-        int i0, i1, j0, j1, ic, jc, pcx, pcz;
-        const int a = g / gc;
-        dev_group_box(P, a, g - a * gc, &i0, &i1, &j0, &j1, &ic, &jc);
-        dev_fp_cell(jc, ic, o.s, o.c, fxc, fzc, &pcx, &pcz);
-        const int pd = (int)((o.code >> (2 * g)) & 3ull) + 1;
-        int hit = 0;
+    int i0, i1, j0, j1, ic, jc, pcx, pcz;
+    const int a = g / gc;
+    dev_group_box(P, a, g - a * gc, &i0, &i1, &j0, &j1, &ic, &jc);
+    dev_fp_cell(jc, ic, s, c, fxc, fzc, &pcx, &pcz);
+    const int pd = (int)((code >> (2 * g)) & 3ull) + 1;
+    int hit = 0;
 #pragma unroll
-        for (int di = 0; di < FR_GROUP; ++di) {
+    for (int di = 0; di < FR_GROUP; ++di) {
 #pragma unroll
-            for (int dj = 0; dj < FR_GROUP; ++dj) {
-                const int i = i0 + di, j = j0 + dj;
-                if (i <= i1 && j <= j1) {
-                    int xl, zl;
-                    dev_fp_cell(j, i, o.s, o.c, fxc, fzc, &xl, &zl);
-                    const int ax = xl > pcx ? xl - pcx : pcx - xl, az = zl > pcz ? zl - pcz : pcz - zl;
-                    if (ax >= pd || az >= pd)
-                        hit |= D[fr_dist_at(TW, xl + pad, zl + pad)] == 0;
-                }
+        for (int dj = 0; dj < FR_GROUP; ++dj) {
+            const int i = i0 + di, j = j0 + dj;
+            if (i <= i1 && j <= j1) {
+                int xl, zl;
+                dev_fp_cell(j, i, s, c, fxc, fzc, &xl, &zl);
+                const int ax = xl > pcx ? xl - pcx : pcx - xl, az = zl > pcz ? zl - pcz : pcz - zl;
+                if (ax >= pd || az >= pd)
+                    hit |= D[fr_dist_at(TW, xl + pad, zl + pad)] == 0;
             }
         }
-        if (hit)
+    }
+    return hit != 0;
+}
+
+/* Per-wave record of up to 64 footprints waiting for their group phase. */
+struct FrOpenSet {
+    int x[64], z[64];
+    float s[64], c[64];
+    uint32_t open[64];
+    int pref[64];
+    unsigned long long code[64];
+};
+
+/* Group phase of footprints 0..m-1 of a per-wave set, the open groups of all
+ * of them dealt to the lanes in rounds of 64 (a footprint near obstacles
+ * with many open groups no longer holds the wave for one round per group).
+ * False when any collides. */
+__device__ bool dev_groups_dealt(const FrPlanDev &P, FrOpenSet &b, int m)
+{
+    const int lane = dev_lane();
+    const int cnt = lane < m ? __popc(b.open[lane]) : 0;
+    const int incl = dev_wave_incl_scan(cnt);
+    const int total = dev_bcast_i(incl, 63);
+    if (!total)
+        return true;
+    b.pref[lane] = incl - cnt;
+    __builtin_amdgcn_wave_barrier();
+    for (int t0 = 0; t0 < total; t0 += 64) {
+        const int t = t0 + lane;
+        bool hit = false;
+        if (t < total) {
+            /* footprint j: the last with pref[j] <= t (it has open groups) */
+            int lo = 0, hi = m - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (b.pref[mid] <= t)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            uint32_t open = b.open[lo];
+            for (int r = t - b.pref[lo]; r > 0; --r)
+                open &= open - 1;
+            hit = dev_group_hit(P, b.x[lo], b.z[lo], b.s[lo], b.c[lo], b.code[lo], __ffs((int)open) - 1);
+        }
+        if (__ballot(hit))
             return false;
     }
     return true;
 }
 
-__device__ bool dev_lane_feasible(const FrPlanDev &P, int x, int z, float h)
-{
-    FrOpen o;
-    const int r = dev_lane_probe(P, x, z, h, &o);
-    if (r != 0)
-        return r > 0;
-    return dev_lane_groups(P, x, z, o);
-}
 
 /* One Hermite point of graph_direct_goal_connection.cu:16-167: position,
  * in-bounds cell, first derivative. */
@@ -478,21 +512,21 @@ struct FrOpenBuf {
     unsigned long long code[FR_OPEN_BUF];
 };
 
-/* Group phase of the first m (<= 64) buffered footprints, one per lane; the
- * rest move to the front.  False when any collides. */
-__device__ bool dev_open_flush(const FrPlanDev &P, FrOpenBuf &b, int m, int *nb)
+/* Group phase of the first m (<= 64) buffered footprints (their groups dealt
+ * to the lanes); the rest move to the front.  False when any collides. */
+__device__ bool dev_open_flush(const FrPlanDev &P, FrOpenBuf &b, FrOpenSet &set, int m, int *nb)
 {
     const int lane = dev_lane();
-    bool bad = false;
     if (lane < m) {
-        FrOpen o;
-        o.s = b.s[lane];
-        o.c = b.c[lane];
-        o.open = b.open[lane];
-        o.code = b.code[lane];
-        bad = !dev_lane_groups(P, b.x[lane], b.z[lane], o);
+        set.x[lane] = b.x[lane];
+        set.z[lane] = b.z[lane];
+        set.s[lane] = b.s[lane];
+        set.c[lane] = b.c[lane];
+        set.open[lane] = b.open[lane];
+        set.code[lane] = b.code[lane];
     }
-    if (__ballot(bad))
+    __builtin_amdgcn_wave_barrier();
+    if (!dev_groups_dealt(P, set, m))
         return false;
     *nb -= m;
     if (lane < *nb) {
@@ -708,7 +742,10 @@ __device__ __forceinline__ float dev_wave_curve(const FrPlanDev &P, const FrHerm
     float node_cost = base;
     if (passes & 2) {
         __shared__ FrOpenBuf s_open[FR_BLOCK / 64];
-        FrOpenBuf &ob = s_open[__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))];
+        __shared__ FrOpenSet s_oset[FR_BLOCK / 64];
+        const int wslot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+        FrOpenBuf &ob = s_open[wslot];
+        FrOpenSet &os = s_oset[wslot];
         int nb = 0;
         const unsigned long long below = (1ull << lane) - 1ull;
         last_x = -1;
@@ -751,11 +788,11 @@ __device__ __forceinline__ float dev_wave_curve(const FrPlanDev &P, const FrHerm
                 ob.code[r] = o.code;
             }
             nb += __popcll(om);
-            if (nb >= 64 && !dev_open_flush(P, ob, 64, &nb))
+            if (nb >= 64 && !dev_open_flush(P, ob, os, 64, &nb))
                 return -1;
         }
         while (nb > 0)
-            if (!dev_open_flush(P, ob, nb < 64 ? nb : 64, &nb))
+            if (!dev_open_flush(P, ob, os, nb < 64 ? nb : 64, &nb))
                 return -1;
     }
     if (last_x != gx && last_z != gz)
@@ -1200,17 +1237,27 @@ __global__ void __launch_bounds__(FR_BLOCK) k_select(const FrPlanDev *plans, FrC
  * rollout, one cell per lane, all lanes' loads in flight together; the cost
  * is added cell by cell in walk order.  False when any footprint collides. */
 __device__ bool dev_roll_flush(const FrPlanDev &P, const int *bx, const int *bz, const float *bh, int m,
-                               float &node_cost)
+                               float &node_cost, FrOpenSet &set)
 {
     const int lane = dev_lane();
     float add = 0;
-    bool bad = false;
+    int pr = 1;
+    FrOpen o = {};
     if (lane < m) {
         const int cx = bx[lane], cz = bz[lane];
         add = (P.intrinsic ? P.intrinsic[cz * P.W + cx] : 0.0f) + 1.0f;
-        bad = !dev_lane_feasible(P, cx, cz, bh[lane]);
+        pr = dev_lane_probe(P, cx, cz, bh[lane], &o);
+        set.x[lane] = cx;
+        set.z[lane] = cz;
+        set.s[lane] = o.s;
+        set.c[lane] = o.c;
+        set.open[lane] = pr == 0 ? o.open : 0u;
+        set.code[lane] = o.code;
     }
-    if (__ballot(bad))
+    if (__ballot(pr < 0))
+        return false;
+    __builtin_amdgcn_wave_barrier();
+    if (__ballot(pr == 0) && !dev_groups_dealt(P, set, m))
         return false;
     for (int r = 0; r < m; ++r)
         node_cost += dev_bcast_f(add, r);
@@ -1223,6 +1270,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *__restrict
     __shared__ int s_bx[FR_BLOCK / 64][FR_ROLL_BUF];
     __shared__ int s_bz[FR_BLOCK / 64][FR_ROLL_BUF];
     __shared__ float s_bh[FR_BLOCK / 64][FR_ROLL_BUF];
+    __shared__ FrOpenSet s_set[FR_BLOCK / 64];
     const int n = counters[FR_CNT_EXP];
     const int wpb = blockDim.x >> 6;
     /* wave index made uniform, so a rollout's descriptor and its plan's
@@ -1356,7 +1404,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *__restrict
             }
             while (nb >= 64 || (done && nb > 0)) {
                 const int m = nb < 64 ? nb : 64;
-                if (!dev_roll_flush(P, bx, bz, bh, m, node_cost)) {
+                if (!dev_roll_flush(P, bx, bz, bh, m, node_cost, s_set[wv])) {
                     ok = false;
                     break;
                 }
