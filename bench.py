@@ -70,6 +70,8 @@ def parse():
     ap.add_argument("--max-iter", type=int, default=200)
     ap.add_argument("--opt-rounds", type=int, default=20)
     ap.add_argument("--plain-plans", type=int, default=-1, help="plans of the first timed step run plain (-1: all)")
+    ap.add_argument("--plain-steps", type=int, default=1,
+                    help="timed steps whose plans also run plain (success compared plan by plan)")
     ap.add_argument("--cpu-sample", type=int, default=2, help="plans timed on the CPU oracle (0 = skip)")
     ap.add_argument("--iters-per-poll", type=int, default=1)
     ap.add_argument("--roofline-iters", type=int, default=32)
@@ -289,7 +291,10 @@ def main():
                "found_p50_p99_max_ms": [round(float(np.percentile(lat_ok, q)), 3) for q in (50, 99, 100)] if lat_ok else None,
                "failed_p50_p99_max_ms": [round(float(np.percentile(lat_fail, q)), 3) for q in (50, 99, 100)] if lat_fail else None}
     # plain form on the plans of the first timed step, with kernel timing
-    plain_set = first if args.plain_plans < 0 else first[:args.plain_plans]
+    plain_set = [pl for k in range(args.warmup, args.warmup + min(max(args.plain_steps, 1), args.steps))
+                 for pl in steps[k]]
+    if args.plain_plans >= 0:
+        plain_set = plain_set[:args.plain_plans]
     plain, oks = None, {}
     prof_p = None
     if plain_set:
@@ -298,7 +303,7 @@ def main():
         plain, oks = run_plain(P, plain_set, args, frame_of)
         L.fastrrt_profile_enable(False)
         prof_p = read_prof(L)
-    same = {r.tag: r.ok for r in results[0] if r.tag in oks}
+    same = {r.tag: r.ok for rs in results for r in rs if r.tag in oks}
     if same:
         plain["success_same_plans_batched"] = float(np.mean(list(same.values())))
         plain["plans_solved_plain_not_batched"] = int(sum(1 for t in same if oks[t] and not same[t]))

@@ -1521,36 +1521,41 @@ __device__ __forceinline__ bool dev_solve_runs(const FrCtl &C)
 __global__ void __launch_bounds__(FR_BLOCK) k_erase_mark(const FrPlanDev *plans, const FrCtl *ctl, const int2 *tiles, const int *n_tiles_p)
 {
     const int n_tiles = *n_tiles_p;
-    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const int wpb = blockDim.x >> 6;
+    const int nw = gridDim.x * wpb;
+    const int lane = dev_lane();
+    for (int t = blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); t < n_tiles; t += nw) {
         const int2 tl = tiles[t];
-        const FrPlanDev &P = plans[tl.x];
-        const FrCtl &C = ctl[tl.x];
+        const int slot = __builtin_amdgcn_readfirstlane(tl.x);
+        const FrPlanDev &P = plans[slot];
+        const FrCtl &C = ctl[slot];
         if (C.status != FR_SEARCHING || !dev_solve_runs(C))
             continue;
         const int nodes = C.n_node_pre - C.n_conv;
-        const int base = tl.y * FR_TILE;
-        for (int k = 0; k < 4; ++k) {
-            const int pos = base + k * FR_BLOCK + threadIdx.x;
-            if (pos >= P.N)
-                break;
-            uint8_t m = 0;
-            if (P.type[pos] != FR_TYPE_NULL) {
-                long curr = pos;
-                int i = nodes + 4;
-                while (i-- > 0) {
-                    const int ppx = P.px[curr], ppz = P.pz[curr];
-                    if (ppx == -1 && ppz == -1)
-                        break;
-                    const long next = (long)ppz * P.W + ppx;
-                    const int tn = (next >= 0 && next < P.N) ? P.type[next] : FR_TYPE_NULL;
-                    if (tn == FR_TYPE_COLLISION || tn == FR_TYPE_NULL) {
-                        m = 1;
-                        break;
-                    }
-                    curr = next;
+        const int base = __builtin_amdgcn_readfirstlane(tl.y) * FR_TILE;
+        const uint4 v = dev_tile_types(P, base);
+        /* marks are read for non-NULL cells only (k_erase_apply) */
+        const uint32_t nz = dev_bytes_nz(v);
+        if (!__ballot(nz != 0))
+            continue;
+        for (uint32_t m = nz; m; m &= m - 1) {
+            const int pos = base + 16 * lane + __ffs((int)m) - 1;
+            uint8_t mk = 0;
+            long curr = pos;
+            int i = nodes + 4;
+            while (i-- > 0) {
+                const int ppx = P.px[curr], ppz = P.pz[curr];
+                if (ppx == -1 && ppz == -1)
+                    break;
+                const long next = (long)ppz * P.W + ppx;
+                const int tn = (next >= 0 && next < P.N) ? P.type[next] : FR_TYPE_NULL;
+                if (tn == FR_TYPE_COLLISION || tn == FR_TYPE_NULL) {
+                    mk = 1;
+                    break;
                 }
+                curr = next;
             }
-            P.mark[pos] = m;
+            P.mark[pos] = mk;
         }
     }
 }
