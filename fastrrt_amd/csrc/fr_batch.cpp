@@ -78,7 +78,7 @@ class FrBatch {
         if (max_plans <= 0 || max_cells <= 0)
             throw std::runtime_error("invalid batch capacity");
         FR_CHECK(hipHostMalloc((void **)&h_ctl_, sizeof(FrCtl) * max_plans, hipHostMallocDefault));
-        FR_CHECK(hipHostMalloc((void **)&h_fin_list_, sizeof(int) * fin_max_, hipHostMallocDefault));
+        FR_CHECK(hipHostMalloc((void **)&h_fin_list_, sizeof(int) * 2 * fin_max_, hipHostMallocDefault));
         FR_CHECK(hipHostMalloc((void **)&h_idle_, sizeof(FrCtl), hipHostMallocDefault));
         std::memset(h_idle_, 0, sizeof(FrCtl));
         h_idle_->status = FR_IDLE;
@@ -86,7 +86,7 @@ class FrBatch {
         FR_CHECK(hipHostMalloc((void **)&h_pack_, (size_t)fin_max_ * kPackStride, hipHostMallocMapped));
         FR_CHECK(hipHostGetDevicePointer((void **)&d_pack_, h_pack_, 0));
         FR_CHECK(hipMalloc(&d_list_, sizeof(int) * max_plans));
-        FR_CHECK(hipMalloc(&fin_list_, sizeof(int) * fin_max_));
+        FR_CHECK(hipMalloc(&fin_list_, sizeof(int) * 2 * fin_max_)); /* slots, then round limits */
         FR_CHECK(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
         FR_CHECK(hipEventCreateWithFlags(&fin_ev_, hipEventDisableTiming));
         eng_.upload_plans(); /* all-zero descriptors, including the padding plan's */
@@ -392,7 +392,7 @@ class FrBatch {
                 found.push_back(i);
             else if (h_ctl_[i].status == FR_FAILED)
                 failed.push_back(i);
-            else if (dl > 0 && t - slots_[i].p.t_submit + 1.5 * poll_ema_ + 0.1 > dl)
+            else if (dl > 0 && t - slots_[i].p.t_submit + 2.0 * poll_ema_ + 0.2 > dl)
                 failed.push_back(i); /* with a margin for a slower next iteration */
         }
         for (int sl : failed)
@@ -407,6 +407,20 @@ class FrBatch {
      * (:204-232) of up to fin_max_ plans that reached their goal, on the
      * second stream.  The control blocks were prepared by k_finish (rounds
      * from the request); the list is padded with the idle plan. */
+    /* A plan with a deadline runs as many of its optimisation rounds as fit
+     * in the time it has left: a round costs about fin_round_ms_ (measured),
+     * and its result reaches the host one poll after the graph ends. */
+    int round_limit(int s, double t) const
+    {
+        const Slot &sl = slots_[s];
+        const double dl = sl.p.req.deadline_ms;
+        if (dl <= 0)
+            return sl.p.req.optimize_rounds;
+        const double left = dl - (t - sl.p.t_submit) - 2.0 * poll_ema_ - 0.3;
+        const int r = (int)std::floor(left / std::max(fin_round_ms_, 1e-3)) - 1;
+        return std::max(0, std::min(r, sl.p.req.optimize_rounds));
+    }
+
     void launch_finalize(const std::vector<int> &found)
     {
         int rounds = 0;
@@ -419,9 +433,17 @@ class FrBatch {
         }
         if (!fin_exec_ || rounds > fin_rounds_)
             capture_finalize(std::max(rounds, fin_rounds_));
-        for (int q = 0; q < fin_max_; ++q)
-            h_fin_list_[q] = q < (int)fin_slots_.size() ? fin_slots_[q] : dummy_;
-        FR_CHECK(hipMemcpyAsync(fin_list_, h_fin_list_, sizeof(int) * fin_max_, hipMemcpyHostToDevice, st2_));
+        const double t = now_ms() - t0_;
+        fin_used_rounds_ = 0;
+        for (int q = 0; q < fin_max_; ++q) {
+            const bool real = q < (int)fin_slots_.size();
+            h_fin_list_[q] = real ? fin_slots_[q] : dummy_;
+            h_fin_list_[fin_max_ + q] = real ? round_limit(fin_slots_[q], t) : 0;
+            if (real)
+                fin_used_rounds_ = std::max(fin_used_rounds_, h_fin_list_[fin_max_ + q]);
+        }
+        fin_t_launch_ = t;
+        FR_CHECK(hipMemcpyAsync(fin_list_, h_fin_list_, sizeof(int) * 2 * fin_max_, hipMemcpyHostToDevice, st2_));
         FR_CHECK(hipGraphLaunch(fin_exec_, st2_));
         FR_CHECK(hipEventRecord(fin_ev_, st2_));
         for (int s : fin_slots_)
@@ -445,6 +467,7 @@ class FrBatch {
         const bool prof = fr_prof_on();
         fr_prof_enable(false);
         FR_CHECK(hipStreamBeginCapture(st2_, hipStreamCaptureModeThreadLocal));
+        fr_launch_fin_limits(L, fin_list_, fin_list_ + n, n, st2_);
         for (int r = 0; r < rounds; ++r) {
             fr_launch_goal_scan_multi(L, fin_list_, n, grid_cells, FR_GATE_OPT, st2_);
             fr_launch_best_node(L, fin_list_, n, grid_cells, nullptr, st2_, FR_GATE_OPT);
@@ -465,6 +488,11 @@ class FrBatch {
      * record is in pinned memory and every control block is IDLE again. */
     void collect_finalized()
     {
+        /* graph time per round, measured up to this poll (an upper bound) */
+        if (fin_used_rounds_ >= 5) {
+            const double dt = now_ms() - t0_ - fin_t_launch_;
+            fin_round_ms_ = 0.8 * fin_round_ms_ + 0.2 * (dt / (fin_used_rounds_ + 1));
+        }
         const int nf = (int)fin_slots_.size();
         for (int q = 0; q < nf; ++q) {
             const uint8_t *rec = h_pack_ + (size_t)q * kPackStride;
@@ -539,6 +567,8 @@ class FrBatch {
     hipGraph_t fin_graph_ = nullptr;
     hipGraphExec_t fin_exec_ = nullptr;
     int fin_rounds_ = 0;
+    int fin_used_rounds_ = 0;
+    double fin_t_launch_ = 0, fin_round_ms_ = 0.06;
     bool fin_busy_ = false;
     int finalizing_ = 0;
     std::vector<int> fin_slots_;

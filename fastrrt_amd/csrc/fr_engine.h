@@ -68,6 +68,8 @@ enum { FR_GATE_NONE = 0, FR_GATE_OPT = 1, FR_GATE_FOUND = 2 };
 void fr_launch_best_node(const FrLaunch &L, const int *d_slots, int n, int max_N, const FrBestNodeArgs *ov,
                          hipStream_t st, int gate = FR_GATE_NONE);
 void fr_launch_walk(const FrLaunch &L, const int *d_slots, int n, hipStream_t st, int gate = FR_GATE_NONE);
+/* per-entry cap on the optimisation rounds of a batch finalisation */
+void fr_launch_fin_limits(const FrLaunch &L, const int *d_list, const int *d_limits, int n, hipStream_t st);
 /* pinned-host records of finished plans (control block, then the path from byte FR_PACK_PATH_OFF) */
 #define FR_PACK_PATH_OFF 512
 void fr_launch_fin_pack(const FrLaunch &L, const int *d_list, int n, uint8_t *pack, long stride, int path_max,

@@ -2726,6 +2726,26 @@ void fr_launch_best_node(const FrLaunch &L, const int *d_slots, int n, int max_N
                        gate);
 }
 
+/* Start of a batch finalisation: list entry q may run at most limits[q]
+ * optimisation rounds (the time left before its deadline). */
+__global__ void __launch_bounds__(FR_BLOCK) k_fin_limits(FrCtl *ctl, const int *list, const int *limits, int n)
+{
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        FrCtl &C = ctl[list[q]];
+        if (C.status != FR_FOUND)
+            continue;
+        if (limits[q] < C.opt_limit)
+            C.opt_limit = limits[q] > 0 ? limits[q] : 0;
+        if (C.opt_rounds >= C.opt_limit)
+            C.opt_active = 0;
+    }
+}
+
+void fr_launch_fin_limits(const FrLaunch &L, const int *d_list, const int *d_limits, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fin_limits, dim3(1), dim3(FR_BLOCK), 0, st, L.ctl, d_list, d_limits, n);
+}
+
 /* End of a batch finalisation: each listed plan's control block and path
  * (when it fits) written straight into pinned host memory, record q at
  * pack + q * stride (the control block, then float4 waypoints from byte
