@@ -58,6 +58,7 @@ struct Slot {
     bool busy = false;
     bool finalizing = false;
     int ticket = -1;
+    long admit_seq = 0; /* first control-block snapshot that shows this plan */
     Pending p;
     std::unique_ptr<FrSlab> slab;
     long cells = 0;
@@ -77,7 +78,11 @@ class FrBatch {
     {
         if (max_plans <= 0 || max_cells <= 0)
             throw std::runtime_error("invalid batch capacity");
-        FR_CHECK(hipHostMalloc((void **)&h_ctl_, sizeof(FrCtl) * max_plans, hipHostMallocDefault));
+        for (int b = 0; b < 2; ++b) {
+            FR_CHECK(hipHostMalloc((void **)&h_snap_[b], sizeof(FrCtl) * max_plans, hipHostMallocDefault));
+            std::memset(h_snap_[b], 0, sizeof(FrCtl) * max_plans);
+            FR_CHECK(hipEventCreateWithFlags(&snap_ev_[b], hipEventDisableTiming));
+        }
         FR_CHECK(hipHostMalloc((void **)&h_fin_list_, sizeof(int) * 2 * fin_max_, hipHostMallocDefault));
         FR_CHECK(hipHostMalloc((void **)&h_idle_, sizeof(FrCtl), hipHostMallocDefault));
         std::memset(h_idle_, 0, sizeof(FrCtl));
@@ -105,7 +110,10 @@ class FrBatch {
             (void)hipGraphDestroy(fin_graph_);
         (void)hipEventDestroy(fin_ev_);
         (void)hipStreamDestroy(st2_);
-        (void)hipHostFree(h_ctl_);
+        for (int b = 0; b < 2; ++b) {
+            (void)hipHostFree(h_snap_[b]);
+            (void)hipEventDestroy(snap_ev_[b]);
+        }
         (void)hipHostFree(h_fin_list_);
         (void)hipHostFree(h_idle_);
         (void)hipHostFree(h_pack_);
@@ -130,7 +138,13 @@ class FrBatch {
 
     int in_flight() const { return (int)queue_.size() + running_; }
 
-    /* admit -> `iterations` loop iterations -> retire */
+    /* admit -> `iterations` loop iterations -> retire.  The host works one
+     * step behind the device: after queueing this step's iterations and a
+     * copy of the control blocks, it acts on the previous step's copy while
+     * the device runs, so the device does not wait for the host between
+     * steps.  A decision taken on that copy lands after the iterations
+     * already queued (a plan retired by its deadline may have run one more
+     * iteration; the deadline margin covers it). */
     int step(int iterations)
     {
         admit();
@@ -138,12 +152,30 @@ class FrBatch {
             if (running_ > finalizing_) {
                 for (int k = 0; k < iterations; ++k)
                     launch_iteration();
+                const int b = (int)(snap_seq_ & 1);
+                FR_CHECK(hipMemcpyAsync(h_snap_[b], eng_.launch().ctl, sizeof(FrCtl) * slots_.size(),
+                                        hipMemcpyDeviceToHost, eng_.stream()));
+                FR_CHECK(hipEventRecord(snap_ev_[b], eng_.stream()));
+                const long prev = snap_seq_ - 1;
+                ++snap_seq_;
+                if (prev >= 0 && prev >= snap_done_)
+                    process_snapshot(prev);
+            } else if (snap_seq_ - 1 >= snap_done_) {
+                process_snapshot(snap_seq_ - 1); /* the last copy not acted on */
             } else {
                 FR_CHECK(hipEventSynchronize(fin_ev_)); /* only finalisation left */
+                poll_and_retire(nullptr, -1);
             }
-            poll_and_retire();
         }
         return in_flight();
+    }
+
+    void process_snapshot(long seq)
+    {
+        const int b = (int)(seq & 1);
+        FR_CHECK(hipEventSynchronize(snap_ev_[b]));
+        snap_done_ = seq + 1;
+        poll_and_retire(h_snap_[b], seq);
     }
 
     bool poll(fastrrt_plan_result *res, float *path, int cap)
@@ -282,12 +314,8 @@ class FrBatch {
         eng_.upload_ints(d_list_, admitted.data(), (int)admitted.size());
         fr_launch_slab_reset(eng_.launch(), d_list_, (int)admitted.size(), eng_.stream());
         eng_.setup(admitted, FR_GOAL_LAZY);
-        for (int sl : admitted) {
-            FrCtl c{};
-            c.status = FR_SEARCHING;
-            c.pending_start = 1;
-            std::memcpy(&h_ctl_[sl], &c, sizeof(FrCtl));
-        }
+        for (int sl : admitted)
+            slots_[sl].admit_seq = snap_seq_;
         fr_launch_batch_start(eng_.launch(), d_list_, (int)admitted.size(), eng_.stream());
         FR_CHECK(hipGetLastError());
         running_ += (int)admitted.size();
@@ -363,12 +391,12 @@ class FrBatch {
         }
     }
 
-    void poll_and_retire()
+    /* snap: control blocks copied after the iterations of step `seq` (null:
+     * nothing new to read).  A slot admitted after that copy was taken is
+     * only checked against its deadline. */
+    void poll_and_retire(const FrCtl *snap, long seq)
     {
         const int ns = (int)slots_.size();
-        FR_CHECK(hipMemcpyAsync(h_ctl_, eng_.launch().ctl, sizeof(FrCtl) * ns, hipMemcpyDeviceToHost,
-                                eng_.stream()));
-        FR_CHECK(hipStreamSynchronize(eng_.stream()));
         std::vector<int> found, failed;
         const double t = now_ms() - t0_;
         /* time between polls: a plan whose deadline falls before the next poll
@@ -384,19 +412,22 @@ class FrBatch {
             collect_finalized();
             changed = true;
         }
+        static const FrCtl none{};
         for (int i = 0; i < ns; ++i) {
             if (!slots_[i].busy || slots_[i].finalizing)
                 continue;
             const double dl = slots_[i].p.req.deadline_ms;
-            if (h_ctl_[i].status == FR_FOUND)
+            const bool seen = snap && seq >= slots_[i].admit_seq;
+            const int st = seen ? snap[i].status : FR_SEARCHING;
+            if (st == FR_FOUND)
                 found.push_back(i);
-            else if (h_ctl_[i].status == FR_FAILED)
+            else if (st == FR_FAILED)
                 failed.push_back(i);
             else if (dl > 0 && t - slots_[i].p.t_submit + 2.0 * poll_ema_ + 0.2 > dl)
                 failed.push_back(i); /* with a margin for a slower next iteration */
         }
         for (int sl : failed)
-            retire(sl, FR_FAILED, nullptr, 0, h_ctl_[sl]);
+            retire(sl, FR_FAILED, nullptr, 0, snap && seq >= slots_[sl].admit_seq ? snap[sl] : none);
         if (!found.empty() && !fin_busy_)
             launch_finalize(found);
         if (changed || !failed.empty())
@@ -557,7 +588,10 @@ class FrBatch {
     std::deque<Pending> queue_;
     std::deque<Finished> done_;
     std::map<std::tuple<int, int, int, int>, std::vector<std::unique_ptr<FrSlab>>> pool_;
-    FrCtl *h_ctl_ = nullptr;
+    FrCtl *h_snap_[2] = {nullptr, nullptr};  /* pinned control-block copies, alternating */
+    hipEvent_t snap_ev_[2] = {nullptr, nullptr};
+    long snap_seq_ = 0;                      /* copies queued */
+    long snap_done_ = 0;                     /* copies acted on */
     int *d_list_ = nullptr;
     /* finalisation on the second stream */
     int dummy_;
