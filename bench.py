@@ -20,20 +20,25 @@ key (seed, stream), so a plan's search is the same in both forms.
 Success is compared plan by plan on the plans both forms ran.  Frames are
 uploaded before the timed region (device-resident inputs).
 
-roofline: the dominant kernel of the batched form, k_expand (one wave per
-rollout).  The timed region retires plans on wall-clock deadlines, so its
-work depends on timing (and a profiler's overhead changes it); the roofline
-is therefore measured on a deterministic pass after it: the first timed
-step's plans, batched, `--roofline-iters` iterations, no deadline, no
-optimisation rounds.  k_expand is timed with HIP events on the engine's
-stream around its captured graph.  `python bench.py --roofline-only` runs
-that pass alone (the command the committed rocprofv3 summaries profile).
-Algorithmic bytes per rollout: 24 B queue entry read, 8 B source heading +
-cost, 12 B result written, 16 B claim read-modify-write, 20 B of
-distance-plane centre pixels = 80 B.  `traffic` is the HBM bytes per k_expand
-dispatch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on that
-command, scaled by the same counters' reading of a copy kernel of known
-size (profiles/k_expand_traffic.json).
+roofline: the batched form's unit of work, one launch of the captured
+iteration graph (one loop iteration of every running plan: 17 kernels, of
+which the two lazy goal-plane kernels and k_expand take about half the time).
+The timed region retires plans on wall-clock deadlines, so its work depends
+on timing (and a profiler's overhead changes it); the roofline is therefore
+measured on a deterministic pass after it: the first timed step's plans,
+batched, `--roofline-iters` iterations, no deadline, no optimisation rounds.
+The graph is timed with HIP events recorded on the engine's stream around
+each launch.  `python bench.py --roofline-only` runs that pass alone (the
+command the committed rocprofv3 summaries profile).  Algorithmic bytes per
+plan-iteration (the path's per-unit figure): 6 B per grid cell (4 B node
+type, 1 B obstacle plane, 1 B goal-connection flag) + 8 B per expansion
+(source heading and cost) + 24 B per new node, with new nodes counted as
+expansions (an upper bound; the term is < 0.2 % of the total at 256^2).
+`traffic` is the HBM bytes per iteration measured by rocprofv3 --pmc
+FETCH_SIZE / WRITE_SIZE on that command, summed over the iteration's
+kernels and scaled by the same counters' reading of a copy kernel of known
+size (profiles/iteration_traffic.json).  The k_expand figures of the same
+pass are reported beside it.
 
 cpu_baseline: the CPU oracle (one thread) on `--cpu-sample` plans of the
 same workload.
@@ -53,7 +58,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-ROLLOUT_BYTES = 80.0
+CELL_BYTES = 6.0     # per grid cell per plan-iteration: node type 4 B, obstacle 1 B, goal flag 1 B
+EXP_BYTES = 8.0      # per expansion: source heading + cost
+NEW_BYTES = 24.0     # per new node: parent, heading, cost, type, derive, region
+ROLLOUT_BYTES = 80.0 # k_expand per rollout: queue entry 24, source 8, result 12, claim 16, centre pixels 20
 SEED = 1234
 
 
@@ -170,29 +178,41 @@ def run_batched(b, plans, args, frame_of, max_iter=None, opt_rounds=None, deadli
 
 
 def roofline_pass(L, batch, plans, args, frame_of):
-    """k_expand over a deterministic batched pass (no deadline)."""
+    """The iteration graph (and k_expand) over a deterministic batched pass."""
     L.fastrrt_profile_reset()
     L.fastrrt_profile_enable(True)
     res, _ = run_batched(batch, plans, args, frame_of, max_iter=args.roofline_iters, opt_rounds=0, deadline=0.0)
     L.fastrrt_profile_enable(False)
-    ex_n, ex_ms = read_prof(L)["expand"]
-    if not ex_n:
+    prof = read_prof(L)
+    it_n, it_ms = prof["iteration"]
+    ex_n, ex_ms = prof["expand"]
+    if not it_n or not ex_n:
         return None
+    cells = sum(pl.sc.width * pl.sc.height for pl in plans) / len(plans)
+    plan_iters = sum(r.iterations for r in res)
     rollouts = sum(r.expansions for r in res)
-    per_launch = ROLLOUT_BYTES * rollouts / ex_n
-    avg_ms = ex_ms / ex_n
+    per_launch = (CELL_BYTES * cells * plan_iters + (EXP_BYTES + NEW_BYTES) * rollouts) / it_n
+    avg_ms = it_ms / it_n
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_expand_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "iteration_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             tr = json.load(fh)
         if tr.get("batch") == len(plans) and tr.get("roofline_iters") == args.roofline_iters:
-            traffic = tr["hbm_bytes_per_dispatch"]
-    return {"bound": "hbm", "kernel": "k_expand", "achieved": round(achieved, 4), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 7), "traffic": traffic,
-            "avg_launch_ms": round(avg_ms, 5), "launches": ex_n, "rollouts_per_launch": round(rollouts / ex_n, 1),
+            traffic = tr["hbm_bytes_per_iteration"]
+    ex_avg = ex_ms / ex_n
+    ex_bytes = ROLLOUT_BYTES * rollouts / ex_n
+    return {"bound": "hbm", "kernel": "iteration graph (17 kernels, one launch per loop iteration of every plan)",
+            "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+            "avg_launch_ms": round(avg_ms, 5), "launches": it_n, "plans": len(plans),
+            "plan_iterations_per_launch": round(plan_iters / it_n, 1),
+            "rollouts_per_launch": round(rollouts / it_n, 1),
             "algorithmic_bytes_per_launch": round(per_launch, 1),
+            "k_expand": {"avg_launch_ms": round(ex_avg, 5), "launches": ex_n,
+                         "algorithmic_bytes_per_launch": round(ex_bytes, 1),
+                         "achieved_gbs": round(ex_bytes / (ex_avg * 1e-3) / 1e9, 4)},
             "pass": f"{len(plans)} plans of the first timed step, batched, {args.roofline_iters} iterations, "
                     "no deadline, no optimisation rounds"}
 
@@ -266,7 +286,8 @@ def main():
     batch = Batch(max_plans=B, max_cells=B * 256 * 256)
     first = steps[args.warmup]
     if args.roofline_only:
-        run_batched(batch, steps[0], args, frame_of, max_iter=args.roofline_iters, opt_rounds=0, deadline=0.0)
+        # warm-up on the same plans: both passes are the same deterministic work
+        run_batched(batch, first, args, frame_of, max_iter=args.roofline_iters, opt_rounds=0, deadline=0.0)
         print(json.dumps({"roofline": roofline_pass(L, batch, first, args, frame_of)}), flush=True)
         return
     for k in range(args.warmup):
