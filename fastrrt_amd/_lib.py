@@ -44,6 +44,7 @@ SIGNATURES = {
     "save_current_graph_state": (None, [c_void_p, c_char_p]),
     "load_graph_state": (None, [c_void_p, c_char_p]),
     "fastrrt_set_seed": (None, [c_void_p, c_u64, c_u64]),
+    "fastrrt_set_density_region": (None, [c_void_p, c_int]),
     "fastrrt_iterations": (c_int, [c_void_p]),
     "fastrrt_iteration_stats": (None, [c_void_p, ip, ip, ip]),
     "fastrrt_export_state": (None, [c_void_p, i32p, i32p, i32p, i32p, fp, fp, u8p, fp, fp]),
@@ -98,7 +99,7 @@ class PlanRequest(ctypes.Structure):
         ("lower_bound_x", c_int), ("lower_bound_z", c_int), ("upper_bound_x", c_int), ("upper_bound_z", c_int),
         ("max_steering_angle_deg", c_float), ("vehicle_length", c_float), ("max_curvature", c_float),
         ("max_path_size", c_float), ("dist_to_goal_tolerance", c_float), ("heading_error_tolerance_rad", c_float),
-        ("max_iterations", c_int), ("optimize_rounds", c_int), ("smart", c_int), ("pad0", c_int),
+        ("max_iterations", c_int), ("optimize_rounds", c_int), ("smart", c_int), ("region_side", c_int),
         ("deadline_ms", c_double), ("seed", c_u64), ("stream", c_u64), ("user_tag", c_i64),
     ]
 

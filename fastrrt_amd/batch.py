@@ -21,7 +21,8 @@ class PlanSpec:
     def __init__(self, frame: SearchFrame, start, goal, min_dist=(22, 40), ego_lb=(-1, -1), ego_ub=(-1, -1),
                  velocity=1.0, max_steering_deg=40.0, vehicle_length=5.412658773, max_curvature=-1.0,
                  max_path_size=40.0, dist_to_goal_tolerance=20.0, heading_error_tolerance=math.radians(10),
-                 max_iterations=200, optimize_rounds=20, smart=False, deadline_ms=0.0, seed=0, stream=0, tag=0):
+                 max_iterations=200, optimize_rounds=20, smart=False, deadline_ms=0.0, seed=0, stream=0, tag=0,
+                 region_side=0):
         self.frame = frame
         r = _lib.PlanRequest()
         r.frame = frame.get_cuda_ptr()
@@ -40,6 +41,7 @@ class PlanSpec:
         r.max_iterations = int(max_iterations)
         r.optimize_rounds = int(optimize_rounds)
         r.smart = 1 if smart else 0
+        r.region_side = int(region_side)
         r.deadline_ms = float(deadline_ms)
         r.seed = int(seed) & ((1 << 64) - 1)
         r.stream = int(stream) & ((1 << 64) - 1)

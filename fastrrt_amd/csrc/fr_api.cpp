@@ -1055,6 +1055,17 @@ void fastrrt_set_seed(void *ptr, uint64_t seed, uint64_t stream)
     })
 }
 
+void fastrrt_set_density_region(void *ptr, int side)
+{
+    FR_TRY_VOID({
+        FrFastRRT *r = R(ptr);
+        if (side < 16)
+            throw std::runtime_error("density region side must be at least 16 px");
+        r->g.prm.region_side = side;
+        r->g.rebind();
+    })
+}
+
 int fastrrt_iterations(void *ptr) { FR_TRY(-1, { return R(ptr)->iterations; }) }
 
 void fastrrt_iteration_stats(void *ptr, int *n_expansions, int *n_new, int *n_checked)

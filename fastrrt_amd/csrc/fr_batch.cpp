@@ -128,6 +128,8 @@ class FrBatch {
             throw std::runtime_error("request without a frame");
         if ((long)f->W * f->H > max_cells_)
             throw std::runtime_error("request grid exceeds the batch cell budget");
+        if (r.region_side != 0 && r.region_side < 16)
+            throw std::runtime_error("density region side must be 0 (128 px) or at least 16 px");
         Pending p;
         p.req = r;
         p.ticket = next_ticket_++;
@@ -238,7 +240,7 @@ class FrBatch {
         p.max_iter = r.max_iterations > 0 ? r.max_iterations : 0;
         p.seed = r.seed;
         p.stream = r.stream;
-        p.region_side = 128;
+        p.region_side = r.region_side > 0 ? r.region_side : 128;
     }
 
     std::unique_ptr<FrSlab> take_slab(int W, int H, int mx, int mz)

@@ -188,6 +188,12 @@ class FastRRT:
     def set_seed(self, seed: int, stream: int = 0) -> None:
         _lib.load().fastrrt_set_seed(self._ptr, seed, stream)
 
+    def set_density_region(self, side_px: int) -> None:
+        """Side of the density regions of smart expansion (128 in the
+        reference; 32 for 1024x1024 searches)."""
+        _lib.load().fastrrt_set_density_region(self._ptr, int(side_px))
+        _lib.check("set_density_region")
+
     def search_init(self, copy_intrinsic_costs_from_frame: bool = False) -> None:
         _lib.load().search_init(self._ptr, copy_intrinsic_costs_from_frame)
         _lib.check("search_init")

@@ -65,8 +65,10 @@ def _clear_box(obst, x, z, hx, hz):
     obst[max(0, z - hz):min(H, z + hz + 1), max(0, x - hx):min(W, x + hx + 1)] = False
 
 
-def make_frame(width, height, coverage, seed, keep_free=(), corridors=0, corridor_width=(20, 60)):
-    """Obstacle mask -> float3 frame.  keep_free: [(x, z, half_x, half_z), ...]."""
+def make_frame(width, height, coverage, seed, keep_free=(), corridors=0, corridor_width=(20, 60), carve=False):
+    """Obstacle mask -> float3 frame.  keep_free: [(x, z, half_x, half_z), ...].
+    corridors: random walks that add obstacle strips, or with `carve` clear
+    free corridors of the full width."""
     rng = _rng(seed)
     obst = np.zeros((height, width), dtype=bool)
     target = coverage * width * height
@@ -96,7 +98,11 @@ def make_frame(width, height, coverage, seed, keep_free=(), corridors=0, corrido
                 break
             r = cw // 2
             xi, zi = int(x), int(z)
-            obst[max(0, zi - r // 4):min(height, zi + r // 4 + 1), max(0, xi - r // 4):min(width, xi + r // 4 + 1)] = True
+            if carve:
+                obst[max(0, zi - r):min(height, zi + r + 1), max(0, xi - r):min(width, xi + r + 1)] = False
+            else:
+                obst[max(0, zi - r // 4):min(height, zi + r // 4 + 1),
+                     max(0, xi - r // 4):min(width, xi + r // 4 + 1)] = True
     for (x, z, hx, hz) in keep_free:
         _clear_box(obst, x, z, hx, hz)
     frame = np.zeros((height, width, 3), dtype=np.float32)
@@ -124,6 +130,21 @@ def scenes_256_sweep(n, base_seed=1000, coverages=(0.05, 0.15, 0.30)):
             gh = float(rng.uniform(-math.pi / 4, math.pi / 4))
             out.append(scene_256(seed, cov, gx, gh))
     return out
+
+
+def scene_1024(seed, coverage=0.20, corridors=12, min_dist=(22, 40)):
+    """The 1024x1024 workload (136 m): rectangles to `coverage`, then free
+    random-walk corridors 20-60 px wide carved through them, start
+    (512,512,0), goal x ~ U[256,768] on row 0, goal heading ~ U[-pi/4, pi/4]."""
+    rng = _rng(seed ^ 0x51ED270B)
+    gx = int(rng.integers(256, 769))
+    gh = float(rng.uniform(-math.pi / 4, math.pi / 4))
+    hx, hz = min_dist[0] // 2 + 1, min_dist[1] // 2 + 1
+    start = (512, 512, 0.0)
+    goal = (gx, 0, gh)
+    frame = make_frame(1024, 1024, coverage, seed, keep_free=[(start[0], start[1], hx, hz), (goal[0], goal[1], hx, hz)],
+                       corridors=corridors, carve=True)
+    return Scene(frame, start, goal, seed, coverage, 136.0, min_dist, (-1, -1), (-1, -1))
 
 
 def scene_sized(seed, size, coverage, min_dist=(22, 40)):

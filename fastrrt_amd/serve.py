@@ -8,6 +8,9 @@ served  - a Poisson stream of plan requests of mixed grid size arrives in
           p50/p99 latency from arrival, deadline-miss rate, success rate.
 batched - a fixed set of plans of uneven grid size submitted together and
           advanced together until every plan is done.
+1024    - the 1024x1024 configuration (rectangles plus random-walk strips,
+          smart expansion with 32 px density regions, 300 ms per plan), plain
+          loop and batched form on the same scenes.
 
 Frames are drawn from a pool of seeded scenes per size (uploaded once, as a
 deployment keeps its frames on the device); every request has its own
@@ -15,6 +18,7 @@ start/goal-bearing scene and its own Philox stream.
 
     python -m fastrrt_amd.serve --mode served --duration 10 --rate 2000
     python -m fastrrt_amd.serve --mode batched --plans 512
+    python -m fastrrt_amd.serve --mode 1024 --plans 24
 """
 import argparse
 import json
@@ -148,9 +152,77 @@ def run_batched(n_plans=512, sizes=BATCHED_SIZES, seed=12, max_iter=200, opt_rou
             "max_iter": max_iter, "deadline_ms": deadline_ms}
 
 
+def run_1024(n_plans=24, seed=13, timeout_ms=300.0, smart=True, region_side=32, opt_rounds=20, coverage=0.20,
+             forms=("plain", "batched")):
+    """The 1024x1024 configuration: `n_plans` scenes (20 % rectangles plus
+    strips), smart expansion with 32 px density regions, a 300 ms budget per
+    plan.  plain: one plan at a time through the FastRRT handle (timeout
+    = the budget, the reference's protocol); batched: all plans submitted
+    together with deadline = the budget."""
+    from . import pyfastrrt as P
+    scs = [scenes.scene_1024(seed * 1000 + i, coverage) for i in range(n_plans)]
+    frames = [SearchFrame(sc.frame, sc.class_costs) for sc in scs]
+    out = {"mode": "1024", "plans": n_plans, "budget_ms": timeout_ms, "smart": smart, "region_side": region_side,
+           "coverage": coverage}
+    if "plain" in forms:
+        sc0 = scs[0]
+        ego = P.EgoParams(1024, 1024, sc0.perception_m, sc0.perception_m, 40.0, 5.412658773, sc0.ego_lb, sc0.ego_ub,
+                          -1.0, sc0.class_costs)
+        rrt = P.FastRRT(ego)
+        rrt.set_density_region(region_side)
+        lat, ok, iters, loop_ms = [], [], [], 0.0
+        t_all = time.perf_counter()
+        for i, sc in enumerate(scs):
+            sp = P.SearchParams(frames[i], P.Waypoint(sc.start[0], sc.start[1], P.angle(sc.start[2])),
+                                P.Waypoint(sc.goal[0], sc.goal[1], P.angle(sc.goal[2])), 1.0, sc.min_dist,
+                                int(timeout_ms), 40.0, 20.0, P.angle.new_deg(10))
+            rrt.set_seed(seed, 900000 + i)
+            t0 = time.perf_counter()
+            rrt.set_plan_data(sp)
+            rrt.search_init()
+            t1 = time.perf_counter()
+            it = 0
+            while not rrt.goal_reached() and rrt.loop(smart):
+                it += 1
+            t2 = time.perf_counter()
+            good = rrt.goal_reached()
+            if good:
+                r = 0
+                while r < opt_rounds and rrt.path_optimize():
+                    r += 1
+            path = rrt.get_planned_path() if good else None
+            t3 = time.perf_counter()
+            lat.append((t3 - t0) * 1e3)
+            ok.append(bool(good and path))
+            iters.append(it)
+            loop_ms += (t2 - t1) * 1e3
+        wall = time.perf_counter() - t_all
+        out["plain"] = {"plans_per_s": n_plans / wall, "p50_ms": float(np.percentile(lat, 50)),
+                        "p99_ms": float(np.percentile(lat, 99)), "success": float(np.mean(ok)),
+                        "mean_iterations": float(np.mean(iters)),
+                        "iteration_ms": loop_ms / max(1, sum(iters))}
+    if "batched" in forms:
+        b = Batch(max_plans=n_plans, max_cells=n_plans * 1024 * 1024)
+        res = None
+        for rep in range(2):  # the first pass allocates the slabs
+            t0 = time.perf_counter()
+            for i, sc in enumerate(scs):
+                b.submit(PlanSpec(frames[i], sc.start, sc.goal, min_dist=sc.min_dist, ego_lb=sc.ego_lb,
+                                  ego_ub=sc.ego_ub, max_iterations=100000, optimize_rounds=opt_rounds,
+                                  deadline_ms=timeout_ms, seed=seed, stream=900000 + i, tag=i, smart=smart,
+                                  region_side=region_side))
+            res = b.drain()
+            wall = time.perf_counter() - t0
+        lat = np.array([r.latency_ms for r in res])
+        out["batched"] = {"plans_per_s": n_plans / wall, "p50_ms": float(np.percentile(lat, 50)),
+                          "p99_ms": float(np.percentile(lat, 99)), "success": float(np.mean([r.ok for r in res])),
+                          "mean_iterations": float(np.mean([r.iterations for r in res]))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", choices=("served", "batched"), default="served")
+    ap.add_argument("--mode", choices=("served", "batched", "1024"), default="served")
     ap.add_argument("--duration", type=float, default=10.0)
     ap.add_argument("--rate", type=float, default=2000.0)
     ap.add_argument("--plans", type=int, default=512)
@@ -164,9 +236,11 @@ def main():
         sizes = tuple(int(s) for s in args.sizes.split(",")) if args.sizes else SERVED_SIZES
         out = run_served(args.duration, args.rate, sizes=sizes, per_size=args.per_size, max_plans=args.max_plans,
                          max_iter=args.max_iter)
-    else:
+    elif args.mode == "batched":
         sizes = tuple(int(s) for s in args.sizes.split(",")) if args.sizes else BATCHED_SIZES
         out = run_batched(args.plans, sizes=sizes, max_iter=args.max_iter, deadline_ms=args.deadline_ms)
+    else:
+        out = run_1024(args.plans)
     print(json.dumps(out), flush=True)
 
 

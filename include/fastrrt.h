@@ -85,6 +85,10 @@ void load_graph_state(void *ptr, const char *filename);
 /* New: Philox key and stream of the next search_init (the reference seeds
  * cuRAND from the wall clock, random_number_gen.cu:23-40). */
 void fastrrt_set_seed(void *ptr, uint64_t seed, uint64_t stream);
+/* New: side in pixels of the density regions of smart expansion (the
+ * reference fixes 128, graph_expansion_search_zones.cu:35-40; a 1024x1024
+ * search uses 32).  At least 16; kept across set_plan_data. */
+void fastrrt_set_density_region(void *ptr, int side);
 /* New: loop() iterations since search_init. */
 int fastrrt_iterations(void *ptr);
 /* New: work of the last loop(): rollouts, nodes added, nodes goal-checked. */
@@ -197,7 +201,7 @@ typedef struct fastrrt_plan_request {
     int max_iterations;       /* loop() iterations before the plan fails (0: none) */
     int optimize_rounds;      /* path_optimize() calls after the goal is reached */
     int smart;                /* smart (density-gated) expansion */
-    int pad0;
+    int region_side;          /* density region side in px for smart expansion (0: 128, as the reference) */
     double deadline_ms;       /* from submission; the plan fails when it passes (<= 0: none) */
     uint64_t seed, stream;    /* Philox key and stream */
     int64_t user_tag;

@@ -25,7 +25,7 @@ class Params:
 
 
 class OraclePlanner:
-    def __init__(self, sc, prm, seed, stream):
+    def __init__(self, sc, prm, seed, stream, region_side=None):
         self.L = O.lib()
         self.sc = sc
         self.r = self.L.og_rrt_new(sc.width, sc.height, sc.perception_m, sc.perception_m, prm.steer_deg,
@@ -37,6 +37,8 @@ class OraclePlanner:
         self.L.og_rrt_set_plan_data(self.r, O.fptr(self.frame), O.fptr(self.costs), len(self.costs), sc.start[0],
                                     sc.start[1], sc.start[2], sc.goal[0], sc.goal[1], sc.goal[2], prm.v,
                                     prm.min_dist[0], prm.min_dist[1], -1, prm.max_path, prm.dist_tol, prm.herr)
+        if region_side:
+            self.L.og_set_region_side(self.L.og_rrt_graph(self.r), region_side)
 
     def __del__(self):
         try:
@@ -85,7 +87,7 @@ class OraclePlanner:
 
 
 class GpuPlanner:
-    def __init__(self, sc, prm, seed, stream):
+    def __init__(self, sc, prm, seed, stream, region_side=None):
         from fastrrt_amd import pyfastrrt as P
         self.P = P
         self.sc = sc
@@ -98,6 +100,8 @@ class GpuPlanner:
                             prm.max_path, prm.dist_tol, P.angle(float(prm.herr)))
         self.rrt.set_seed(seed, stream)
         self.rrt.set_plan_data(sp)
+        if region_side:
+            self.rrt.set_density_region(region_side)
 
     def state(self):
         s = self.rrt.export_state(goal=False)

@@ -25,14 +25,14 @@ SMALL = Params(min_dist=(10, 16), max_path=30.0, dist_tol=12.0)
 FULL = Params()
 
 
-def _run_side_by_side(sc, prm, seed, stream, max_iter, smart=False, goal_planes_first=False):
+def _run_side_by_side(sc, prm, seed, stream, max_iter, smart=False, goal_planes_first=False, region_side=None):
     """Oracle and GPU planner in lock step, grids compared bit for bit after
     every loop().  The handle evaluates the goal-connection planes lazily; they
     are compared before the search (which finishes the pass up front) or after
     it (the search ran on lazily evaluated cells)."""
     print(f"[parity] seed={seed} size={sc.width} cov={sc.coverage} smart={smart}", flush=True)
-    o = OraclePlanner(sc, prm, seed, stream)
-    g = GpuPlanner(sc, prm, seed, stream)
+    o = OraclePlanner(sc, prm, seed, stream, region_side)
+    g = GpuPlanner(sc, prm, seed, stream, region_side)
     if goal_planes_first:
         bad = diff_goal_planes(o.goal_planes(), g.goal_planes())
         assert not bad, f"goal-connection planes differ: {bad}"
@@ -68,6 +68,15 @@ def test_loop_parity_small(seed):
 def test_loop_parity_small_smart(seed):
     sc = _small_scene(seed)
     _run_side_by_side(sc, SMALL, 99 + seed, seed, max_iter=40, smart=True)
+
+
+@pytest.mark.parametrize("side", [16, 32])
+def test_loop_parity_smart_density_regions(side):
+    """Smart expansion with density regions smaller than the reference's
+    128 px (the 1024x1024 configuration uses 32): region counts decide which
+    nodes expand, so every iteration's grid pins them."""
+    sc = _small_scene(31 + side)
+    _run_side_by_side(sc, SMALL, 500 + side, side, max_iter=40, smart=True, region_side=side)
 
 
 @pytest.mark.parametrize("seed", [3, 7])

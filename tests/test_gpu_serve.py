@@ -32,3 +32,17 @@ def test_batched_mixed_sizes_all_return():
     assert sum(v["plans"] for v in out["by_size"].values()) == 24
     assert out["success_rate"] > 0.0
     assert all(v["mean_iterations"] <= 80 for v in out["by_size"].values())
+
+
+def test_1024_configuration_small():
+    """The 1024x1024 runner (smart expansion, 32 px density regions) on two
+    plans, both forms; every result is a valid status and the figures are
+    finite."""
+    out = serve.run_1024(n_plans=2, seed=3, timeout_ms=60.0, opt_rounds=2)
+    print(f"[1024] {out}", flush=True)
+    for form in ("plain", "batched"):
+        f = out[form]
+        assert math.isfinite(f["plans_per_s"]) and f["plans_per_s"] > 0
+        assert 0.0 <= f["success"] <= 1.0
+        assert f["mean_iterations"] > 0
+    assert out["batched"]["p99_ms"] <= 60.0 + 5.0
