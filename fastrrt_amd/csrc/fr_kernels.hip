@@ -1518,46 +1518,135 @@ __device__ __forceinline__ bool dev_solve_runs(const FrCtl &C)
 
 /* __CUDA_solveGraphCollision_erase_trees (graph_collision.cu:33-83), deciding
  * on the pre-erase grid: marks only. */
-__global__ void __launch_bounds__(FR_BLOCK) k_erase_mark(const FrPlanDev *plans, const FrCtl *ctl, const int2 *tiles, const int *n_tiles_p)
+#define FR_ERASE_BLOCK 1024
+#define FR_ERASE_CAP 4096
+
+/* One block per plan that runs the erase.  The plan's non-NULL cells are
+ * gathered into LDS in cell order; each one's verdict (some ancestor is
+ * COLLISION or NULL) is then found by pointer jumping over parent indices in
+ * LDS, doubling the covered chain length each round, instead of each cell
+ * walking its chain through global memory.  The rounds cover at least the
+ * walk's bound of nodes + 4 steps; a chain ends at the root (its own parent
+ * in the jump) or at its first bad parent, as the walk does.  A plan with
+ * more than FR_ERASE_CAP non-NULL cells takes the walk. */
+__global__ void __launch_bounds__(FR_ERASE_BLOCK) k_erase_mark(const FrPlanDev *plans, const FrCtl *ctl,
+                                                               const int *slots, const int *n_slots_p)
 {
-    const int n_tiles = *n_tiles_p;
-    const int wpb = blockDim.x >> 6;
-    const int nw = gridDim.x * wpb;
-    const int lane = dev_lane();
-    for (int t = blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); t < n_tiles; t += nw) {
-        const int2 tl = tiles[t];
-        const int slot = __builtin_amdgcn_readfirstlane(tl.x);
-        const FrPlanDev &P = plans[slot];
-        const FrCtl &C = ctl[slot];
-        if (C.status != FR_SEARCHING || !dev_solve_runs(C))
-            continue;
-        const int nodes = C.n_node_pre - C.n_conv;
-        const int base = __builtin_amdgcn_readfirstlane(tl.y) * FR_TILE;
-        const uint4 v = dev_tile_types(P, base);
-        /* marks are read for non-NULL cells only (k_erase_apply) */
-        const uint32_t nz = dev_bytes_nz(v);
-        if (!__ballot(nz != 0))
-            continue;
-        for (uint32_t m = nz; m; m &= m - 1) {
-            const int pos = base + 16 * lane + __ffs((int)m) - 1;
-            uint8_t mk = 0;
-            long curr = pos;
-            int i = nodes + 4;
-            while (i-- > 0) {
-                const int ppx = P.px[curr], ppz = P.pz[curr];
-                if (ppx == -1 && ppz == -1)
-                    break;
-                const long next = (long)ppz * P.W + ppx;
-                const int tn = (next >= 0 && next < P.N) ? P.type[next] : FR_TYPE_NULL;
-                if (tn == FR_TYPE_COLLISION || tn == FR_TYPE_NULL) {
-                    mk = 1;
-                    break;
-                }
-                curr = next;
-            }
-            P.mark[pos] = mk;
-        }
+    __shared__ int s_pos[FR_ERASE_CAP];
+    __shared__ uint16_t s_anc[2][FR_ERASE_CAP];
+    __shared__ uint8_t s_bad[2][FR_ERASE_CAP];
+    __shared__ int s_wsum[FR_ERASE_BLOCK / 64];
+    if ((int)blockIdx.x >= *n_slots_p)
+        return;
+    const int slot = slots[blockIdx.x];
+    const FrPlanDev &P = plans[slot];
+    const FrCtl &C = ctl[slot];
+    if (C.status != FR_SEARCHING || !dev_solve_runs(C))
+        return;
+    const int nodes = C.n_node_pre - C.n_conv;
+    const int t = threadIdx.x, lane = dev_lane(), wid = t >> 6;
+    const int nchunks = (P.N + 15) / 16;
+    const int per = (nchunks + FR_ERASE_BLOCK - 1) / FR_ERASE_BLOCK;
+    const int c0 = t * per, c1 = c0 + per < nchunks ? c0 + per : nchunks;
+    /* count this thread's non-NULL cells, block-wide exclusive scan */
+    int cnt = 0;
+    for (int c = c0; c < c1; ++c) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(P.type + 16 * c);
+        uint32_t nz = dev_bytes_nz(v);
+        if (16 * c + 16 > P.N)
+            nz &= (1u << (P.N - 16 * c)) - 1u;
+        cnt += __popc(nz);
     }
+    const int incl = dev_wave_incl_scan(cnt);
+    if (lane == 63)
+        s_wsum[wid] = incl;
+    __syncthreads();
+    int base = 0, total = 0;
+    for (int w = 0; w < FR_ERASE_BLOCK / 64; ++w) {
+        base += w < wid ? s_wsum[w] : 0;
+        total += s_wsum[w];
+    }
+    const int off = base + incl - cnt;
+    if (total > FR_ERASE_CAP) {
+        /* the walk of graph_collision.cu:33-83 */
+        for (int c = c0; c < c1; ++c) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(P.type + 16 * c);
+            uint32_t nz = dev_bytes_nz(v);
+            if (16 * c + 16 > P.N)
+                nz &= (1u << (P.N - 16 * c)) - 1u;
+            for (uint32_t m = nz; m; m &= m - 1) {
+                const int pos = 16 * c + __ffs((int)m) - 1;
+                uint8_t mk = 0;
+                long curr = pos;
+                int i = nodes + 4;
+                while (i-- > 0) {
+                    const int ppx = P.px[curr], ppz = P.pz[curr];
+                    if (ppx == -1 && ppz == -1)
+                        break;
+                    const long next = (long)ppz * P.W + ppx;
+                    const int tn = (next >= 0 && next < P.N) ? P.type[next] : FR_TYPE_NULL;
+                    if (tn == FR_TYPE_COLLISION || tn == FR_TYPE_NULL) {
+                        mk = 1;
+                        break;
+                    }
+                    curr = next;
+                }
+                P.mark[pos] = mk;
+            }
+        }
+        return;
+    }
+    /* the non-NULL cells in cell order */
+    int w = off;
+    for (int c = c0; c < c1; ++c) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(P.type + 16 * c);
+        uint32_t nz = dev_bytes_nz(v);
+        if (16 * c + 16 > P.N)
+            nz &= (1u << (P.N - 16 * c)) - 1u;
+        for (uint32_t m = nz; m; m &= m - 1)
+            s_pos[w++] = 16 * c + __ffs((int)m) - 1;
+    }
+    __syncthreads();
+    /* first step of every chain: root (own parent), bad parent, or the parent's index */
+    for (int k = t; k < total; k += FR_ERASE_BLOCK) {
+        const int pos = s_pos[k];
+        const int ppx = P.px[pos], ppz = P.pz[pos];
+        int anc = k, bad = 0;
+        if (!(ppx == -1 && ppz == -1)) {
+            const long next = (long)ppz * P.W + ppx;
+            const int tn = (next >= 0 && next < P.N) ? P.type[next] : FR_TYPE_NULL;
+            if (tn == FR_TYPE_COLLISION || tn == FR_TYPE_NULL) {
+                bad = 1;
+            } else { /* non-NULL, so in the list */
+                int lo = 0, hi = total - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_pos[mid] < (int)next)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                anc = lo;
+            }
+        }
+        s_anc[0][k] = (uint16_t)anc;
+        s_bad[0][k] = (uint8_t)bad;
+    }
+    __syncthreads();
+    const int rounds = 32 - __clz(nodes + 4);
+    int cur = 0;
+    for (int r = 0; r < rounds; ++r) {
+        for (int k = t; k < total; k += FR_ERASE_BLOCK) {
+            const int a = s_anc[cur][k];
+            s_bad[cur ^ 1][k] = s_bad[cur][k] | s_bad[cur][a];
+            s_anc[cur ^ 1][k] = s_anc[cur][a];
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    /* marks are read for non-NULL cells only (k_erase_apply) */
+    for (int k = t; k < total; k += FR_ERASE_BLOCK)
+        P.mark[s_pos[k]] = s_bad[cur][k];
 }
 
 /* Apply the erase marks and __CUDA_solveGraphCollision_set_nodes
@@ -2658,7 +2747,8 @@ void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st)
     dim3 b(FR_BLOCK);
     hipLaunchKernelGGL(k_commit, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.exp_q, L.counters);
     hipLaunchKernelGGL(k_density, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles);
-    hipLaunchKernelGGL(k_erase_mark, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles);
+    hipLaunchKernelGGL(k_erase_mark, dim3(L.max_slots), dim3(FR_ERASE_BLOCK), 0, st, L.plans, L.ctl, L.slots,
+                       L.n_slots);
     hipLaunchKernelGGL(k_erase_apply, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles,
                        L.fresh_q, L.counters);
     hipLaunchKernelGGL(k_coll_parents, dim3(L.grid_items), b, 0, st, L.plans, L.fresh_q, L.counters);
