@@ -79,6 +79,18 @@ def test_loop_parity_smart_density_regions(side):
     _run_side_by_side(sc, SMALL, 500 + side, side, max_iter=40, smart=True, region_side=side)
 
 
+def test_loop_parity_deep_trees_behind_a_wall():
+    """A wall across the grid keeps the goal out of reach, so the tree grows
+    for many iterations (up to ~2 500 nodes, repeated collision erases of
+    deep subtrees): every iteration's grid is compared bit for bit."""
+    sc = scenes.scene_sized(77, 256, 0.03, min_dist=(10, 16))
+    frame = sc.frame.copy()
+    frame[40:60, :, 0] = scenes.OBSTACLE_CLASS
+    sc.frame = frame
+    o, g, it = _run_side_by_side(sc, SMALL, 4242, 7, max_iter=120)
+    assert not o.goal_reached() and it == 120
+
+
 @pytest.mark.parametrize("seed", [3, 7])
 def test_path_optimize_parity(seed):
     sc = _small_scene(seed, coverage=0.05)
