@@ -61,7 +61,7 @@ HBM_PEAK_GBS = 8000.0
 CELL_BYTES = 6.0     # per grid cell per plan-iteration: node type 4 B, obstacle 1 B, goal flag 1 B
 EXP_BYTES = 8.0      # per expansion: source heading + cost
 NEW_BYTES = 24.0     # per new node: parent, heading, cost, type, derive, region
-ROLLOUT_BYTES = 80.0 # k_expand per rollout: queue entry 24, source 8, result 12, claim 16, centre pixels 20
+ROLLOUT_BYTES = 76.0 # k_expand per rollout: queue entry 20, source 8, result 12, claim 16, centre pixels 20
 SEED = 1234
 
 

@@ -21,7 +21,6 @@
 #define FR_TILE 1024          /* cells per tile of the flattened tile table */
 #define FR_BLOCK 256          /* threads per block of the tile kernels */
 #define FR_DIST_CAP 63        /* largest cap of the Chebyshev obstacle-distance plane (min(pad, 63) is kept) */
-#define FR_CHK_STRIDE 64      /* deferred footprint tests kept per rollout */
 #define FR_BIG_COST 99999999999LL
 
 enum FrStatus {
@@ -138,7 +137,6 @@ struct FrExp {
     int end;                  /* -1: rejected */
     float heading;
     float cost;
-    int n_chk;                /* footprint tests deferred to the wave pass */
 };
 
 /* One node to goal-check / connect this iteration. */
@@ -156,6 +154,7 @@ struct FrFresh {
 struct FrPair {
     int fresh;
     int cell;
+    float cost;               /* node -> cell curve cost (set when the curve is valid) */
 };
 
 #define FR_FRESH_CONVERTED 1

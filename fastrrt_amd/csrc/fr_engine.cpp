@@ -279,8 +279,6 @@ FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
     FR_CHECK(hipMalloc(&d_scratch_, sizeof(int) * 64));
     L_.pair_cap = std::min<long>(std::max<long>(16 * queue_cap, 1L << 16), 1L << 26);
     FR_CHECK(hipMalloc(&L_.pairs, sizeof(FrPair) * L_.pair_cap));
-    L_.chk_cap = std::min<long>((queue_cap + 1) * FR_CHK_STRIDE, 1L << 24);
-    FR_CHECK(hipMalloc(&L_.chk, sizeof(int2) * L_.chk_cap));
     L_.demand_cap = queue_cap + 1;
     FR_CHECK(hipMalloc(&L_.demand, sizeof(int2) * L_.demand_cap));
     FR_CHECK(hipMalloc(&L_.demand2, sizeof(int2) * L_.demand_cap));
@@ -313,7 +311,6 @@ FrEngine::~FrEngine()
     for (auto e : int_ev_)
         (void)hipEventDestroy(e);
     (void)hipFree(L_.pairs);
-    (void)hipFree(L_.chk);
     (void)hipFree(L_.demand);
     (void)hipFree(L_.demand2);
     if (own_stream_)

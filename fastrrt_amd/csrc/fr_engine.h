@@ -33,8 +33,6 @@ struct FrLaunch {
     FrFresh *fresh_q;
     FrPair *pairs;
     long pair_cap;
-    int2 *chk;                /* per rollout: (cell, heading bits) footprint tests, FR_CHK_STRIDE each */
-    long chk_cap;
     int2 *demand;             /* (slot, cell) goal-connection evaluations queued this iteration */
     int2 *demand2;            /* the queued cells whose goal curves passed curvature and centres */
     long demand_cap;          /* of both */

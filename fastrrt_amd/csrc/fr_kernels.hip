@@ -92,7 +92,7 @@ __device__ __forceinline__ bool dev_fp_certified(const FrPlanDev &P, int x, int 
 
 /* kinematics.cu:123-231.  Returns 0 when rejected. */
 __device__ int dev_rollout(const FrPlanDev &P, int sx, int sz, float steer, float path_size, int *ex, int *ez,
-                           float *eh, float *ec, int2 *defer = nullptr, int *n_defer = nullptr)
+                           float *eh, float *ec)
 {
     const int W = P.W, H = P.H;
     float x = P.og_y - (float)sz;
@@ -133,14 +133,8 @@ __device__ int dev_rollout(const FrPlanDev &P, int sx, int sz, float steer, floa
             break;
         size += 1;
         node_cost += (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
-        /* Feasibility only rejects the rollout, so a test may run later (in
-         * the wave pass) without changing the result. */
-        if (!dev_fp_certified(P, cx, cz, heading)) {
-            if (defer && *n_defer < FR_CHK_STRIDE)
-                defer[(*n_defer)++] = make_int2(cz * W + cx, __float_as_int(heading));
-            else if (!dev_feasible(P, cx, cz, heading))
-                return 0;
-        }
+        if (!dev_fp_certified(P, cx, cz, heading) && !dev_feasible(P, cx, cz, heading))
+            return 0;
         last_x = cx;
         last_z = cz;
     }
@@ -1206,7 +1200,6 @@ __global__ void __launch_bounds__(FR_BLOCK) k_select(const FrPlanDev *plans, FrC
             e.end = -1;
             e.heading = 0;
             e.cost = 0;
-            e.n_chk = 0;
             q[w++] = e;
         }
         if (lane == 0) {
@@ -1422,7 +1415,6 @@ __global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *__restrict
             continue;
         FrExp &o = q[i];
         o.end = -1;
-        o.n_chk = 0;
         if (!ok || size < 1)
             continue;
         if ((float)last_x == P.og_x && (float)last_z == P.og_y)
@@ -2134,7 +2126,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_connect_emit(const FrPlanDev *plan
 }
 
 /* one wave per (node, flagged cell) pair */
-__global__ void __launch_bounds__(FR_BLOCK) k_connect_eval(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, const FrPair *pairs, long pair_cap,
+__global__ void __launch_bounds__(FR_BLOCK) k_connect_eval(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, FrPair *pairs, long pair_cap,
                                const int *counters)
 {
     long n = counters[FR_CNT_PAIRS];
@@ -2153,9 +2145,11 @@ __global__ void __launch_bounds__(FR_BLOCK) k_connect_eval(const FrPlanDev *plan
         if ((long)f.pair_base + f.pair_cnt <= p || p < f.pair_base)
             continue; /* a pair slot past the capacity check */
         const FrPlanDev &P = plans[f.slot];
-        const long long lc = dev_pair_lcost(P, f.pos, pr.cell, nullptr, true);
+        float c;
+        const long long lc = dev_pair_lcost(P, f.pos, pr.cell, &c, true);
         if (lc >= FR_BIG_COST || dev_lane() != 0)
             continue;
+        pairs[p].cost = c; /* read back by k_finish for the winning pair */
         atomicMin(&f.key, dev_node_key(lc, (int)(p - f.pair_base)));
         atomicMin((unsigned long long *)&ctl[f.slot].best_lcost, fr_lcost_key(lc));
     }
@@ -2253,7 +2247,7 @@ __device__ int dev_wave_window_cell(const FrPlanDev &P, int node, int rank)
  * goal test, iteration bookkeeping.  One wave per plan: the lanes share the
  * winner's Hermite check, lane 0 writes. */
 __global__ void __launch_bounds__(FR_BLOCK) k_finish(const FrPlanDev *plans, FrCtl *ctl, const int *slots, const int *n_slots_p,
-                         const FrFresh *fq)
+                         const FrFresh *fq, const FrPair *pairs)
 {
     const int i = blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
     const int n_slots = *n_slots_p;
@@ -2274,11 +2268,18 @@ __global__ void __launch_bounds__(FR_BLOCK) k_finish(const FrPlanDev *plans, FrC
         npos = f.pos;
         nz = f.pos / P.W;
         nx = f.pos - nz * P.W;
-        cell = dev_wave_window_cell(P, f.pos, dev_key_rank(f.key));
-        if (cell >= 0)
-            (void)dev_pair_lcost(P, f.pos, cell, &c_cost, true);
-        else
-            bk = ~0ull; /* unreachable: the rank counts flags present since the pair pass */
+        if (!(f.flags & FR_FRESH_INLINE)) {
+            /* the winning pair of k_connect_eval, its curve cost kept */
+            const FrPair pr = pairs[f.pair_base + dev_key_rank(f.key)];
+            cell = pr.cell;
+            c_cost = pr.cost;
+        } else {
+            cell = dev_wave_window_cell(P, f.pos, dev_key_rank(f.key));
+            if (cell >= 0)
+                (void)dev_pair_lcost(P, f.pos, cell, &c_cost, true);
+            else
+                bk = ~0ull; /* unreachable: the rank counts flags present since the pair pass */
+        }
     }
     if (!lead)
         return;
@@ -2770,7 +2771,7 @@ void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st)
     fr_prof_end(FR_PROF_CONNECT, st);
     hipLaunchKernelGGL(k_connect_pick, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);
     hipLaunchKernelGGL(k_finish, dim3(fr_grid_for(L.max_slots, FR_BLOCK / 64, 65535)), b, 0, st, L.plans, L.ctl,
-                       L.slots, L.n_slots, L.fresh_q);
+                       L.slots, L.n_slots, L.fresh_q, L.pairs);
 }
 
 void fr_launch_iteration(const FrLaunch &L, hipStream_t st)
