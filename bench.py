@@ -11,11 +11,12 @@ key (seed, stream), so a plan's search is the same in both forms.
   batched (value) - the step's plans submitted together to one Batch and
                     advanced together (one hipGraph replay per loop iteration
                     of every running plan), each with deadline = the budget;
-  plain           - the plans of the first timed step one at a time through
-                    the FastRRT handle (the reference's loop protocol: one
-                    host sync per loop() call), timeout = the budget - 2 ms
-                    (set_plan_data and path_optimize fall outside the
-                    reference's timeout, fastrrt.cpp:91, :110-114).
+  plain           - the plans of the first `--plain-steps` timed steps one at
+                    a time through the FastRRT handle (the reference's loop
+                    protocol: one host sync per loop() call), timeout = the
+                    budget - 2 ms (set_plan_data and path_optimize fall
+                    outside the reference's timeout, fastrrt.cpp:91,
+                    :110-114).
 
 Success is compared plan by plan on the plans both forms ran.  Frames are
 uploaded before the timed region (device-resident inputs).
@@ -70,7 +71,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=96, help="plans per step")
+    ap.add_argument("--batch", type=int, default=168, help="plans per step")
     ap.add_argument("--budget-ms", type=float, default=20.0, help="per-plan latency budget (batched deadline)")
     ap.add_argument("--plain-timeout-ms", type=int, default=-1,
                     help="FastRRT handle timeout of the plain form (-1: budget - 2 ms, so that its p99 stays "
@@ -78,7 +79,7 @@ def parse():
     ap.add_argument("--max-iter", type=int, default=200)
     ap.add_argument("--opt-rounds", type=int, default=20)
     ap.add_argument("--plain-plans", type=int, default=-1, help="plans of the first timed step run plain (-1: all)")
-    ap.add_argument("--plain-steps", type=int, default=1,
+    ap.add_argument("--plain-steps", type=int, default=3,
                     help="timed steps whose plans also run plain (success compared plan by plan)")
     ap.add_argument("--cpu-sample", type=int, default=2, help="plans timed on the CPU oracle (0 = skip)")
     ap.add_argument("--iters-per-poll", type=int, default=1)
