@@ -1216,8 +1216,10 @@ __global__ void __launch_bounds__(FR_BLOCK) k_select(const FrPlanDev *plans, FrC
  * 64 steps at a time: the heading and position recurrences are float sums
  * whose rounding order matters, so they are accumulated step by step (every
  * lane runs the uniform chain and keeps its own step's value); the per-step
- * sincos, the cell of each step and the footprint tests are spread over the
- * lanes.  A step's cell is new iff it differs from the previous step's cell
+ * sincos and the cell of each step are spread over the lanes, and the new
+ * cells' footprints are tested 64 at a time once the walk's geometry is
+ * known (dev_roll_flush).  A step's cell is new iff it differs from the
+ * previous step's cell
  * (the reference's last_x / last_z: a repeated cell never changes it, a new
  * in-grid cell becomes it, a new out-of-grid cell ends the walk).  The walk
  * ends before step k when k > loop_max, or after the step that is out of the
@@ -1815,13 +1817,6 @@ __device__ __forceinline__ void dev_window(const FrPlanDev &P, int x, int z, int
     *z1 = b > P.H ? P.H : b;
     *x0 = c < 0 ? 0 : c;
     *x1 = d > P.W ? P.W : d;
-}
-
-/* the q-th cell (row-major) of a window whose rows are wx cells wide */
-__device__ __forceinline__ int dev_window_at(const FrPlanDev &P, int z0, int x0, int wx, int q)
-{
-    const int r = q / wx;
-    return (z0 + r) * P.W + x0 + (q - r * wx);
 }
 
 /* A window by 32-cell words of a bit plane: row r of the window covers the
