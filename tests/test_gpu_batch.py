@@ -97,3 +97,33 @@ def test_deadline_retires_plans():
     for r in res:
         assert r.status in (2, 3)
         assert r.latency_ms < 30.0 + 200.0  # retired at the first poll past the deadline
+
+
+def test_batch_smart_regions_equals_oracle():
+    """Smart expansion with 32 px density regions in the batch engine (the
+    request's region side) gives the oracle's search and path."""
+    prm = Params(min_dist=(10, 16), max_path=30.0, dist_tol=12.0)
+    for i in range(3):
+        sc = _scene(10 + i)
+        frame = P.SearchFrame(sc.frame, sc.class_costs)
+        b = Batch(max_plans=1, max_cells=sc.width * sc.height)
+        spec = PlanSpec(frame, sc.start, sc.goal, min_dist=(10, 16), max_path_size=30.0, dist_to_goal_tolerance=12.0,
+                        max_iterations=40, optimize_rounds=2, seed=778, stream=i, tag=i, smart=True, region_side=32)
+        b.submit(spec)
+        (r,) = b.drain()
+        o = OraclePlanner(sc, prm, 778, i, region_side=32)
+        o.search_init()
+        it = 0
+        while not o.goal_reached() and o.loop(True):
+            it += 1
+            if it >= 40:
+                break
+        if o.goal_reached():
+            k = 0
+            while k < 2 and o.path_optimize():
+                k += 1
+            po = o.path()
+            assert r.status == (2 if po.shape[0] else 4)
+            assert r.path.shape == po.shape and (r.path.view(np.int32) == po.view(np.int32)).all(), i
+        else:
+            assert r.status == 3
