@@ -55,6 +55,8 @@ SIGNATURES = {
     "cudagraph_initialize": (c_void_p, [c_int, c_int, c_float, c_float, c_float, c_float, c_int, c_int, c_int, c_int,
                                         c_int, c_int, fp, c_float]),
     "cudagraph_destroy": (None, [c_void_p]),
+    "compute_apf_repulsion": (None, [c_void_p, c_void_p, c_float, c_int]),
+    "compute_apf_attraction": (None, [c_void_p, c_void_p, c_float, c_int, c_int]),
     "get_intrinsic_costs": (fp, [c_void_p]),
     "destroy_intrinsic_costs_ptr": (None, [fp]),
     "add": (None, [c_void_p, c_int, c_int, c_float, c_int, c_int, c_float]),

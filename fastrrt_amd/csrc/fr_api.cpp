@@ -157,6 +157,10 @@ class FrGraph {
             (void)hipFree(d_region);
         if (d_graph_costs)
             (void)hipFree(d_graph_costs);
+        if (d_apf_bits_)
+            (void)hipFree(d_apf_bits_);
+        if (d_apf_tab_)
+            (void)hipFree(d_apf_tab_);
     }
 
     int W() const { return prm.W; }
@@ -618,6 +622,68 @@ class FrGraph {
         return h;
     }
 
+    /* apf.cu:123-155.  The graph's own class table and ego box are used, as
+     * in the reference (_classCosts, _searchSpaceParams). */
+    FrApfArgs apf_args(const FrFrame *frame)
+    {
+        if (!frame)
+            throw std::runtime_error("null frame");
+        if (frame->W != prm.W || frame->H != prm.H)
+            throw std::runtime_error("frame size differs from the grid size");
+        if (!d_graph_costs)
+            throw std::runtime_error("class costs are not set");
+        FrApfArgs a;
+        a.W = prm.W;
+        a.H = prm.H;
+        a.xyz = frame->d_xyz;
+        a.costs = d_graph_costs;
+        a.n_classes = (int)graph_costs.size();
+        a.lbx = prm.lbx;
+        a.lbz = prm.lbz;
+        a.ubx = prm.ubx;
+        a.ubz = prm.ubz;
+        a.intrinsic = slab.intrinsic;
+        return a;
+    }
+
+    void apf_repulsion(const FrFrame *frame, float kr, int radius)
+    {
+        const FrApfArgs a = apf_args(frame);
+        const int w = fr_apf_window(radius, prm.W, prm.H);
+        if (w > FR_APF_MAX_WINDOW)
+            throw std::runtime_error("repulsion window beyond 2048 px (radius and grid both larger)");
+        const size_t words = (size_t)prm.H * ((prm.W + 63) / 64);
+        const size_t tab = (size_t)(w + 1) * (w + 1);
+        if (words > apf_words_ || tab > apf_tab_) {
+            eng.sync();
+            if (d_apf_bits_)
+                (void)hipFree(d_apf_bits_);
+            if (d_apf_tab_)
+                (void)hipFree(d_apf_tab_);
+            d_apf_bits_ = nullptr;
+            d_apf_tab_ = nullptr;
+            apf_words_ = apf_tab_ = 0;
+            FR_CHECK(hipMalloc(&d_apf_bits_, 8 * words));
+            FR_CHECK(hipMalloc(&d_apf_tab_, 4 * tab + 4 * (size_t)(FR_APF_MAX_WINDOW + 1)));
+            apf_words_ = words;
+            apf_tab_ = tab;
+        }
+        fr_launch_apf_repulsion(a, (float)(kr * 0.5), radius, d_apf_bits_, d_apf_tab_, (int *)(d_apf_tab_ + tab),
+                                st());
+        FR_CHECK(hipGetLastError());
+        eng.sync();
+        dirty();
+    }
+
+    void apf_attraction(const FrFrame *frame, float ka, int gx, int gz)
+    {
+        const FrApfArgs a = apf_args(frame);
+        fr_launch_apf_attraction(a, (float)(ka * 0.5), gx, gz, st());
+        FR_CHECK(hipGetLastError());
+        eng.sync();
+        dirty();
+    }
+
     bool gc_flag(int x, int z) { return in(x, z) && read1(slab.gc_flag, x, z) != 0; }
     float gc_cost(int x, int z) { return in(x, z) ? read1(slab.gc_cost, x, z) : 0.0f; }
     float gc_heading(int x, int z) { return in(x, z) ? read1(slab.gc_heading, x, z) : 0.0f; }
@@ -650,6 +716,10 @@ class FrGraph {
     bool gc_pending = false; /* lazy goal-connection planes not fully evaluated */
     int *d_tmp = nullptr;
     uint32_t *d_region = nullptr;
+    /* potential-field scratch: edge bit plane, offset table + disc spans */
+    uint64_t *d_apf_bits_ = nullptr;
+    float *d_apf_tab_ = nullptr;
+    size_t apf_words_ = 0, apf_tab_ = 0;
 };
 
 /* frames carry a generation counter so a re-uploaded frame is re-prepared */
@@ -1099,6 +1169,16 @@ void *cudagraph_initialize(int width, int height, float perception_width_m, floa
 }
 
 void cudagraph_destroy(void *ptr) { FR_TRY_VOID({ delete static_cast<FrGraph *>(ptr); }) }
+
+void compute_apf_repulsion(void *ptr, void *frame, float kr, int radius)
+{
+    FR_TRY_VOID({ G(ptr)->apf_repulsion(F(frame), kr, radius); })
+}
+
+void compute_apf_attraction(void *ptr, void *frame, float ka, int goal_x, int goal_z)
+{
+    FR_TRY_VOID({ G(ptr)->apf_attraction(F(frame), ka, goal_x, goal_z); })
+}
 
 float *get_intrinsic_costs(void *ptr)
 {

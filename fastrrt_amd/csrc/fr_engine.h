@@ -87,6 +87,23 @@ void fr_launch_accept(const FrLaunch &L, int slot, int N, hipStream_t st);
 void fr_launch_solve_collisions(const FrLaunch &L, int slot, int N, int nodes, hipStream_t st);
 void fr_launch_region_count(const FrLaunch &L, int slot, int N, uint32_t *d_out, hipStream_t st);
 
+/* Potential-field intrinsic costs of one grid (fr_apf.hip). */
+struct FrApfArgs {
+    int W, H;
+    const float *xyz;       /* frame, three floats per pixel (class id first) */
+    const float *costs;     /* class cost table */
+    int n_classes;
+    int lbx, lbz, ubx, ubz; /* ego box */
+    float *intrinsic;       /* W*H, updated in place */
+};
+#define FR_APF_MAX_WINDOW 2048
+/* half-width of the repulsion window actually read: min(radius, max(W, H) - 1), 0 if radius < 1 */
+int fr_apf_window(int radius, int W, int H);
+/* scratch: d_bits H*ceil(W/64) words, d_tab (w+1)^2 floats, d_span w+1 ints, w = fr_apf_window */
+void fr_launch_apf_repulsion(const FrApfArgs &a, float kr_half, int radius, uint64_t *d_bits, float *d_tab,
+                             int *d_span, hipStream_t st);
+void fr_launch_apf_attraction(const FrApfArgs &a, float ka_half, int gx, int gz, hipStream_t st);
+
 /* Optional per-kernel timing with HIP events recorded on the launching
  * stream around the kernels named below. */
 enum FrProfKernel { FR_PROF_GOAL_CONNECTION = 0, FR_PROF_EXPAND = 1, FR_PROF_CONNECT = 2, FR_PROF_ITERATION = 3,

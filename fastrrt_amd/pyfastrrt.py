@@ -304,6 +304,16 @@ class CudaGraph:
         except Exception:
             pass
 
+    def compute_apf_repulsion(self, frame: SearchFrame, kr: float, radius: int) -> None:
+        """Potential-field repulsion into the intrinsic costs (cuda_graph.py:276-278)."""
+        _lib.load().compute_apf_repulsion(self._ptr, frame.get_cuda_ptr(), kr, radius)
+        _lib.check("compute_apf_repulsion")
+
+    def compute_apf_attraction(self, frame: SearchFrame, ka: float, goal_x: int, goal_z: int) -> None:
+        """Potential-field attraction into the intrinsic costs (cuda_graph.py:280-282)."""
+        _lib.load().compute_apf_attraction(self._ptr, frame.get_cuda_ptr(), ka, goal_x, goal_z)
+        _lib.check("compute_apf_attraction")
+
     def get_intrinsic_costs(self) -> np.ndarray:
         L = _lib.load()
         p = L.get_intrinsic_costs(self._ptr)

@@ -117,6 +117,13 @@ void *cudagraph_initialize(int width, int height, float perception_width_m, floa
                            int upper_bound_z, float *segmentation_class_cost, float max_curvature);
 /* :35-39 */
 void cudagraph_destroy(void *ptr);
+/* :41-46 (apf.cu:23-98,123-138): potential-field repulsion into the intrinsic
+ * costs; every obstacle pixel is set to 100*kr/2 before the sums, each pixel
+ * sums its contributions in ascending source order; a window wider than
+ * 2048 px (radius and grid side both larger) is refused (last error set) */
+void compute_apf_repulsion(void *ptr, void *frame, float kr, int radius);
+/* :48-53 (apf.cu:100-121,140-155): potential-field attraction */
+void compute_apf_attraction(void *ptr, void *frame, float ka, int goal_x, int goal_z);
 /* :55-73: W*H intrinsic costs */
 float *get_intrinsic_costs(void *ptr);
 /* :75-78 */

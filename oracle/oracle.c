@@ -675,6 +675,84 @@ float og_check_direct_connection(og_graph *g, const float *frame, float max_curv
                                            dist_proc, 0);
 }
 
+/* ---------------------------------------------------------------------- */
+/* Potential-field intrinsic costs (src/cuda/apf.cu)                        */
+/* ---------------------------------------------------------------------- */
+
+static int og_in_ego(const og_graph *g, int x, int z)
+{
+    return x >= g->lb_x && x <= g->ub_x && z >= g->ub_z && z <= g->lb_z;
+}
+
+/* apf.cu:17-21: outside the grid counts as an obstacle */
+static int og_apf_obstacle(const og_graph *g, const float *frame, int x, int z)
+{
+    if (!og_in(g, x, z))
+        return 1;
+    return og_is_obstacle_px(g, frame, og_pos(g, x, z));
+}
+
+/* apf.cu:23-98, 123-138.  Every obstacle cell outside the ego box is set to
+ * 100*Kr/2; one with a free (or in-grid-free) 4-neighbour then adds
+ * Kr/2*(1/p - 1/p0)^2 to every other cell within Euclidean distance p0 =
+ * radius that is outside the ego box.  The reference's per-thread set races
+ * with other threads' atomic adds; here all sets come first, then the
+ * sources add in ascending cell order (one of the reference's outcomes). */
+void og_apf_repulsion(og_graph *g, const float *frame, float kr, int radius)
+{
+    const int W = g->W, H = g->H;
+    const float kr_half = (float)(kr * 0.5);
+    for (int z = 0; z < H; ++z)
+        for (int x = 0; x < W; ++x)
+            if (!og_in_ego(g, x, z) && og_is_obstacle_px(g, frame, og_pos(g, x, z)))
+                g->intrinsic[og_pos(g, x, z)] = 100 * kr_half;
+    for (int z = 0; z < H; ++z)
+        for (int x = 0; x < W; ++x) {
+            if (og_in_ego(g, x, z) || !og_is_obstacle_px(g, frame, og_pos(g, x, z)))
+                continue;
+            if (og_apf_obstacle(g, frame, x - 1, z) && og_apf_obstacle(g, frame, x + 1, z) &&
+                og_apf_obstacle(g, frame, x, z - 1) && og_apf_obstacle(g, frame, x, z + 1))
+                continue;
+            const float p0 = (float)radius;
+            for (int h = z - radius; h <= z + radius; ++h) {
+                if (h < 0)
+                    continue;
+                if (h >= H)
+                    break;
+                for (int w = x - radius; w <= x + radius; ++w) {
+                    if (w < 0)
+                        continue;
+                    if (w >= W)
+                        break;
+                    if (og_in_ego(g, w, h) || (w == x && h == z))
+                        continue;
+                    const float p = sqrtf((float)((w - x) * (w - x) + (h - z) * (h - z)));
+                    if (p > p0)
+                        continue;
+                    const float f = 1 / p - 1 / p0;
+                    const float c = kr_half * f * f;
+                    g->intrinsic[og_pos(g, w, h)] += c;
+                }
+            }
+        }
+}
+
+/* apf.cu:100-121, 140-155: every free cell loses Ka/2 * squared distance to
+ * the goal (the ego box is not skipped here). */
+void og_apf_attraction(og_graph *g, const float *frame, float ka, int gx, int gz)
+{
+    const float ka_half = (float)(ka * 0.5);
+    for (int z = 0; z < g->H; ++z)
+        for (int x = 0; x < g->W; ++x) {
+            const long pos = og_pos(g, x, z);
+            if (og_is_obstacle_px(g, frame, pos))
+                continue;
+            const int dx = gx - x, dz = gz - z;
+            const float d = (float)(dx * dx + dz * dz) * ka_half;
+            g->intrinsic[pos] = g->intrinsic[pos] - d;
+        }
+}
+
 /* graph_direct_goal_connection.cu:169-227 */
 void og_process_direct_goal_connection(og_graph *g, const float *frame, int gx, int gz, float gh, float max_curv)
 {

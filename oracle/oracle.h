@@ -13,6 +13,9 @@
  *   - when several expansions end on one cell, the lowest source cell wins;
  *   - subtree erase decides on the pre-erase snapshot, then applies;
  *   - collision re-parenting resets child counts before incrementing parents;
+ *   - potential-field repulsion sets every obstacle cell before any sum
+ *     reaches it, and each cell sums its contributions in ascending source
+ *     cell order;
  *   - both argmin reductions break ties on the lowest cell index (and, inside
  *     one node's goal window, on the last window cell in scan order, which is
  *     what the reference's sequential per-thread scan keeps).
@@ -22,6 +25,8 @@
  *   collision erase, best node, goal test, dump format, Philox: pinned by the
  *   reference's known-answer tests and closed forms.
  *   density gating (smart expansion): pinned by brute-force recount only.
+ *   potential-field costs: pinned by the closed form of one isolated obstacle
+ *   and by brute-force sums in both source orders on small grids.
  */
 #ifndef FR_ORACLE_H
 #define FR_ORACLE_H
@@ -104,6 +109,8 @@ int og_check_goal_reached(og_graph *g, int gx, int gz, float gh, float tol, floa
 void og_find_best_node(og_graph *g, const float *frame, float heading, float radius, int x, int z,
                        float max_herr, int out[2]);
 int og_check_graph_consistent(og_graph *g);
+void og_apf_repulsion(og_graph *g, const float *frame, float kr, int radius);
+void og_apf_attraction(og_graph *g, const float *frame, float ka, int gx, int gz);
 int og_check_feasible_connection(og_graph *g, const float *frame, int sx, int sz, int ex, int ez, float v);
 int og_interpolate_hermite(int W, int H, int x1, int z1, float h1, int x2, int z2, float h2, float *out,
                            int cap);

@@ -91,6 +91,8 @@ def lib():
         "og_check_goal_reached": (c_int, [P, c_int, c_int, c_float, c_float, c_float]),
         "og_find_best_node": (None, [P, fp, c_float, c_float, c_int, c_int, c_float, ip]),
         "og_check_graph_consistent": (c_int, [P]),
+        "og_apf_repulsion": (None, [P, fp, c_float, c_int]),
+        "og_apf_attraction": (None, [P, fp, c_float, c_int, c_int]),
         "og_check_feasible_connection": (c_int, [P, fp, c_int, c_int, c_int, c_int, c_float]),
         "og_interpolate_hermite": (c_int, [c_int, c_int, c_int, c_int, c_float, c_int, c_int, c_float, fp, c_int]),
         "og_dump_graph": (None, [P, ctypes.c_char_p]),
@@ -154,6 +156,14 @@ class Graph:
 
     def add(self, x, z, h, px, pz, cost):
         self.L.og_add(self.g, x, z, h, px, pz, cost)
+
+    def apf_repulsion(self, frame, kr, radius):
+        frame = np.ascontiguousarray(frame, np.float32)
+        self.L.og_apf_repulsion(self.g, fptr(frame), kr, radius)
+
+    def apf_attraction(self, frame, ka, gx, gz):
+        frame = np.ascontiguousarray(frame, np.float32)
+        self.L.og_apf_attraction(self.g, fptr(frame), ka, gx, gz)
 
     def add_temporary(self, x, z, h, px, pz, cost):
         self.L.og_add_temporary(self.g, x, z, h, px, pz, cost)
