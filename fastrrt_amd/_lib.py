@@ -55,6 +55,8 @@ SIGNATURES = {
     "cudagraph_initialize": (c_void_p, [c_int, c_int, c_float, c_float, c_float, c_float, c_int, c_int, c_int, c_int,
                                         c_int, c_int, fp, c_float]),
     "cudagraph_destroy": (None, [c_void_p]),
+    "can_connect_to_goal_using_hermite": (c_bool, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_float]),
+    "cudagraph_can_connect_hermite": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_float, fp]),
     "compute_apf_repulsion": (None, [c_void_p, c_void_p, c_float, c_int]),
     "compute_apf_attraction": (None, [c_void_p, c_void_p, c_float, c_int, c_int]),
     "get_intrinsic_costs": (fp, [c_void_p]),

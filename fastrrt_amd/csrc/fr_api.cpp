@@ -393,6 +393,23 @@ class FrGraph {
         return {h[0], h[1]};
     }
 
+    /* graph_expansion.cu:246-261 (the frame's class costs; the footprint and
+     * ego box are the graph's, as this build's frame handle carries no search
+     * parameters).  first: the goal pixel is an obstacle; second: the
+     * curve's result (cost, -1, -2). */
+    std::pair<bool, float> can_connect_hermite(const FrFrame *frame, int x, int z, int gx, int gz, int goal_heading)
+    {
+        if (!has_physical)
+            throw std::runtime_error("physical parameters are not set");
+        prep_frame(frame, true);
+        fr_launch_hermite_probe(eng.launch(), 0, x, z, gx, gz, goal_heading, (float)prm.max_steer_rad,
+                                (float *)d_tmp, st());
+        float h[2];
+        FR_CHECK(hipMemcpyAsync(h, d_tmp, 2 * sizeof(float), hipMemcpyDeviceToHost, st()));
+        eng.sync();
+        return {h[1] != 0.0f, h[0]};
+    }
+
     /* graph_direct_goal_connection.cu:200-227 (planes from the frame's costs).
      * lazy: a cell's planes are evaluated when a node's connection window
      * first covers it (same values: the evaluation is a function of the
@@ -1242,6 +1259,28 @@ float *list_all(void *ptr)
 }
 
 void free_list_all(float *p) { delete[] p; }
+
+/* The reference converts the curve's float to bool: a failed curve (-1, -2)
+ * answers true, only a zero cost or a blocked goal answer false. */
+bool can_connect_to_goal_using_hermite(void *ptr, void *frame, int x, int z, int goal_x, int goal_z,
+                                       float goal_heading)
+{
+    FR_TRY(false, {
+        const auto r = G(ptr)->can_connect_hermite(F(frame), x, z, goal_x, goal_z, host_trunc((double)goal_heading));
+        return !r.first && r.second != 0.0f;
+    })
+}
+
+int cudagraph_can_connect_hermite(void *ptr, void *frame, int x, int z, int goal_x, int goal_z, float goal_heading,
+                                  float *result)
+{
+    FR_TRY(-1, {
+        const auto r = G(ptr)->can_connect_hermite(F(frame), x, z, goal_x, goal_z, host_trunc((double)goal_heading));
+        if (result)
+            *result = r.second;
+        return r.first ? 1 : 0;
+    })
+}
 
 void solve_collisions(void *ptr) { FR_TRY_VOID({ G(ptr)->solve_collisions(); }) }
 

@@ -83,6 +83,9 @@ void fr_launch_set_collision(const FrLaunch &L, int slot, int x, int z, int npx,
                              hipStream_t st);
 void fr_launch_derivate(const FrLaunch &L, int slot, float steer, float path_size, float v, int x, int z,
                         int *d_out, hipStream_t st);
+/* CudaGraph::canConnectToGoal: d_out[0] = curve result, d_out[1] = goal pixel blocked */
+void fr_launch_hermite_probe(const FrLaunch &L, int slot, int x, int z, int gx, int gz, int goal_heading,
+                             float max_curv, float *d_out, hipStream_t st);
 void fr_launch_accept(const FrLaunch &L, int slot, int N, hipStream_t st);
 void fr_launch_solve_collisions(const FrLaunch &L, int slot, int N, int nodes, hipStream_t st);
 void fr_launch_region_count(const FrLaunch &L, int slot, int N, uint32_t *d_out, hipStream_t st);

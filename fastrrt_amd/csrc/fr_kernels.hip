@@ -3002,6 +3002,102 @@ __global__ void __launch_bounds__(FR_BLOCK) k_op_set_collision(const FrPlanDev *
         P.derive[cpp]--;
 }
 
+/* CudaGraph::canConnectToGoal (graph_expansion.cu:246-261), a host call in
+ * the reference, over cudaHermite (graph_hermite_connection.cu:11-136) with
+ * fillInterpolation and computeDistance false: the curve length is the
+ * frame's distance-to-goal channel, the walk starts from the node's own cell,
+ * the curvature is |x'z'' - z'x''| / v (pow(v, 3/2) with integer 3/2 = 1)
+ * against the bound, failing with -2; no endpoint test.  out[0] = the
+ * curve's result (cost, -1 or -2), out[1] = 1 when the goal pixel is an
+ * obstacle or outside the grid (the reference answers false before any
+ * curve).  One thread: a test hook, not a loop phase. */
+__global__ void __launch_bounds__(64) k_op_hermite(const FrPlanDev *plans, int slot, int x, int z, int gx, int gz,
+                                                   int goal_heading, float max_curv, float *out)
+{
+    if (threadIdx.x != 0)
+        return;
+    const FrPlanDev &P = plans[slot];
+    const int W = P.W, H = P.H;
+    out[0] = -1;
+    out[1] = 0;
+    if (gx < 0 || gx >= W || gz < 0 || gz >= H) {
+        out[1] = 1;
+        return;
+    }
+    {
+        const int cls = fr_trunc_f(P.frame[3 * ((long)gz * W + gx)]);
+        if (cls < 0 || cls >= P.n_classes || P.class_costs[cls] < 0) {
+            out[1] = 1;
+            return;
+        }
+    }
+    if (x < 0 || x >= W || z < 0 || z >= H)
+        return;
+    const int pos = z * W + x;
+    const float distance = P.frame[3 * (long)pos + 1];
+    const int num_points = fr_host_trunc_d((double)distance);
+    const float a1 = (float)((double)P.heading[pos] - FR_HALF_PI_D);
+    const float a2 = (float)((double)goal_heading - FR_HALF_PI_D);
+    float s1, c1, s2, c2;
+    fr_sincosf(a1, &s1, &c1);
+    fr_sincosf(a2, &s2, &c2);
+    const double t1x = (double)(distance * c1), t1y = (double)(distance * s1);
+    const double t2x = (double)(distance * c2), t2y = (double)(distance * s2);
+    const double dx0 = (double)x, dz0 = (double)z, dgx = (double)gx, dgz = (double)gz;
+    int last_x = x, last_z = z;
+    float node_cost = P.cost[pos];
+    for (int i = 0; i < num_points; ++i) {
+        const double t = (0.0 + (double)i) / (double)(num_points - 1);
+        const double t2 = t * t, t3 = t2 * t;
+        const double h00 = 2 * t3 - 3 * t2 + 1;
+        const double h10 = t3 - 2 * t2 + t;
+        const double h01 = -2 * t3 + 3 * t2;
+        const double h11 = t3 - t2;
+        const double px = h00 * dx0 + h10 * t1x + h01 * dgx + h11 * t2x;
+        const double pz = h00 * dz0 + h10 * t1y + h01 * dgz + h11 * t2y;
+        if (px < 0 || px >= W)
+            continue;
+        if (pz < 0 || pz >= H)
+            continue;
+        const int cx = fr_host_trunc_d(px);
+        const int cz = fr_host_trunc_d(pz);
+        if (cx == last_x && cz == last_z)
+            continue;
+        if (cx < 0 || cx >= W)
+            continue;
+        if (cz < 0 || cz >= H)
+            continue;
+        node_cost += (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
+        const double t00 = 6 * t2 - 6 * t;
+        const double t10 = 3 * t2 - 4 * t + 1;
+        const double t01 = -6 * t2 + 6 * t;
+        const double t11 = 3 * t2 - 2 * t;
+        const double ddx = t00 * dx0 + t10 * t1x + t01 * dgx + t11 * t2x;
+        const double ddz = t00 * dz0 + t10 * t1y + t01 * dgz + t11 * t2y;
+        const float heading = (float)((double)fr_atan2f((float)ddz, (float)ddx) + FR_HALF_PI_D);
+        if (max_curv > 0) {
+            const double d00 = 12 * t - 6;
+            const double d10 = 6 * t - 4;
+            const double d01 = -12 * t + 6;
+            const double d11 = 6 * t - 2;
+            const double dd2x = d00 * dx0 + d10 * t1x + d01 * dgx + d11 * t2x;
+            const double dd2z = d00 * dz0 + d10 * t1y + d01 * dgz + d11 * t2y;
+            const double v = ddx * ddx + ddz * ddz;
+            const float k = (float)(fr_fabs_d(ddx * dd2z - ddz * dd2x) / v);
+            if (k > max_curv) {
+                out[0] = -2;
+                return;
+            }
+        }
+        last_x = cx;
+        last_z = cz;
+        if (!dev_feasible(P, cx, cz, heading))
+            return;
+    }
+    if (num_points > 0)
+        out[0] = node_cost;
+}
+
 /* CudaGraph::derivateNode -> expand_node, host sequential semantics
  * (graph_expansion.cu:103-145,224-244).  out = end cell or (-1, -1). */
 __global__ void __launch_bounds__(FR_BLOCK) k_op_derivate(const FrPlanDev *plans, int slot, float steer, float path_size, float v, int x, int z,
@@ -3135,6 +3231,13 @@ void fr_launch_derivate(const FrLaunch &L, int slot, float steer, float path_siz
                         hipStream_t st)
 {
     hipLaunchKernelGGL(k_op_derivate, dim3(1), dim3(1), 0, st, L.plans, slot, steer, path_size, v, x, z, d_out);
+}
+
+void fr_launch_hermite_probe(const FrLaunch &L, int slot, int x, int z, int gx, int gz, int goal_heading,
+                             float max_curv, float *d_out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_op_hermite, dim3(1), dim3(64), 0, st, L.plans, slot, x, z, gx, gz, goal_heading, max_curv,
+                       d_out);
 }
 
 void fr_launch_accept(const FrLaunch &L, int slot, int N, hipStream_t st)

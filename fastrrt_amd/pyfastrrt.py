@@ -304,6 +304,25 @@ class CudaGraph:
         except Exception:
             pass
 
+    def can_connect_to_goal_using_hermite(self, frame: SearchFrame, x: int, z: int, goal_x: int, goal_z: int,
+                                          goal_heading: float) -> bool:
+        """cuda_graph.py's binding of python_cuda_graph_bind.cpp:147-153: the
+        curve length comes from the frame's distance-to-goal channel."""
+        r = _lib.load().can_connect_to_goal_using_hermite(self._ptr, frame.get_cuda_ptr(), x, z, goal_x, goal_z,
+                                                          goal_heading)
+        _lib.check("can_connect_to_goal_using_hermite")
+        return bool(r)
+
+    def can_connect_hermite_value(self, frame: SearchFrame, x: int, z: int, goal_x: int, goal_z: int,
+                                  goal_heading: float):
+        """(goal pixel blocked, curve result: cost, -1 or -2)."""
+        out = ctypes.c_float(0.0)
+        r = _lib.load().cudagraph_can_connect_hermite(self._ptr, frame.get_cuda_ptr(), x, z, goal_x, goal_z,
+                                                      goal_heading, ctypes.byref(out))
+        if r < 0:
+            _lib.check("cudagraph_can_connect_hermite")
+        return (r == 1, out.value)
+
     def compute_apf_repulsion(self, frame: SearchFrame, kr: float, radius: int) -> None:
         """Potential-field repulsion into the intrinsic costs (cuda_graph.py:276-278)."""
         _lib.load().compute_apf_repulsion(self._ptr, frame.get_cuda_ptr(), kr, radius)

@@ -146,6 +146,14 @@ void clear(void *ptr);
 float *list_all(void *ptr);
 /* :142-145 */
 void free_list_all(float *p);
+/* :147-153 (graph_expansion.cu:246-261, graph_hermite_connection.cu:11-136):
+ * the Hermite curve from node (x, z) to the goal, its length read from the
+ * frame's distance-to-goal channel (pixel channel 1), goal_heading truncated
+ * to an int as at the reference's call; false when the goal pixel is an
+ * obstacle, otherwise the curve's float result converted to bool as the
+ * reference does (a failed curve, -1 or -2, answers true) */
+bool can_connect_to_goal_using_hermite(void *ptr, void *frame, int x, int z, int goal_x, int goal_z,
+                                       float goal_heading);
 /* :155-159 */
 void solve_collisions(void *ptr);
 /* :161-167 */
@@ -183,6 +191,11 @@ void cudagraph_find_best_node(void *ptr, void *frame, float heading, float radiu
                               float max_heading_error, int *out);
 bool cudagraph_check_goal_reached(void *ptr, int goal_x, int goal_z, float heading, float tolerance,
                                   float max_heading_error);
+/* canConnectToGoal with the curve's value (graph_hermite_connection.cu:11-136):
+ * returns 1 when the goal pixel is an obstacle (no curve), 0 otherwise with
+ * *result = the cost, -1 (collision or no points) or -2 (curvature); -1 on error */
+int cudagraph_can_connect_hermite(void *ptr, void *frame, int x, int z, int goal_x, int goal_z, float goal_heading,
+                                  float *result);
 /* derivateNode returning the end cell (graph.h:177); out = (x, z) */
 void cudagraph_derivate_node(void *ptr, void *frame, float steering_angle, float path_size, float velocity, int x,
                              int z, int *out);

@@ -675,6 +675,91 @@ float og_check_direct_connection(og_graph *g, const float *frame, float max_curv
                                            dist_proc, 0);
 }
 
+/* graph_hermite_connection.cu:11-136 (cudaHermite with fillInterpolation and
+ * computeDistance false) behind CudaGraph::canConnectToGoal
+ * (graph_expansion.cu:246-261), which runs on the host: host truncation.
+ * The curve length is the frame's distance-to-goal channel; the walk starts
+ * from the node's own cell (its first point is skipped), the curvature test
+ * divides by pow(v, 3/2) with integer 3/2 = 1, i.e. by v, and fails with -2;
+ * there is no endpoint test.  goal_heading is an int at the reference's call.
+ * Returns the cost, -1 (collision, no points, start outside the grid) or
+ * -2 (curvature); *goal_blocked is set when the goal pixel is an obstacle
+ * (or outside the grid), where canConnectToGoal answers false before any
+ * curve is walked (the curve's float result converts to bool otherwise). */
+float og_can_connect_hermite(og_graph *g, const float *frame, float max_curv, int x, int z, int gx, int gz,
+                             int goal_heading, int *goal_blocked)
+{
+    const int W = g->W, H = g->H;
+    *goal_blocked = !og_in(g, gx, gz) || og_is_obstacle_px(g, frame, og_pos(g, gx, gz));
+    if (*goal_blocked)
+        return -1;
+    if (!og_in(g, x, z))
+        return -1;
+    g->n_hermite_walks++;
+    const long pos = og_pos(g, x, z);
+    const float distance = frame[3 * pos + 1];
+    const int num_points = og_host_trunc_d((double)distance);
+    const float local_heading = g->heading[pos];
+    const float a1 = (float)((double)local_heading - FR_HALF_PI_D);
+    const float a2 = (float)((double)goal_heading - FR_HALF_PI_D);
+    float s1, c1, s2, c2;
+    fr_sincosf(a1, &s1, &c1);
+    fr_sincosf(a2, &s2, &c2);
+    const float t1x = distance * c1, t1y = distance * s1;
+    const float t2x = distance * c2, t2y = distance * s2;
+    int last_x = x, last_z = z;
+    float node_cost = g->cost[pos];
+    for (int i = 0; i < num_points; ++i) {
+        const double t = (0.0 + (double)i) / (double)(num_points - 1);
+        const double t2 = t * t, t3 = t2 * t;
+        const double h00 = 2 * t3 - 3 * t2 + 1;
+        const double h10 = t3 - 2 * t2 + t;
+        const double h01 = -2 * t3 + 3 * t2;
+        const double h11 = t3 - t2;
+        const double px = h00 * x + h10 * (double)t1x + h01 * gx + h11 * (double)t2x;
+        const double pz = h00 * z + h10 * (double)t1y + h01 * gz + h11 * (double)t2y;
+        if (px < 0 || px >= W)
+            continue;
+        if (pz < 0 || pz >= H)
+            continue;
+        const int cx = og_host_trunc_d(px);
+        const int cz = og_host_trunc_d(pz);
+        if (cx == last_x && cz == last_z)
+            continue;
+        if (cx < 0 || cx >= W)
+            continue;
+        if (cz < 0 || cz >= H)
+            continue;
+        node_cost += g->intrinsic[(long)cz * W + cx] + 1.0f;
+        const double t00 = 6 * t2 - 6 * t;
+        const double t10 = 3 * t2 - 4 * t + 1;
+        const double t01 = -6 * t2 + 6 * t;
+        const double t11 = 3 * t2 - 2 * t;
+        const double ddx = t00 * x + t10 * (double)t1x + t01 * gx + t11 * (double)t2x;
+        const double ddz = t00 * z + t10 * (double)t1y + t01 * gz + t11 * (double)t2y;
+        const float heading = (float)((double)fr_atan2f((float)ddz, (float)ddx) + FR_HALF_PI_D);
+        const double d00 = 12 * t - 6;
+        const double d10 = 6 * t - 4;
+        const double d01 = -12 * t + 6;
+        const double d11 = 6 * t - 2;
+        const double dd2x = d00 * x + d10 * (double)t1x + d01 * gx + d11 * (double)t2x;
+        const double dd2z = d00 * z + d10 * (double)t1y + d01 * gz + d11 * (double)t2y;
+        if (max_curv > 0) {
+            const double v = ddx * ddx + ddz * ddz;
+            const float k = (float)(fr_fabs_d(ddx * dd2z - ddz * dd2x) / v);
+            if (k > max_curv)
+                return -2;
+        }
+        last_x = cx;
+        last_z = cz;
+        if (!og_feasible(g, frame, last_x, last_z, heading))
+            return -1;
+    }
+    if (num_points <= 0)
+        return -1;
+    return node_cost;
+}
+
 /* ---------------------------------------------------------------------- */
 /* Potential-field intrinsic costs (src/cuda/apf.cu)                        */
 /* ---------------------------------------------------------------------- */

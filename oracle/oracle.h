@@ -109,6 +109,8 @@ int og_check_goal_reached(og_graph *g, int gx, int gz, float gh, float tol, floa
 void og_find_best_node(og_graph *g, const float *frame, float heading, float radius, int x, int z,
                        float max_herr, int out[2]);
 int og_check_graph_consistent(og_graph *g);
+float og_can_connect_hermite(og_graph *g, const float *frame, float max_curv, int x, int z, int gx, int gz,
+                             int goal_heading, int *goal_blocked);
 void og_apf_repulsion(og_graph *g, const float *frame, float kr, int radius);
 void og_apf_attraction(og_graph *g, const float *frame, float ka, int gx, int gz);
 int og_check_feasible_connection(og_graph *g, const float *frame, int sx, int sz, int ex, int ez, float v);

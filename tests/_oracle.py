@@ -91,6 +91,7 @@ def lib():
         "og_check_goal_reached": (c_int, [P, c_int, c_int, c_float, c_float, c_float]),
         "og_find_best_node": (None, [P, fp, c_float, c_float, c_int, c_int, c_float, ip]),
         "og_check_graph_consistent": (c_int, [P]),
+        "og_can_connect_hermite": (c_float, [P, fp, c_float, c_int, c_int, c_int, c_int, c_int, ip]),
         "og_apf_repulsion": (None, [P, fp, c_float, c_int]),
         "og_apf_attraction": (None, [P, fp, c_float, c_int, c_int]),
         "og_check_feasible_connection": (c_int, [P, fp, c_int, c_int, c_int, c_int, c_float]),

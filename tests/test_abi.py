@@ -27,9 +27,9 @@ def test_header_declares_the_reference_abi():
                "path_optimize", "export_graph_nodes", "release_export_graph_nodes", "get_planned_path",
                "interpolate_planned_path", "release_planned_path_data", "interpolate_planned_path_p",
                "ideal_curve", "compute_region_debug_performance", "save_current_graph_state", "load_graph_state"}
-    # src/python_cuda_graph_bind.cpp:7-220 (the Hermite test hook is outside the planning loop)
+    # src/python_cuda_graph_bind.cpp:7-220, all 25
     graph = {"cudagraph_initialize", "cudagraph_destroy", "get_intrinsic_costs", "destroy_intrinsic_costs_ptr",
-             "compute_apf_repulsion", "compute_apf_attraction",
+             "compute_apf_repulsion", "compute_apf_attraction", "can_connect_to_goal_using_hermite",
              "add", "add_temporary", "derivate_node", "accept_derived_nodes", "check_in_graph", "get_heading",
              "clear", "list_all", "free_list_all", "solve_collisions", "process_direct_goal_connection",
              "is_directly_connected_to_goal", "direct_connection_to_goal_cost", "direct_connection_to_goal_heading",
