@@ -282,6 +282,7 @@ FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
     L_.demand_cap = queue_cap + 1;
     FR_CHECK(hipMalloc(&L_.demand, sizeof(int2) * L_.demand_cap));
     FR_CHECK(hipMalloc(&L_.demand2, sizeof(int2) * L_.demand_cap));
+    FR_CHECK(hipMalloc(&L_.demand2_geo, sizeof(FrGcGeo) * L_.demand_cap));
     FR_CHECK(hipMemsetAsync(L_.ctl, 0, sizeof(FrCtl) * max_slots, st_));
     FR_CHECK(hipMemsetAsync(L_.counters, 0, sizeof(int) * FR_CNT_N, st_));
     L_.grid_tiles = (int)std::min<long>(max_tiles, 2048);
@@ -313,6 +314,7 @@ FrEngine::~FrEngine()
     (void)hipFree(L_.pairs);
     (void)hipFree(L_.demand);
     (void)hipFree(L_.demand2);
+    (void)hipFree(L_.demand2_geo);
     if (own_stream_)
         (void)hipStreamDestroy(st_);
 }

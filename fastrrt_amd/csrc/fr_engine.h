@@ -22,6 +22,13 @@ enum { FR_CNT_EXP = 0, FR_CNT_FRESH = 1, FR_CNT_PAIRS = 2, FR_CNT_DEMAND = 3, FR
  * (lazy), or evaluated for every cell (eager) */
 enum { FR_GOAL_NONE = 0, FR_GOAL_LAZY = 1, FR_GOAL_EAGER = 2 };
 
+/* A goal curve's set-up (tangents, start heading, point count), handed from
+ * the curvature-and-centre pass to the footprint pass. */
+struct FrGcGeo {
+    float t1x, t1y, t2x, t2y, lh;
+    int np;
+};
+
 struct FrLaunch {
     FrPlanDev *plans;
     FrCtl *ctl;
@@ -35,6 +42,7 @@ struct FrLaunch {
     long pair_cap;
     int2 *demand;             /* (slot, cell) goal-connection evaluations queued this iteration */
     int2 *demand2;            /* the queued cells whose goal curves passed curvature and centres */
+    FrGcGeo *demand2_geo;     /* their curves' set-up, beside them */
     long demand_cap;          /* of both */
     int *n_slots;             /* device: entries of `slots` */
     int grid_tiles, grid_items, grid_slots, grid_waves;

@@ -470,13 +470,27 @@ __device__ bool dev_groups_dealt(const FrPlanDev &P, FrOpenSet &b, int m)
 /* One Hermite point of graph_direct_goal_connection.cu:16-167: position,
  * in-bounds cell, first derivative. */
 struct FrHermite {
-    double dx0, dz0, dgx, dgz, t1x, t1y, t2x, t2y, denom;
+    double dx0, dz0, dgx, dgz, t1x, t1y, t2x, t2y, denom, rden;
 };
+
+/* t = i / denom, correctly rounded.  For denom <= 32768 (every curve on grids
+ * up to 23170 px on a side) it is the product with the rounded reciprocal
+ * plus one fused correction, which equals the quotient for every 0 <= i <=
+ * denom + 1 (checked exhaustively); denom = 0 gives NaN as 0 / 0 does. */
+__device__ __forceinline__ double dev_curve_t(const FrHermite &hm, int i)
+{
+    const double a = 0.0 + (double)i;
+    if (hm.denom <= 32768.0) {
+        const double q0 = a * hm.rden;
+        return __builtin_fma(__builtin_fma(-q0, hm.denom, a), hm.rden, q0);
+    }
+    return a / hm.denom;
+}
 
 __device__ __forceinline__ bool dev_hermite_point(const FrPlanDev &P, const FrHermite &hm, int i, bool host_nan,
                                                   int *cx, int *cz, double *t_out)
 {
-    const double t = (0.0 + (double)i) / hm.denom;
+    const double t = dev_curve_t(hm, i);
     const double t2 = t * t, t3 = t2 * t;
     const double h00 = 2 * t3 - 3 * t2 + 1;
     const double h10 = t3 - 2 * t2 + t;
@@ -570,6 +584,7 @@ __device__ __forceinline__ bool dev_curve_setup(int x, int z, float lh, int gx, 
     hm->dgx = (double)gx;
     hm->dgz = (double)gz;
     hm->denom = (double)(num_points - 1);
+    hm->rden = 1.0 / hm->denom;
     return true;
 }
 
@@ -593,6 +608,7 @@ __device__ __forceinline__ FrHermite dev_bcast_hermite(const FrHermite &h, int l
     o.t2x = dev_bcast_d(h.t2x, lane);
     o.t2y = dev_bcast_d(h.t2y, lane);
     o.denom = dev_bcast_d(h.denom, lane);
+    o.rden = dev_bcast_d(h.rden, lane);
     return o;
 }
 
@@ -654,7 +670,9 @@ __device__ __forceinline__ void dev_curve_d1(const FrHermite &hm, double t, doub
 }
 
 /* curvature test of graph_direct_goal_connection.cu (float of the double
- * curvature against max_curv) */
+ * curvature against max_curv).  The float estimate from the hardware
+ * reciprocal square root is within 1e-5 of the exact value, so only values
+ * within 0.1% of the bound (and non-finite ones) take the exact division. */
 __device__ __forceinline__ bool dev_curve_too_sharp(const FrHermite &hm, double t, float max_curv)
 {
     double ddx, ddz;
@@ -667,61 +685,50 @@ __device__ __forceinline__ bool dev_curve_too_sharp(const FrHermite &hm, double 
     const double dd2z = d00 * hm.dz0 + d10 * hm.t1y + d01 * hm.dgz + d11 * hm.t2y;
     const double v = ddx * ddx + ddz * ddz;
     const double cr = ddx * dd2z - ddz * dd2x;
-    const float k = (float)((cr < 0 ? -cr : cr) / (v * __builtin_sqrt(v)));
+    const double acr = cr < 0 ? -cr : cr;
+    const float vf = (float)v;
+    if (vf > 1e-20f && vf < 1e24f) { /* rs^3 stays a normal float */
+        const float rs = __builtin_amdgcn_rsqf(vf);
+        const float kf = (float)acr * (rs * rs * rs);
+        if (kf < max_curv * 0.999f)
+            return false;
+        if (kf > max_curv * 1.001f && kf <= 3.0e38f)
+            return true;
+    }
+    const float k = (float)(acr / (v * __builtin_sqrt(v)));
     return k > max_curv;
 }
 
-/* Pass 0 over FR_P0_CHUNKS chunks of 64 points at a time: the geometry and
- * curvature of every point first, then the centre cells' bytes, all loads
- * of the chunks in flight together (the pass is bound by their latency). */
-#define FR_P0_CHUNKS 4
-
+/* Pass 0, chunk by chunk of 64 points: each chunk's geometry and curvature,
+ * then its centre cells' loads; the previous chunk's bytes are tested while
+ * these are in flight.  Most rejected curves meet their first obstacle pixel
+ * within the first chunk, so the walk usually ends there. */
 __device__ __forceinline__ bool dev_wave_curve_pass0(const FrPlanDev &P, const FrHermite &hm, int num_points,
                                                      float max_curv, bool skip_fp, bool host_nan, int &last_x,
                                                      int &last_z)
 {
     last_x = -1;
     last_z = -1;
-    for (int b0 = 0; b0 < num_points; b0 += 64 * FR_P0_CHUNKS) {
-        int cxs[FR_P0_CHUNKS], czs[FR_P0_CHUNKS];
-        bool accs[FR_P0_CHUNKS];
-        bool bad = false;
-#pragma unroll
-        for (int u = 0; u < FR_P0_CHUNKS; ++u) {
-            const int b = b0 + 64 * u;
-            accs[u] = false;
-            cxs[u] = 0;
-            czs[u] = 0;
-            if (b < num_points) {
-                double t;
-                unsigned long long am;
-                accs[u] = dev_curve_point(P, hm, b, num_points, host_nan, last_x, last_z, &cxs[u], &czs[u], &t, &am);
-                if (accs[u] && max_curv > 0 && dev_curve_too_sharp(hm, t, max_curv))
-                    bad = true;
-            }
-        }
+    int tv_prev = 1, dv_prev = 1;
+    for (int b = 0; b < num_points; b += 64) {
+        double t;
+        unsigned long long am;
+        int cx, cz;
+        const bool acc = dev_curve_point(P, hm, b, num_points, host_nan, last_x, last_z, &cx, &cz, &t, &am);
+        const bool bad = acc && max_curv > 0 && dev_curve_too_sharp(hm, t, max_curv);
         if (__ballot(bad))
             return false;
-        if (skip_fp)
-            continue;
-        int tv[FR_P0_CHUNKS], dv[FR_P0_CHUNKS];
-#pragma unroll
-        for (int u = 0; u < FR_P0_CHUNKS; ++u) {
-            tv[u] = 1;
-            dv[u] = 1;
-            if (accs[u]) {
-                tv[u] = P.trav[czs[u] * P.W + cxs[u]];
-                dv[u] = P.dist[fr_dist_at(P.TW, cxs[u] + P.pad, czs[u] + P.pad)];
-            }
+        int tv = 1, dv = 1;
+        if (acc && !skip_fp) {
+            tv = P.trav[cz * P.W + cx];
+            dv = P.dist[fr_dist_at(P.TW, cx + P.pad, cz + P.pad)];
         }
-#pragma unroll
-        for (int u = 0; u < FR_P0_CHUNKS; ++u)
-            if (!(tv[u] & 1) && dv[u] == 0)
-                bad = true;
-        if (__ballot(bad))
+        if (__ballot(!(tv_prev & 1) && dv_prev == 0))
             return false;
+        tv_prev = tv;
+        dv_prev = dv;
     }
-    return true;
+    return !__ballot(!(tv_prev & 1) && dv_prev == 0);
 }
 
 /* The passes of dev_wave_check_direct over a set-up curve. */
@@ -1951,10 +1958,11 @@ __device__ __forceinline__ bool dev_gc_lane_setup(const FrPlanDev *plans, int2 d
 }
 
 /* Curvature and centre cells of the queued cells' goal curves; a rejection
- * is stored, the other cells are queued for k_gc_foot (at most as many as
- * k_gc_eval reads, so dq2 has dq's capacity). */
+ * is stored, the other cells are queued for k_gc_foot with their curves'
+ * set-up (at most as many as k_gc_eval reads, so dq2 has dq's capacity). */
 __global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *__restrict__ plans, const int2 *__restrict__ dq,
-                                                      long dq_cap, int2 *__restrict__ dq2, int *__restrict__ counters)
+                                                      long dq_cap, int2 *__restrict__ dq2,
+                                                      FrGcGeo *__restrict__ geo2, int *__restrict__ counters)
 {
     long n = counters[FR_CNT_DEMAND];
     if (n > dq_cap)
@@ -1974,6 +1982,12 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *__restric
         if (lane < cnt) {
             d = dq[b0 + lane];
             ok = dev_gc_lane_setup(plans, d, false, &hm, &np, &lh, &base);
+            /* the curve's first point is the cell itself: an obstacle pixel
+             * there rejects it without a walk */
+            const FrPlanDev &P = plans[d.x];
+            const int z = d.y / P.W, x = d.y - z * P.W;
+            if (ok && !(P.trav[d.y] & 1) && P.dist[fr_dist_at(P.TW, x + P.pad, z + P.pad)] == 0)
+                ok = false;
         }
         const unsigned long long okm = __ballot(ok);
         unsigned long long kept = 0; /* cells of the block queued for k_gc_foot */
@@ -1999,15 +2013,27 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *__restric
             if (lane == 0)
                 at = atomicAdd(&counters[FR_CNT_DEMAND2], __popcll(kept));
             at = dev_bcast_i(at, 0);
-            if ((kept >> lane) & 1ull)
-                dq2[at + __popcll(kept & ((1ull << lane) - 1ull))] = d;
+            if ((kept >> lane) & 1ull) {
+                const int o = at + __popcll(kept & ((1ull << lane) - 1ull));
+                dq2[o] = d;
+                FrGcGeo g;
+                g.t1x = (float)hm.t1x;
+                g.t1y = (float)hm.t1y;
+                g.t2x = (float)hm.t2x;
+                g.t2y = (float)hm.t2y;
+                g.lh = lh;
+                g.np = np;
+                geo2[o] = g;
+            }
         }
     }
 }
 
-/* Footprints and cost of the cells k_gc_eval kept. */
+/* Footprints and cost of the cells k_gc_eval kept, from the set-up it
+ * handed over (the tangents are floats widened to double, so the curve is
+ * the one k_gc_eval walked). */
 __global__ void __launch_bounds__(FR_BLOCK) k_gc_foot(const FrPlanDev *__restrict__ plans, const int2 *__restrict__ dq2,
-                                                      const int *__restrict__ counters)
+                                                      const FrGcGeo *__restrict__ geo2, const int *__restrict__ counters)
 {
     const long n = counters[FR_CNT_DEMAND2];
     const int wpb = blockDim.x >> 6;
@@ -2023,7 +2049,22 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_foot(const FrPlanDev *__restric
         float lh = 0, base = 0;
         if (lane < cnt) {
             d = dq2[b0 + lane];
-            (void)dev_gc_lane_setup(plans, d, true, &hm, &np, &lh, &base);
+            const FrGcGeo g = geo2[b0 + lane];
+            const FrPlanDev &P = plans[d.x];
+            const int z = d.y / P.W, x = d.y - z * P.W;
+            hm.t1x = (double)g.t1x;
+            hm.t1y = (double)g.t1y;
+            hm.t2x = (double)g.t2x;
+            hm.t2y = (double)g.t2y;
+            hm.dx0 = (double)x;
+            hm.dz0 = (double)z;
+            hm.dgx = (double)P.gx;
+            hm.dgz = (double)P.gz;
+            np = g.np;
+            hm.denom = (double)(np - 1);
+            hm.rden = 1.0 / hm.denom;
+            lh = g.lh;
+            base = P.gc_cost[d.y];
         }
         float res = -1;
         for (int k = 0; k < cnt; ++k) {
@@ -2755,8 +2796,8 @@ void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st)
     hipLaunchKernelGGL(k_gc_demand, dim3(L.grid_waves), b, 0, st, L.plans, L.fresh_q, L.demand, L.demand_cap,
                        L.counters);
     hipLaunchKernelGGL(k_gc_eval, dim3(L.grid_waves), b, 0, st, L.plans, L.demand, L.demand_cap, L.demand2,
-                       L.counters);
-    hipLaunchKernelGGL(k_gc_foot, dim3(L.grid_waves), b, 0, st, L.plans, L.demand2, L.counters);
+                       L.demand2_geo, L.counters);
+    hipLaunchKernelGGL(k_gc_foot, dim3(L.grid_waves), b, 0, st, L.plans, L.demand2, L.demand2_geo, L.counters);
     fr_prof_end(FR_PROF_GOAL_CONNECTION, st);
     hipLaunchKernelGGL(k_connect_emit, dim3(L.grid_waves), b, 0, st, L.plans, L.ctl, L.fresh_q, L.pairs,
                        L.pair_cap, L.counters);
