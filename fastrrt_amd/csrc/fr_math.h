@@ -166,38 +166,51 @@ FR_HD float fr_tanf(float x)
     return (float)(s / c);
 }
 
-/* atan in FP64: three half-angle reductions atan(a) = 2 atan(a/(1+sqrt(1+a^2)))
- * bring |a| <= 1 down to <= tan(pi/32); an odd Taylor series finishes. */
+/* atan of a in [0, 1] (pi/2 - atan a when inv), in FP64.  With j*pi/16 the
+ * nearest of five angles and c = tan(j*pi/16) (rounded to FP64),
+ * atan(a) = j*pi/16 + atan(t), t = (a - c) / (1 + a c), |t| <= tan(pi/32),
+ * and the odd Taylor series of atan(t) to t^17 finishes (the first term left
+ * out is below 4e-21).  The tangents and thresholds tan((2j+1)pi/32) come from
+ * a 60-digit evaluation. */
+FR_HD double fr_atan_unit(double a, int inv)
+{
+    const int j = (a > 0.09849140335716425) + (a > 0.3033466836073424) + (a > 0.5345111359507917) +
+                  (a > 0.8206787908286604);
+    double c = 0.0;
+    c = j == 1 ? 0.198912367379658 : c;
+    c = j == 2 ? 0.41421356237309503 : c;
+    c = j == 3 ? 0.6681786379192989 : c;
+    c = j == 4 ? 1.0 : c;
+    const double t = (a - c) / (1.0 + a * c);
+    const double t2 = t * t;
+    double p = 1.0 / 17.0;
+    p = 1.0 / 15.0 - t2 * p;
+    p = 1.0 / 13.0 - t2 * p;
+    p = 1.0 / 11.0 - t2 * p;
+    p = 1.0 / 9.0 - t2 * p;
+    p = 1.0 / 7.0 - t2 * p;
+    p = 1.0 / 5.0 - t2 * p;
+    p = 1.0 / 3.0 - t2 * p;
+    double r = (double)j * 0.19634954084936207 + (t - t * (t2 * p));
+    if (inv)
+        r = FR_HALF_PI_D - r;
+    return r;
+}
+
 FR_HD double fr_atan_d(double x)
 {
     if (x != x)
         return x;
     int neg = x < 0;
     double a = neg ? -x : x;
-    int inv = 0;
+    double r;
     if (a > 1.0) {
-        if (a - a != 0.0) { /* infinity */
+        if (a - a != 0.0) /* infinity */
             return neg ? -FR_HALF_PI_D : FR_HALF_PI_D;
-        }
-        a = 1.0 / a;
-        inv = 1;
+        r = fr_atan_unit(1.0 / a, 1);
+    } else {
+        r = fr_atan_unit(a, 0);
     }
-    a = a / (1.0 + __builtin_sqrt(1.0 + a * a));
-    a = a / (1.0 + __builtin_sqrt(1.0 + a * a));
-    a = a / (1.0 + __builtin_sqrt(1.0 + a * a));
-    double a2 = a * a;
-    double p = 1.0 / 19.0;
-    p = 1.0 / 17.0 - a2 * p;
-    p = 1.0 / 15.0 - a2 * p;
-    p = 1.0 / 13.0 - a2 * p;
-    p = 1.0 / 11.0 - a2 * p;
-    p = 1.0 / 9.0 - a2 * p;
-    p = 1.0 / 7.0 - a2 * p;
-    p = 1.0 / 5.0 - a2 * p;
-    p = 1.0 / 3.0 - a2 * p;
-    double r = 8.0 * (a - a * (a2 * p));
-    if (inv)
-        r = FR_HALF_PI_D - r;
     return neg ? -r : r;
 }
 
@@ -228,7 +241,8 @@ FR_HD double fr_atan2_d(double y, double x)
     } else if (ay - ay != 0.0) {
         r = FR_HALF_PI_D;
     } else {
-        r = fr_atan_d(ay / ax);
+        /* one division: the smaller magnitude over the larger */
+        r = ay > ax ? fr_atan_unit(ax / ay, 1) : fr_atan_unit(ay / ax, 0);
         if (fr_signbit_d(x))
             r = FR_PI_D - r;
     }
