@@ -1473,13 +1473,13 @@ __global__ void __launch_bounds__(FR_BLOCK) k_commit(const FrPlanDev *plans, FrC
 }
 
 /* __CUDA_count_nodes_in_density_region (graph_expansion_search_zones.cu:42-66) */
-__global__ void __launch_bounds__(FR_BLOCK) k_density(const FrPlanDev *plans, const FrCtl *ctl, const int2 *tiles, const int *n_tiles_p)
+/* Region counts of smart plans after the expansion (wave `wave` of `nw`
+ * over the tiles); run by the density blocks of k_erase_mark. */
+__device__ void dev_density_tiles(const FrPlanDev *plans, const FrCtl *ctl, const int2 *tiles, int n_tiles, int wave,
+                                  int nw)
 {
-    const int n_tiles = *n_tiles_p;
-    const int wpb = blockDim.x >> 6;
-    const int nw = gridDim.x * wpb;
     const int lane = dev_lane();
-    for (int t = blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); t < n_tiles; t += nw) {
+    for (int t = wave; t < n_tiles; t += nw) {
         const int2 tl = tiles[t];
         const int slot = __builtin_amdgcn_readfirstlane(tl.x);
         const FrPlanDev &P = plans[slot];
@@ -1522,7 +1522,11 @@ __device__ __forceinline__ bool dev_solve_runs(const FrCtl &C)
 #define FR_ERASE_BLOCK 1024
 #define FR_ERASE_CAP 4096
 
-/* One block per plan that runs the erase.  The plan's non-NULL cells are
+/* Blocks [0, n_mark_blocks): one block per plan that runs the erase.  The
+ * blocks after them count the smart plans' density regions
+ * (graph_expansion_search_zones.cu:42-66, after the expansion and before the
+ * erase, which this kernel only marks), so that pass needs no launch of its
+ * own.  The plan's non-NULL cells are
  * gathered into LDS in cell order; each one's verdict (some ancestor is
  * COLLISION or NULL) is then found by pointer jumping over parent indices in
  * LDS, doubling the covered chain length each round, instead of each cell
@@ -1531,12 +1535,22 @@ __device__ __forceinline__ bool dev_solve_runs(const FrCtl &C)
  * in the jump) or at its first bad parent, as the walk does.  A plan with
  * more than FR_ERASE_CAP non-NULL cells takes the walk. */
 __global__ void __launch_bounds__(FR_ERASE_BLOCK) k_erase_mark(const FrPlanDev *plans, const FrCtl *ctl,
-                                                               const int *slots, const int *n_slots_p)
+                                                               const int *slots, const int *n_slots_p,
+                                                               const int2 *tiles, const int *n_tiles_p,
+                                                               int n_mark_blocks)
 {
     __shared__ int s_pos[FR_ERASE_CAP];
     __shared__ uint16_t s_anc[2][FR_ERASE_CAP];
     __shared__ uint8_t s_bad[2][FR_ERASE_CAP];
     __shared__ int s_wsum[FR_ERASE_BLOCK / 64];
+    if ((int)blockIdx.x >= n_mark_blocks) {
+        const int wpb = FR_ERASE_BLOCK / 64;
+        dev_density_tiles(plans, ctl, tiles, *n_tiles_p,
+                          ((int)blockIdx.x - n_mark_blocks) * wpb +
+                              __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)),
+                          ((int)gridDim.x - n_mark_blocks) * wpb);
+        return;
+    }
     if ((int)blockIdx.x >= *n_slots_p)
         return;
     const int slot = slots[blockIdx.x];
@@ -2161,41 +2175,11 @@ __global__ void __launch_bounds__(FR_BLOCK) k_connect_emit(const FrPlanDev *plan
     }
 }
 
-/* one wave per (node, flagged cell) pair */
-__global__ void __launch_bounds__(FR_BLOCK) k_connect_eval(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, FrPair *pairs, long pair_cap,
-                               const int *counters)
+/* Nodes whose pairs did not fit the pair queue scan their window in-thread
+ * (thread `t` of `nt`); run by k_connect_eval after its pairs. */
+__device__ void dev_connect_inline(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, int n, int t, int nt)
 {
-    long n = counters[FR_CNT_PAIRS];
-    if (n > pair_cap)
-        n = pair_cap;
-    const int n_fresh = counters[FR_CNT_FRESH];
-    const int wpb = blockDim.x >> 6;
-    const long nw = (long)gridDim.x * wpb;
-    for (long p = blockIdx.x * (long)wpb + (threadIdx.x >> 6); p < n; p += nw) {
-        const FrPair pr = pairs[p];
-        if (pr.fresh < 0 || pr.fresh >= n_fresh)
-            continue; /* slot reserved by a node whose pairs did not fit */
-        FrFresh &f = fq[pr.fresh];
-        if (f.flags & FR_FRESH_INLINE)
-            continue;
-        if ((long)f.pair_base + f.pair_cnt <= p || p < f.pair_base)
-            continue; /* a pair slot past the capacity check */
-        const FrPlanDev &P = plans[f.slot];
-        float c;
-        const long long lc = dev_pair_lcost(P, f.pos, pr.cell, &c, true);
-        if (lc >= FR_BIG_COST || dev_lane() != 0)
-            continue;
-        pairs[p].cost = c; /* read back by k_finish for the winning pair */
-        atomicMin(&f.key, dev_node_key(lc, (int)(p - f.pair_base)));
-        atomicMin((unsigned long long *)&ctl[f.slot].best_lcost, fr_lcost_key(lc));
-    }
-}
-
-/* nodes whose pairs did not fit the queue scan their window in-thread */
-__global__ void __launch_bounds__(FR_BLOCK) k_connect_inline(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, const int *counters)
-{
-    const int n = counters[FR_CNT_FRESH];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = t; i < n; i += nt) {
         FrFresh &f = fq[i];
         if (!(f.flags & FR_FRESH_INLINE))
             continue;
@@ -2227,6 +2211,39 @@ __global__ void __launch_bounds__(FR_BLOCK) k_connect_inline(const FrPlanDev *pl
         if (bl < FR_BIG_COST)
             atomicMin((unsigned long long *)&ctl[f.slot].best_lcost, fr_lcost_key(bl));
     }
+}
+
+/* one wave per (node, flagged cell) pair, then the nodes whose pairs did
+ * not fit (their flag keeps them out of the pair loop) */
+__global__ void __launch_bounds__(FR_BLOCK) k_connect_eval(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, FrPair *pairs, long pair_cap,
+                               const int *counters)
+{
+    long n = counters[FR_CNT_PAIRS];
+    if (n > pair_cap)
+        n = pair_cap;
+    const int n_fresh = counters[FR_CNT_FRESH];
+    const int wpb = blockDim.x >> 6;
+    const long nw = (long)gridDim.x * wpb;
+    for (long p = blockIdx.x * (long)wpb + (threadIdx.x >> 6); p < n; p += nw) {
+        const FrPair pr = pairs[p];
+        if (pr.fresh < 0 || pr.fresh >= n_fresh)
+            continue; /* slot reserved by a node whose pairs did not fit */
+        FrFresh &f = fq[pr.fresh];
+        if (f.flags & FR_FRESH_INLINE)
+            continue;
+        if ((long)f.pair_base + f.pair_cnt <= p || p < f.pair_base)
+            continue; /* a pair slot past the capacity check */
+        const FrPlanDev &P = plans[f.slot];
+        float c;
+        const long long lc = dev_pair_lcost(P, f.pos, pr.cell, &c, true);
+        if (lc >= FR_BIG_COST || dev_lane() != 0)
+            continue;
+        pairs[p].cost = c; /* read back by k_finish for the winning pair */
+        atomicMin(&f.key, dev_node_key(lc, (int)(p - f.pair_base)));
+        atomicMin((unsigned long long *)&ctl[f.slot].best_lcost, fr_lcost_key(lc));
+    }
+    dev_connect_inline(plans, ctl, fq, n_fresh, (int)(blockIdx.x * blockDim.x + threadIdx.x),
+                       (int)(gridDim.x * blockDim.x));
 }
 
 __global__ void __launch_bounds__(FR_BLOCK) k_connect_pick(const FrPlanDev *plans, FrCtl *ctl, const FrFresh *fq, const int *counters)
@@ -2783,9 +2800,9 @@ void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st)
 {
     dim3 b(FR_BLOCK);
     hipLaunchKernelGGL(k_commit, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.exp_q, L.counters);
-    hipLaunchKernelGGL(k_density, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles);
-    hipLaunchKernelGGL(k_erase_mark, dim3(L.max_slots), dim3(FR_ERASE_BLOCK), 0, st, L.plans, L.ctl, L.slots,
-                       L.n_slots);
+    const int dens_blocks = (L.grid_tiles + FR_ERASE_BLOCK / FR_BLOCK - 1) / (FR_ERASE_BLOCK / FR_BLOCK);
+    hipLaunchKernelGGL(k_erase_mark, dim3(L.max_slots + dens_blocks), dim3(FR_ERASE_BLOCK), 0, st, L.plans, L.ctl,
+                       L.slots, L.n_slots, L.tiles, L.n_tiles, L.max_slots);
     hipLaunchKernelGGL(k_erase_apply, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles,
                        L.fresh_q, L.counters);
     hipLaunchKernelGGL(k_coll_parents, dim3(L.grid_items), b, 0, st, L.plans, L.fresh_q, L.counters);
@@ -2803,7 +2820,6 @@ void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st)
                        L.pair_cap, L.counters);
     hipLaunchKernelGGL(k_connect_eval, dim3(L.grid_waves), b, 0, st, L.plans, L.ctl, L.fresh_q, L.pairs,
                        L.pair_cap, L.counters);
-    hipLaunchKernelGGL(k_connect_inline, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);
     fr_prof_end(FR_PROF_CONNECT, st);
     hipLaunchKernelGGL(k_connect_pick, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);
     hipLaunchKernelGGL(k_finish, dim3(fr_grid_for(L.max_slots, FR_BLOCK / 64, 65535)), b, 0, st, L.plans, L.ctl,
