@@ -22,7 +22,7 @@ Success is compared plan by plan on the plans both forms ran.  Frames are
 uploaded before the timed region (device-resident inputs).
 
 roofline: the batched form's unit of work, one launch of the captured
-iteration graph (one loop iteration of every running plan: 15 kernels, of
+iteration graph (one loop iteration of every running plan: 14 kernels, of
 which the two lazy goal-plane kernels and k_expand take about half the time).
 The timed region retires plans on wall-clock deadlines, so its work depends
 on timing (and a profiler's overhead changes it); the roofline is therefore
@@ -208,7 +208,7 @@ def roofline_pass(L, batch, plans, args, frame_of):
             traffic = tr["hbm_bytes_per_iteration"]
     ex_avg = ex_ms / ex_n
     ex_bytes = ROLLOUT_BYTES * rollouts / ex_n
-    return {"bound": "hbm", "kernel": "iteration graph (15 kernels, one launch per loop iteration of every plan)",
+    return {"bound": "hbm", "kernel": "iteration graph (14 kernels, one launch per loop iteration of every plan)",
             "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
             "avg_launch_ms": round(avg_ms, 5), "launches": it_n, "plans": len(plans),

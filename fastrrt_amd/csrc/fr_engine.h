@@ -16,7 +16,10 @@
 
 #include "fr_device.h"
 
-enum { FR_CNT_EXP = 0, FR_CNT_FRESH = 1, FR_CNT_PAIRS = 2, FR_CNT_DEMAND = 3, FR_CNT_DEMAND2 = 4, FR_CNT_N = 8 };
+/* FR_CNT_CONV: the entries at the head of the fresh queue that the erase
+ * converted (COLLISION -> NODE), queued before the accepted nodes */
+enum { FR_CNT_EXP = 0, FR_CNT_FRESH = 1, FR_CNT_PAIRS = 2, FR_CNT_DEMAND = 3, FR_CNT_DEMAND2 = 4, FR_CNT_CONV = 5,
+       FR_CNT_N = 8 };
 
 /* goal-connection planes at setup: untouched, evaluated when first needed
  * (lazy), or evaluated for every cell (eager) */

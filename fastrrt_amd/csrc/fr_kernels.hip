@@ -1058,6 +1058,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_begin(const FrPlanDev *plans, FrCt
         counters[FR_CNT_PAIRS] = 0;
         counters[FR_CNT_DEMAND] = 0;
         counters[FR_CNT_DEMAND2] = 0;
+        counters[FR_CNT_CONV] = 0;
     }
     if (i >= n_slots)
         return;
@@ -1702,6 +1703,8 @@ __global__ void __launch_bounds__(FR_BLOCK) k_erase_apply(const FrPlanDev *plans
         }
         int total;
         int w = dev_wave_reserve(&counters[FR_CNT_FRESH], __popc(conv), &total);
+        if (lane == 0 && total)
+            atomicAdd(&counters[FR_CNT_CONV], total);
         for (uint32_t m = conv; m; m &= m - 1) {
             FrFresh f;
             f.slot = slot;
@@ -1716,11 +1719,13 @@ __global__ void __launch_bounds__(FR_BLOCK) k_erase_apply(const FrPlanDev *plans
     }
 }
 
-/* second half of set_nodes: the parents of converted nodes gain a child */
-__global__ void __launch_bounds__(FR_BLOCK) k_coll_parents(const FrPlanDev *plans, const FrFresh *fq, const int *counters)
+/* second half of set_nodes: the parents of converted nodes gain a child
+ * (thread t of nt over the converted head of the fresh queue; run at the
+ * start of k_accept, which neither reads the child counts nor writes that
+ * head: its own entries go after it) */
+__device__ void dev_coll_parents(const FrPlanDev *plans, const FrFresh *fq, int n, int t, int nt)
 {
-    const int n = counters[FR_CNT_FRESH];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = t; i < n; i += nt) {
         const FrFresh f = fq[i];
         if (!(f.flags & FR_FRESH_CONVERTED))
             continue;
@@ -1734,10 +1739,13 @@ __global__ void __launch_bounds__(FR_BLOCK) k_coll_parents(const FrPlanDev *plan
 /* count(TEMP), countAll() and __CUDA_accept_derived_nodes
  * (fastrrt.cpp:130-142, graph_expansion.cu:36-65).  Accepted nodes, the
  * re-added start and (after external edits) every node are queued for the
- * goal test and the goal connection search. */
+ * goal test and the goal connection search.  First, the parents of the
+ * nodes the erase converted gain their child (dev_coll_parents). */
 __global__ void __launch_bounds__(FR_BLOCK) k_accept(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, const int *n_tiles_p,
                          FrFresh *fq, int *counters)
 {
+    dev_coll_parents(plans, fq, counters[FR_CNT_CONV], (int)(blockIdx.x * blockDim.x + threadIdx.x),
+                     (int)(gridDim.x * blockDim.x));
     const int n_tiles = *n_tiles_p;
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
@@ -2805,7 +2813,6 @@ void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st)
                        L.slots, L.n_slots, L.tiles, L.n_tiles, L.max_slots);
     hipLaunchKernelGGL(k_erase_apply, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles,
                        L.fresh_q, L.counters);
-    hipLaunchKernelGGL(k_coll_parents, dim3(L.grid_items), b, 0, st, L.plans, L.fresh_q, L.counters);
     hipLaunchKernelGGL(k_accept, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles, L.fresh_q,
                        L.counters);
     fr_prof_begin(FR_PROF_CONNECT, st);
