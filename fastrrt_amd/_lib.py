@@ -22,6 +22,8 @@ SIGNATURES = {
     "fastrrt_frame_create": (c_void_p, [c_int, c_int, fp, fp, c_int]),
     "fastrrt_frame_destroy": (None, [c_void_p]),
     "fastrrt_frame_update": (c_bool, [c_void_p, fp]),
+    "fastrrt_frame_read": (c_bool, [c_void_p, fp]),
+    "fastrrt_frame_process_distance_to_goal": (c_bool, [c_void_p, c_int, c_int]),
     "fastrrt_last_error": (c_char_p, []),
     # FastRRT handle (src/python_bind.cpp)
     "fastrrt_initialize": (c_void_p, [c_int, c_int, c_float, c_float, c_float, c_float, c_int, c_int, c_int, c_int,

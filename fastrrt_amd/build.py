@@ -16,7 +16,7 @@ LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libfastrrt.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["fr_kernels.hip", "fr_apf.hip", "fr_engine.cpp", "fr_api.cpp", "fr_batch.cpp"]
+SOURCES = ["fr_kernels.hip", "fr_apf.hip", "fr_frame.hip", "fr_engine.cpp", "fr_api.cpp", "fr_batch.cpp"]
 
 COMMON = [
     "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",

@@ -1026,6 +1026,33 @@ bool fastrrt_frame_update(void *frame, const float *xyz3)
     })
 }
 
+bool fastrrt_frame_read(void *frame, float *xyz3)
+{
+    FR_TRY(false, {
+        FrFrameObj *f = static_cast<FrFrameObj *>(frame);
+        if (!f || !xyz3)
+            throw std::runtime_error("null frame or pixel buffer");
+        FR_CHECK(hipDeviceSynchronize());
+        FR_CHECK(hipMemcpy(xyz3, f->d_xyz, sizeof(float) * 3 * (size_t)f->W * f->H, hipMemcpyDeviceToHost));
+        return true;
+    })
+}
+
+bool fastrrt_frame_process_distance_to_goal(void *frame, int goal_x, int goal_z)
+{
+    FR_TRY(false, {
+        FrFrameObj *f = static_cast<FrFrameObj *>(frame);
+        if (!f)
+            throw std::runtime_error("null frame");
+        FR_CHECK(hipDeviceSynchronize());
+        fr_launch_frame_distance_to_goal(f->d_xyz, f->W, f->H, goal_x, goal_z, nullptr);
+        FR_CHECK(hipGetLastError());
+        FR_CHECK(hipDeviceSynchronize());
+        f->gen += 1;
+        return true;
+    })
+}
+
 /* ---- FastRRT handle ---- */
 
 void *fastrrt_initialize(int width, int height, float perception_width_m, float perception_height_m,

@@ -101,6 +101,9 @@ void fr_launch_accept(const FrLaunch &L, int slot, int N, hipStream_t st);
 void fr_launch_solve_collisions(const FrLaunch &L, int slot, int N, int nodes, hipStream_t st);
 void fr_launch_region_count(const FrLaunch &L, int slot, int N, uint32_t *d_out, hipStream_t st);
 
+/* Distance-to-goal channel (pixel channel 1) of a frame (fr_frame.hip). */
+void fr_launch_frame_distance_to_goal(float *d_xyz, int W, int H, int gx, int gz, hipStream_t st);
+
 /* Potential-field intrinsic costs of one grid (fr_apf.hip). */
 struct FrApfArgs {
     int W, H;

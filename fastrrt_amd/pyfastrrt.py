@@ -102,6 +102,20 @@ class SearchFrame:
     def get_cuda_ptr(self):
         return self._ptr
 
+    def read(self) -> np.ndarray:
+        """The device pixels, (height, width, 3) float32."""
+        out = np.empty((self._h, self._w, 3), np.float32)
+        if not _lib.load().fastrrt_frame_read(self._ptr, out.ctypes.data_as(_lib.fp)):
+            _lib.check("frame_read")
+        return out
+
+    def process_distance_to_goal(self, goal_x: int, goal_z: int) -> None:
+        """Fill channel 1 with the pixel distance to the goal (libdriveless
+        SearchFrame::processDistanceToGoal; the curve length of
+        CudaGraph.can_connect_to_goal_using_hermite)."""
+        if not _lib.load().fastrrt_frame_process_distance_to_goal(self._ptr, goal_x, goal_z):
+            _lib.check("frame_process_distance_to_goal")
+
     def update(self, pixels):
         px = np.ascontiguousarray(pixels, dtype=np.float32)
         if px.shape != (self._h, self._w, 3):

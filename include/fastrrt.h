@@ -35,6 +35,15 @@ void *fastrrt_frame_create(int width, int height, const float *xyz3, const float
 void fastrrt_frame_destroy(void *frame);
 /* Re-upload the pixels of an existing frame (same size); returns false on error. */
 bool fastrrt_frame_update(void *frame, const float *xyz3);
+/* The frame's pixels back to the host (W*H*3 floats).  No reference
+ * counterpart: the reference reads libdriveless SearchFrame memory directly. */
+bool fastrrt_frame_read(void *frame, float *xyz3);
+/* libdriveless SearchFrame::processDistanceToGoal (called at
+ * unittests/src/test_graph_can_connect_hermite.cpp:51; library absent from
+ * the reference tree): channel 1 of every pixel = sqrtf of its integer
+ * squared pixel distance to (goal_x, goal_z), the curve length
+ * can_connect_to_goal_using_hermite reads. */
+bool fastrrt_frame_process_distance_to_goal(void *frame, int goal_x, int goal_z);
 /* Message of the last failed call on this thread ("" if none). */
 const char *fastrrt_last_error(void);
 

@@ -19,11 +19,29 @@ def _dist_channel(frame, gx, gz):
     frame[:, :, 1] = np.sqrt(((gx - xx) ** 2 + (gz - zz) ** 2).astype(np.float32))
 
 
+def test_frame_distance_to_goal_channel():
+    """process_distance_to_goal fills channel 1 with float32 sqrt of the
+    integer squared distance and leaves channels 0 and 2 alone."""
+    rng = np.random.default_rng(4)
+    px = rng.uniform(0, 30, (70, 90, 3)).astype(np.float32)
+    frame = P.SearchFrame(px, [0, 1])
+    frame.process_distance_to_goal(33, -5)
+    got = frame.read()
+    want = px.copy()
+    _dist_channel(want, 33, -5)
+    assert np.array_equal(got.view(np.int32), want.view(np.int32))
+
+
 def test_reference_hermite_gtests():
     costs_ok = [0, 1, 2, 3, 4, 5]
     costs_blocked = [-1, 1, 2, 3, 4, 5]
     g = P.CudaGraph(256, 256, 256, 256, 40, 5.412658773, 0, 0, -1, -1, -1, -1, np.array(costs_ok))
     g.add(128, 128, P.angle(0.0), -1, -1, 0)
+    # StraightConnectionSuccess, with the frame processed as the gtest does
+    fp = P.SearchFrame(O.empty_frame(256, 256), costs_ok)
+    fp.process_distance_to_goal(128, 0)
+    assert g.can_connect_to_goal_using_hermite(fp, 128, 128, 128, 0, 0.0)
+    assert g.can_connect_hermite_value(fp, 128, 128, 128, 0, 0.0) == (False, 127.0)
     f = O.empty_frame(256, 256)
     _dist_channel(f, 128, 0)
     assert g.can_connect_to_goal_using_hermite(P.SearchFrame(f, costs_ok), 128, 128, 128, 0, 0.0)
