@@ -2053,9 +2053,8 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *__restric
 
 /* Footprints and cost of the cells k_gc_eval kept, from the set-up it
  * handed over (the tangents are floats widened to double, so the curve is
- * the one k_gc_eval walked).  Held to 96 registers for five waves per SIMD
- * (a few bytes spill): the first iteration's long curves run 12 % faster. */
-__global__ void __launch_bounds__(FR_BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) k_gc_foot(const FrPlanDev *__restrict__ plans, const int2 *__restrict__ dq2,
+ * the one k_gc_eval walked). */
+__global__ void __launch_bounds__(FR_BLOCK) k_gc_foot(const FrPlanDev *__restrict__ plans, const int2 *__restrict__ dq2,
                                                       const FrGcGeo *__restrict__ geo2, const int *__restrict__ counters)
 {
     const long n = counters[FR_CNT_DEMAND2];
