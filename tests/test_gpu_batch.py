@@ -127,3 +127,28 @@ def test_batch_smart_regions_equals_oracle():
             assert r.path.shape == po.shape and (r.path.view(np.int32) == po.view(np.int32)).all(), i
         else:
             assert r.status == 3
+
+
+def test_odd_sizes_alone_equals_in_the_mix():
+    """Grids whose sides are not multiples of 8, 32 or 64, non-square, in one
+    batch: each plan's result is its result alone."""
+    from test_gpu_parity import _odd_scene
+    shapes = [(101, 83), (77, 130), (200, 47), (65, 65), (129, 97)]
+    scs = [_odd_scene(w, h, 500 + i) for i, (w, h) in enumerate(shapes)]
+    frames = [P.SearchFrame(sc.frame, sc.class_costs) for sc in scs]
+
+    def spec(i, sc):
+        return PlanSpec(frames[i], sc.start, sc.goal, min_dist=(10, 16), max_path_size=30.0,
+                        dist_to_goal_tolerance=12.0, max_iterations=40, optimize_rounds=3, seed=31, stream=i, tag=i,
+                        smart=(i % 2 == 1), region_side=16 if i % 2 else 0)
+    alone = {}
+    for i, sc in enumerate(scs):
+        b = Batch(max_plans=1, max_cells=sc.width * sc.height)
+        b.submit(spec(i, sc))
+        (r,) = b.drain()
+        alone[i] = _key(r)
+    b = Batch(max_plans=len(scs), max_cells=sum(sc.width * sc.height for sc in scs))
+    for i, sc in enumerate(scs):
+        b.submit(spec(i, sc))
+    mixed = {r.tag: _key(r) for r in b.drain()}
+    assert mixed == alone

@@ -55,6 +55,25 @@ def _run_side_by_side(sc, prm, seed, stream, max_iter, smart=False, goal_planes_
     return o, g, it
 
 
+def _odd_scene(W, H, seed, coverage=0.08, min_dist=(10, 16)):
+    """A non-square grid whose sides are not multiples of 8, 32 or 64 (row
+    and tile boundaries fall inside rows, the distance plane's last tiles are
+    partial)."""
+    hx, hz = min_dist[0] // 2 + 1, min_dist[1] // 2 + 1
+    start = (W // 2, H - hz - 2, 0.0)
+    goal = (W // 3, 1, 0.3)
+    frame = scenes.make_frame(W, H, coverage, seed, keep_free=[(start[0], start[1], hx, hz),
+                                                              (goal[0], goal[1], hx, hz)])
+    return scenes.Scene(frame, start, goal, seed, coverage, W * 0.135316469, min_dist, (-1, -1), (-1, -1))
+
+
+@pytest.mark.parametrize("shape", [(101, 83), (77, 130), (200, 47)])
+def test_loop_parity_odd_sizes(shape):
+    sc = _odd_scene(shape[0], shape[1], 40 + shape[0])
+    _run_side_by_side(sc, SMALL, 91, 3, 40)
+    _run_side_by_side(sc, SMALL, 92, 4, 30, smart=True, region_side=16)
+
+
 @pytest.mark.parametrize("seed", [11, 12, 13, 14])
 def test_loop_parity_small(seed):
     sc = _small_scene(seed)
