@@ -26,7 +26,11 @@
  *   reference's known-answer tests and closed forms.
  *   density gating (smart expansion): pinned by brute-force recount only.
  *   potential-field costs: pinned by the closed form of one isolated obstacle
- *   and by brute-force sums in both source orders on small grids.
+ *   and by brute-force float64 sums on small grids.
+ *   Hermite test hook (canConnectToGoal): pinned by the reference's three
+ *   gtests, the straight-connection closed form, the pinned direct-connection
+ *   check on the same curves and the curvature measure from finite
+ *   differences of the curve.
  */
 #ifndef FR_ORACLE_H
 #define FR_ORACLE_H
