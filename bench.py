@@ -278,7 +278,14 @@ def cpu_baseline(steps, args):
     with mp.get_context("fork").Pool(workers) as pool:
         pool.map(_cpu_plan, [(pl.sc, pl.stream, args.max_iter, args.opt_rounds) for pl in many])
     wall = time.perf_counter() - t0
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(len(many) / wall, 4), "unit": "plans/s", "cores": workers, "kind": "port",
+            "host": {"cpu_model": model, "nproc": os.cpu_count()},
             "sample": f"{len(many)} plans of the timed workload (one in {stride} of the first timed step, all "
                       f"coverages) through the CPU oracle, one process per core, {wall:.1f} s wall",
             "single_thread": {"value": round(len(single) / t_single, 4), "cores": 1,
