@@ -61,6 +61,7 @@ struct FrPlanDev {
     int gc_lazy;              /* evaluate a cell when a node's window first covers it */
     /* scratch planes */
     unsigned long long *claim;
+    int32_t *escr;            /* 4 * N ints: the erase's node list, cell -> list index, two jump buffers */
     uint8_t *mark;
     uint8_t *dist;            /* padded Chebyshev distance to the nearest obstacle pixel, in 8 x 8
                                  tiles of 64 bytes (fr_dist_at) */

@@ -196,6 +196,7 @@ void FrSlab::allocate(int W_, int H_, int mx_, int mz_, int path_cap_, bool with
     const size_t o_path = take(16 * (size_t)path_cap); /* written before read */
     const size_t o_pd = take(16 * (size_t)path_cap);
     const size_t o_claim = take(8 * N);
+    const size_t o_escr = take(16 * N); /* erase scratch, written before read */
     bytes_ = off;
     FR_CHECK(hipMalloc(&base_, bytes_));
     char *b = (char *)base_;
@@ -218,6 +219,7 @@ void FrSlab::allocate(int W_, int H_, int mx_, int mz_, int path_cap_, bool with
     path = (float4 *)(b + o_path);
     pd = (float4 *)(b + o_pd);
     claim = (unsigned long long *)(b + o_claim);
+    escr = (int32_t *)(b + o_escr);
     zero_end_ = zero_end;
     reset(st);
     mx = -1;
@@ -373,6 +375,7 @@ void FrEngine::bind(int slot, FrSlab &s, const FrPlanParams &p, const FrFrameVie
     d.gc_lazy = p.gc_lazy;
     d.claim = s.claim;
     d.mark = s.mark;
+    d.escr = s.escr;
     d.dist = s.dist;
     d.dist_tmp = s.dist_tmp;
     d.trav = s.trav;

@@ -194,6 +194,7 @@ class FrSlab {
     void *zero_base() const { return base_; }
     size_t zero_bytes() const { return zero_end_; }
     unsigned long long *claim = nullptr;
+    int32_t *escr = nullptr; /* 4 * W * H: the erase's node list, index map and jump buffers */
     uint8_t *mark = nullptr, *dist = nullptr, *dist_tmp = nullptr, *trav = nullptr;
     uint32_t *region = nullptr;
     float4 *path = nullptr, *pd = nullptr;

@@ -1534,7 +1534,8 @@ __device__ __forceinline__ bool dev_solve_runs(const FrCtl &C)
  * walking its chain through global memory.  The rounds cover at least the
  * walk's bound of nodes + 4 steps; a chain ends at the root (its own parent
  * in the jump) or at its first bad parent, as the walk does.  A plan with
- * more than FR_ERASE_CAP non-NULL cells takes the walk. */
+ * more than FR_ERASE_CAP non-NULL cells jumps over buffers in global memory
+ * instead of LDS. */
 __global__ void __launch_bounds__(FR_ERASE_BLOCK) k_erase_mark(const FrPlanDev *plans, const FrCtl *ctl,
                                                                const int *slots, const int *n_slots_p,
                                                                const int2 *tiles, const int *n_tiles_p,
@@ -1584,7 +1585,14 @@ __global__ void __launch_bounds__(FR_ERASE_BLOCK) k_erase_mark(const FrPlanDev *
     }
     const int off = base + incl - cnt;
     if (total > FR_ERASE_CAP) {
-        /* the walk of graph_collision.cu:33-83 */
+        /* The same pointer jumping over buffers in global memory (the plan's
+         * erase scratch): the node list in cell order, each listed cell's
+         * list index, and two jump buffers holding the ancestor index with
+         * the bad bit on top.  One block, so the block barrier orders the
+         * rounds. */
+        int *gpos = P.escr, *gidx = P.escr + P.N;
+        int *gj[2] = {P.escr + 2 * (long)P.N, P.escr + 3 * (long)P.N};
+        int w = off;
         for (int c = c0; c < c1; ++c) {
             const uint4 v = *reinterpret_cast<const uint4 *>(P.type + 16 * c);
             uint32_t nz = dev_bytes_nz(v);
@@ -1592,24 +1600,42 @@ __global__ void __launch_bounds__(FR_ERASE_BLOCK) k_erase_mark(const FrPlanDev *
                 nz &= (1u << (P.N - 16 * c)) - 1u;
             for (uint32_t m = nz; m; m &= m - 1) {
                 const int pos = 16 * c + __ffs((int)m) - 1;
-                uint8_t mk = 0;
-                long curr = pos;
-                int i = nodes + 4;
-                while (i-- > 0) {
-                    const int ppx = P.px[curr], ppz = P.pz[curr];
-                    if (ppx == -1 && ppz == -1)
-                        break;
-                    const long next = (long)ppz * P.W + ppx;
-                    const int tn = (next >= 0 && next < P.N) ? P.type[next] : FR_TYPE_NULL;
-                    if (tn == FR_TYPE_COLLISION || tn == FR_TYPE_NULL) {
-                        mk = 1;
-                        break;
-                    }
-                    curr = next;
-                }
-                P.mark[pos] = mk;
+                gpos[w] = pos;
+                gidx[pos] = w;
+                ++w;
             }
         }
+        __syncthreads();
+        for (int k = t; k < total; k += FR_ERASE_BLOCK) {
+            const int pos = gpos[k];
+            const int ppx = P.px[pos], ppz = P.pz[pos];
+            int v = k;
+            if (!(ppx == -1 && ppz == -1)) {
+                const long next = (long)ppz * P.W + ppx;
+                const int tn = (next >= 0 && next < P.N) ? P.type[next] : FR_TYPE_NULL;
+                if (tn == FR_TYPE_COLLISION || tn == FR_TYPE_NULL)
+                    v = (int)((unsigned)k | 0x80000000u);
+                else /* non-NULL, so in the list */
+                    v = gidx[next];
+            }
+            gj[0][k] = v;
+        }
+        __syncthreads();
+        const int grounds = 32 - __clz(nodes + 4);
+        int gcur = 0;
+        for (int r = 0; r < grounds; ++r) {
+            const int *src = gj[gcur];
+            int *dst = gj[gcur ^ 1];
+            for (int k = t; k < total; k += FR_ERASE_BLOCK) {
+                const unsigned mine = (unsigned)src[k];
+                const unsigned up = (unsigned)src[mine & 0x7FFFFFFFu];
+                dst[k] = (int)((up & 0x7FFFFFFFu) | ((mine | up) & 0x80000000u));
+            }
+            __syncthreads();
+            gcur ^= 1;
+        }
+        for (int k = t; k < total; k += FR_ERASE_BLOCK)
+            P.mark[gpos[k]] = (uint8_t)((unsigned)gj[gcur][k] >> 31);
         return;
     }
     /* the non-NULL cells in cell order */

@@ -67,6 +67,18 @@ def _odd_scene(W, H, seed, coverage=0.08, min_dist=(10, 16)):
     return scenes.Scene(frame, start, goal, seed, coverage, W * 0.135316469, min_dist, (-1, -1), (-1, -1))
 
 
+def test_loop_parity_large_trees():
+    """512x512, a small footprint and a wall that cuts the goal off: smart
+    expansion grows the tree past 5 000 cells, beyond the erase's in-LDS
+    capacity of 4 096, and collisions erase subtrees of it (the erase then
+    jumps over buffers in global memory)."""
+    sc = scenes.scene_sized(77, 512, 0.02, min_dist=(4, 6))
+    sc.frame[20:25, :, 0] = scenes.OBSTACLE_CLASS
+    prm = Params(min_dist=(4, 6), max_path=30.0, dist_tol=3.0, herr_deg=1.0)
+    o, g, it = _run_side_by_side(sc, prm, 5, 1, 80, smart=True, region_side=16)
+    assert int((g.state()["type"] != 0).sum()) > 4096
+
+
 @pytest.mark.parametrize("shape", [(101, 83), (77, 130), (200, 47)])
 def test_loop_parity_odd_sizes(shape):
     sc = _odd_scene(shape[0], shape[1], 40 + shape[0])
