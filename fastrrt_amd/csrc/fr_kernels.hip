@@ -1048,11 +1048,14 @@ __global__ void __launch_bounds__(FR_BLOCK) k_setup_goal_connection(const FrPlan
 /* loop iteration                                                           */
 /* ------------------------------------------------------------------------ */
 
+/* One wave per plan: lane 0 resets the plan's per-iteration control fields,
+ * the lanes together clear the density counts the iteration will write. */
 __global__ void __launch_bounds__(FR_BLOCK) k_begin(const FrPlanDev *plans, FrCtl *ctl, const int *slots, const int *n_slots_p, int *counters)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+    const int lane = dev_lane();
     const int n_slots = *n_slots_p;
-    if (i == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         counters[FR_CNT_EXP] = 0;
         counters[FR_CNT_FRESH] = 0;
         counters[FR_CNT_PAIRS] = 0;
@@ -1066,6 +1069,14 @@ __global__ void __launch_bounds__(FR_BLOCK) k_begin(const FrPlanDev *plans, FrCt
     FrCtl &C = ctl[s];
     const FrPlanDev &P = plans[s];
     if (C.status != FR_SEARCHING)
+        return;
+    if (P.smart) {
+        const int nr = P.dens_w * P.dens_h;
+        uint32_t *wr = P.region + ((C.iter + 1) & 1) * nr;
+        for (int r = lane; r < nr; r += 64)
+            wr[r] = 0;
+    }
+    if (lane != 0)
         return;
     C.frontier = C.last_temp >= 100;
     C.force = C.last_temp == 0;
@@ -1082,12 +1093,6 @@ __global__ void __launch_bounds__(FR_BLOCK) k_begin(const FrPlanDev *plans, FrCt
     C.best_node_key = ~0ull;
     C.epoch += 1;
     C.chain_only = 0;
-    if (P.smart) {
-        const int nr = P.dens_w * P.dens_h;
-        uint32_t *wr = P.region + ((C.iter + 1) & 1) * nr;
-        for (int r = 0; r < nr; ++r)
-            wr[r] = 0;
-    }
 }
 
 /* Tile scans: one wave per tile of FR_TILE = 1024 cells, lane l holding
@@ -1565,14 +1570,22 @@ __global__ void __launch_bounds__(FR_ERASE_BLOCK) k_erase_mark(const FrPlanDev *
     const int nchunks = (P.N + 15) / 16;
     const int per = (nchunks + FR_ERASE_BLOCK - 1) / FR_ERASE_BLOCK;
     const int c0 = t * per, c1 = c0 + per < nchunks ? c0 + per : nchunks;
-    /* count this thread's non-NULL cells, block-wide exclusive scan */
+    /* count this thread's non-NULL cells (four chunks' loads in flight at a
+     * time), block-wide exclusive scan */
     int cnt = 0;
-    for (int c = c0; c < c1; ++c) {
-        const uint4 v = *reinterpret_cast<const uint4 *>(P.type + 16 * c);
-        uint32_t nz = dev_bytes_nz(v);
-        if (16 * c + 16 > P.N)
-            nz &= (1u << (P.N - 16 * c)) - 1u;
-        cnt += __popc(nz);
+    for (int cb = c0; cb < c1; cb += 4) {
+        uint4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            v[u] = cb + u < c1 ? *reinterpret_cast<const uint4 *>(P.type + 16 * (cb + u)) : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = cb + u;
+            uint32_t nz = dev_bytes_nz(v[u]);
+            if (16 * c + 16 > P.N)
+                nz &= c < c1 ? (1u << (P.N - 16 * c)) - 1u : 0u;
+            cnt += __popc(nz);
+        }
     }
     const int incl = dev_wave_incl_scan(cnt);
     if (lane == 63)
@@ -1593,7 +1606,7 @@ __global__ void __launch_bounds__(FR_ERASE_BLOCK) k_erase_mark(const FrPlanDev *
         int *gpos = P.escr, *gidx = P.escr + P.N;
         int *gj[2] = {P.escr + 2 * (long)P.N, P.escr + 3 * (long)P.N};
         int w = off;
-        for (int c = c0; c < c1; ++c) {
+        for (int c = c0; cnt && c < c1; ++c) { /* a thread without cells skips its chunks */
             const uint4 v = *reinterpret_cast<const uint4 *>(P.type + 16 * c);
             uint32_t nz = dev_bytes_nz(v);
             if (16 * c + 16 > P.N)
@@ -1626,10 +1639,25 @@ __global__ void __launch_bounds__(FR_ERASE_BLOCK) k_erase_mark(const FrPlanDev *
         for (int r = 0; r < grounds; ++r) {
             const int *src = gj[gcur];
             int *dst = gj[gcur ^ 1];
-            for (int k = t; k < total; k += FR_ERASE_BLOCK) {
-                const unsigned mine = (unsigned)src[k];
-                const unsigned up = (unsigned)src[mine & 0x7FFFFFFFu];
-                dst[k] = (int)((up & 0x7FFFFFFFu) | ((mine | up) & 0x80000000u));
+            /* four entries per thread at a time, so their loads overlap */
+            for (int k0 = t; k0 < total; k0 += 4 * FR_ERASE_BLOCK) {
+                unsigned mine[4], up[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = k0 + u * FR_ERASE_BLOCK;
+                    mine[u] = k < total ? (unsigned)src[k] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = k0 + u * FR_ERASE_BLOCK;
+                    up[u] = k < total ? (unsigned)src[mine[u] & 0x7FFFFFFFu] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = k0 + u * FR_ERASE_BLOCK;
+                    if (k < total)
+                        dst[k] = (int)((up[u] & 0x7FFFFFFFu) | ((mine[u] | up[u]) & 0x80000000u));
+                }
             }
             __syncthreads();
             gcur ^= 1;
@@ -1638,9 +1666,9 @@ __global__ void __launch_bounds__(FR_ERASE_BLOCK) k_erase_mark(const FrPlanDev *
             P.mark[gpos[k]] = (uint8_t)((unsigned)gj[gcur][k] >> 31);
         return;
     }
-    /* the non-NULL cells in cell order */
+    /* the non-NULL cells in cell order (a thread without cells skips its chunks) */
     int w = off;
-    for (int c = c0; c < c1; ++c) {
+    for (int c = c0; cnt && c < c1; ++c) {
         const uint4 v = *reinterpret_cast<const uint4 *>(P.type + 16 * c);
         uint32_t nz = dev_bytes_nz(v);
         if (16 * c + 16 > P.N)
@@ -2819,7 +2847,7 @@ void fr_launch_goal_complete(const FrLaunch &L, const int *d_slots, int n, int m
 void fr_launch_iteration_pre(const FrLaunch &L, hipStream_t st)
 {
     dim3 b(FR_BLOCK);
-    hipLaunchKernelGGL(k_begin, dim3(fr_grid_for(L.max_slots, FR_BLOCK, 65535)), b, 0, st, L.plans, L.ctl, L.slots,
+    hipLaunchKernelGGL(k_begin, dim3(fr_grid_for(L.max_slots, FR_BLOCK / 64, 65535)), b, 0, st, L.plans, L.ctl, L.slots,
                        L.n_slots, L.counters);
     hipLaunchKernelGGL(k_select, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles, L.exp_q,
                        L.counters);
