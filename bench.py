@@ -89,6 +89,10 @@ def parse():
     ap.add_argument("--iters-per-poll", type=int, default=1)
     ap.add_argument("--roofline-iters", type=int, default=32)
     ap.add_argument("--roofline-only", action="store_true", help="run only the deterministic roofline pass")
+    ap.add_argument("--mode", choices=("batched", "served"), default="batched",
+                    help="batched: each step's plans submitted together and drained; served: the steps' plans "
+                         "admitted continuously, a new plan whenever --admit-group slots are free")
+    ap.add_argument("--admit-group", type=int, default=16, help="served mode: free slots that trigger admission")
     return ap.parse_args()
 
 
@@ -178,6 +182,33 @@ def run_batched(b, plans, args, frame_of, max_iter=None, opt_rounds=None, deadli
     while b.step(args.iters_per_poll) > 0:
         res += b.poll()
     res += b.poll()
+    t1 = time.perf_counter()
+    return res, (t1 - t0) * 1e3
+
+
+def run_served(b, plans, args, frame_of, capacity):
+    """Continuous admission: at most `capacity` plans in flight; whenever at
+    least --admit-group slots are free (or the queue's last plans remain),
+    the next plans are submitted.  Each plan's deadline counts from its own
+    submission."""
+    from fastrrt_amd.batch import PlanSpec
+    t0 = time.perf_counter()
+    queue = list(plans)
+    res, inflight = [], 0
+    group = max(1, min(args.admit_group, capacity))
+    while queue or inflight:
+        if queue and (capacity - inflight >= group or len(queue) <= capacity - inflight):
+            while queue and inflight < capacity:
+                pl = queue.pop(0)
+                sc = pl.sc
+                b.submit(PlanSpec(frame_of(sc), sc.start, sc.goal, min_dist=sc.min_dist, ego_lb=sc.ego_lb,
+                                  ego_ub=sc.ego_ub, max_iterations=args.max_iter, optimize_rounds=args.opt_rounds,
+                                  deadline_ms=args.budget_ms, seed=SEED, stream=pl.stream, tag=pl.tag))
+                inflight += 1
+        b.step(args.iters_per_poll)
+        got = b.poll()
+        res += got
+        inflight -= len(got)
     t1 = time.perf_counter()
     return res, (t1 - t0) * 1e3
 
@@ -326,16 +357,26 @@ def main():
         run_batched(batch, first, args, frame_of, max_iter=args.roofline_iters, opt_rounds=0, deadline=0.0)
         print(json.dumps({"roofline": roofline_pass(L, batch, first, args, frame_of)}), flush=True)
         return
-    for k in range(args.warmup):
-        run_batched(batch, steps[k], args, frame_of)
-    # batched form, timed (no kernel timing inside)
     results, walls = [], []
-    t0 = time.perf_counter()
-    for k in range(args.warmup, args.warmup + args.steps):
-        res, wall = run_batched(batch, steps[k], args, frame_of)
+    if args.mode == "served":
+        for k in range(args.warmup):
+            run_served(batch, steps[k], args, frame_of, B)
+        timed = [pl for k in range(args.warmup, args.warmup + args.steps) for pl in steps[k]]
+        t0 = time.perf_counter()
+        res, wall = run_served(batch, timed, args, frame_of, B)
+        t1 = time.perf_counter()
         results.append(res)
-        walls.append(wall)
-    t1 = time.perf_counter()
+        walls.append(wall / args.steps)
+    else:
+        for k in range(args.warmup):
+            run_batched(batch, steps[k], args, frame_of)
+        # batched form, timed (no kernel timing inside)
+        t0 = time.perf_counter()
+        for k in range(args.warmup, args.warmup + args.steps):
+            res, wall = run_batched(batch, steps[k], args, frame_of)
+            results.append(res)
+            walls.append(wall)
+        t1 = time.perf_counter()
     allres = [r for rs in results for r in rs]
     lat = np.array([r.latency_ms for r in allres])
     n_b = len(allres)
@@ -374,7 +415,7 @@ def main():
         "config": {"workload": "256x256 frames, rectangles 4-30 px at 5/15/30% coverage, start (128,128,0), "
                                "goal x~U[64,192] on row 0, heading~U[-pi/4,pi/4]; min_dist 22x40, P=40, r=20, "
                                "steer 40 deg, L=5.41; loop to goal, then path_optimize rounds",
-                   "mode": "batched", "plans_per_step": B, "budget_ms": args.budget_ms,
+                   "mode": args.mode, "plans_per_step": B, "budget_ms": args.budget_ms,
                    "plain_timeout_ms": args.plain_timeout_ms,
                    "max_iter": args.max_iter, "opt_rounds": args.opt_rounds},
         "p50_ms": round(batched["p50_ms"], 3), "p99_ms": round(batched["p99_ms"], 3),
