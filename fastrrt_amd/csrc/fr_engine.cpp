@@ -253,10 +253,12 @@ FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
         own_stream_ = true;
     }
     h_plans_.assign(max_slots, FrPlanDev{});
-    FR_CHECK(hipHostMalloc((void **)&h_plan_stage_, sizeof(FrPlanDev) * max_slots, hipHostMallocDefault));
+    FR_CHECK(hipHostMalloc((void **)&h_plan_stage_, sizeof(FrPlanDev) * max_slots * FR_PLAN_RING,
+                           hipHostMallocDefault));
     FR_CHECK(hipHostMalloc((void **)&h_int_stage_, sizeof(int) * (max_slots + 1) * FR_STAGE_RING,
                            hipHostMallocDefault));
-    FR_CHECK(hipEventCreateWithFlags(&plan_ev_, hipEventDisableTiming));
+    for (int k = 0; k < FR_PLAN_RING; ++k)
+        FR_CHECK(hipEventCreateWithFlags(&plan_ev_[k], hipEventDisableTiming));
     for (auto &e : int_ev_)
         FR_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const long max_tiles = queue_cap / FR_TILE + max_slots + 1;
@@ -310,7 +312,8 @@ FrEngine::~FrEngine()
     (void)hipFree(d_scratch_);
     (void)hipHostFree(h_plan_stage_);
     (void)hipHostFree(h_int_stage_);
-    (void)hipEventDestroy(plan_ev_);
+    for (int k = 0; k < FR_PLAN_RING; ++k)
+        (void)hipEventDestroy(plan_ev_[k]);
     for (auto e : int_ev_)
         (void)hipEventDestroy(e);
     (void)hipFree(L_.pairs);
@@ -396,10 +399,13 @@ void FrEngine::bind(int slot, FrSlab &s, const FrPlanParams &p, const FrFrameVie
 
 void FrEngine::upload_plans()
 {
-    FR_CHECK(hipEventSynchronize(plan_ev_)); /* the previous copy has read the staging */
-    std::memcpy(h_plan_stage_, h_plans_.data(), sizeof(FrPlanDev) * max_slots_);
-    FR_CHECK(hipMemcpyAsync(L_.plans, h_plan_stage_, sizeof(FrPlanDev) * max_slots_, hipMemcpyHostToDevice, st_));
-    FR_CHECK(hipEventRecord(plan_ev_, st_));
+    const int k = plan_next_;
+    plan_next_ = (plan_next_ + 1) % FR_PLAN_RING;
+    FrPlanDev *stage = h_plan_stage_ + (size_t)k * max_slots_;
+    FR_CHECK(hipEventSynchronize(plan_ev_[k])); /* that ring entry's copy has read it */
+    std::memcpy(stage, h_plans_.data(), sizeof(FrPlanDev) * max_slots_);
+    FR_CHECK(hipMemcpyAsync(L_.plans, stage, sizeof(FrPlanDev) * max_slots_, hipMemcpyHostToDevice, st_));
+    FR_CHECK(hipEventRecord(plan_ev_[k], st_));
 }
 
 void FrEngine::upload_ints(int *d_dst, const int *src, int n)

@@ -266,8 +266,12 @@ class FrEngine {
     bool own_stream_;
     FrLaunch L_{};
     std::vector<FrPlanDev> h_plans_;
-    FrPlanDev *h_plan_stage_ = nullptr;   /* pinned */
-    hipEvent_t plan_ev_ = nullptr;
+    /* pinned ring of FR_PLAN_RING descriptor tables, so that an upload
+     * waits only for the copy FR_PLAN_RING uploads back */
+    static const int FR_PLAN_RING = 4;
+    FrPlanDev *h_plan_stage_ = nullptr;
+    hipEvent_t plan_ev_[FR_PLAN_RING] = {};
+    int plan_next_ = 0;
     /* pinned ring of FR_STAGE_RING int arrays of max_slots + 1 entries */
     static const int FR_STAGE_RING = 8;
     int *h_int_stage_ = nullptr;
