@@ -72,7 +72,7 @@ SEED = 1234
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=168, help="plans per step")
     ap.add_argument("--budget-ms", type=float, default=20.0, help="per-plan latency budget (batched deadline)")
@@ -82,7 +82,7 @@ def parse():
     ap.add_argument("--max-iter", type=int, default=200)
     ap.add_argument("--opt-rounds", type=int, default=20)
     ap.add_argument("--plain-plans", type=int, default=-1, help="plans of the first timed step run plain (-1: all)")
-    ap.add_argument("--plain-steps", type=int, default=3,
+    ap.add_argument("--plain-steps", type=int, default=8,
                     help="timed steps whose plans also run plain (success compared plan by plan)")
     ap.add_argument("--cpu-sample", type=int, default=2, help="plans timed on the CPU oracle, one thread (0 = skip)")
     ap.add_argument("--cpu-workers", type=int, default=16, help="processes of the all-core CPU oracle run")
