@@ -328,7 +328,7 @@ def main():
     if args.plain_timeout_ms < 0:
         args.plain_timeout_ms = max(1, int(args.budget_ms) - 2)
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
               "(plans are independent: more GPUs would shard requests)", file=sys.stderr)
         sys.exit(2)
     from fastrrt_amd import scenes

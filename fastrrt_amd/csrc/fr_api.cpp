@@ -394,7 +394,7 @@ class FrGraph {
     }
 
     /* graph_expansion.cu:246-261 (the frame's class costs; the footprint and
-     * ego box are the graph's, as this build's frame handle carries no search
+     * ego box are the graph's, as the frame handle here carries no search
      * parameters).  first: the goal pixel is an obstacle; second: the
      * curve's result (cost, -1, -2). */
     std::pair<bool, float> can_connect_hermite(const FrFrame *frame, int x, int z, int gx, int gz, int goal_heading)

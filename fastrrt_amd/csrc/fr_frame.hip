@@ -6,7 +6,7 @@
  * (called by unittests/src/test_graph_can_connect_hermite.cpp:51 before
  * canConnectToGoal, which reads that channel as the Hermite curve's length,
  * graph_hermite_connection.cu:20).  libdriveless is not part of the
- * reference tree; this build defines the channel as the Euclidean pixel
+ * reference tree; here the channel is the Euclidean pixel
  * distance to the goal, sqrtf of the integer squared distance, which is the
  * length the loop's own goal connection computes when the channel is not
  * processed (graph_direct_goal_connection.cu:29-36).

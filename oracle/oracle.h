@@ -1,7 +1,7 @@
 /*
  * oracle.h — CPU restatement of FastRRT's parallel planning loop.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker the parity tests,
+ * TEST INFRASTRUCTURE ONLY.  This library is what the parity tests,
  * the smoke check and the bench's CPU-baseline leg compare the gfx950 path
  * against.  The product (libfastrrt.so and the fastrrt_amd package) never
  * links, loads or calls it.
