@@ -134,23 +134,35 @@ class FrGraph {
     {
         if (W <= 0 || H <= 0 || (long)W * H > (1L << 28))
             throw std::runtime_error("invalid grid size");
-        slab.allocate(W, H, 0, 0, 0, true, eng.stream());
-        prm.W = W;
-        prm.H = H;
-        /* coordinate origin at the grid centre (graph.cu:258-261,279) */
-        prm.og_x = (float)(W / 2);
-        prm.og_y = (float)(H / 2);
-        FR_CHECK(hipMalloc(&d_tmp, 64 * sizeof(int)));
-        FR_CHECK(hipMalloc(&d_region, sizeof(uint32_t) * ((W / 16 + 1) * (H / 16 + 1) + 1)));
-        FR_CHECK(hipMemsetAsync(d_tmp, 0, 64 * sizeof(int), eng.stream()));
-        rebind();
-        std::vector<int> one{0};
-        eng.set_active(one);
-        eng.reset_ctl(0, FR_IDLE);
+        try {
+            slab.allocate(W, H, 0, 0, 0, true, eng.stream());
+            prm.W = W;
+            prm.H = H;
+            /* coordinate origin at the grid centre (graph.cu:258-261,279) */
+            prm.og_x = (float)(W / 2);
+            prm.og_y = (float)(H / 2);
+            FR_CHECK(hipMalloc(&d_tmp, 64 * sizeof(int)));
+            FR_CHECK(hipMalloc(&d_region, sizeof(uint32_t) * ((W / 16 + 1) * (H / 16 + 1) + 1)));
+            FR_CHECK(hipMemsetAsync(d_tmp, 0, 64 * sizeof(int), eng.stream()));
+            rebind();
+            std::vector<int> one{0};
+            eng.set_active(one);
+            eng.reset_ctl(0, FR_IDLE);
+        } catch (...) {
+            /* a half-built graph runs no destructor */
+            (void)hipStreamSynchronize(eng.stream());
+            free_buffers();
+            throw;
+        }
     }
     ~FrGraph()
     {
         eng.sync();
+        free_buffers();
+    }
+
+    void free_buffers()
+    {
         if (d_tmp)
             (void)hipFree(d_tmp);
         if (d_region)
@@ -161,6 +173,11 @@ class FrGraph {
             (void)hipFree(d_apf_bits_);
         if (d_apf_tab_)
             (void)hipFree(d_apf_tab_);
+        d_tmp = nullptr;
+        d_region = nullptr;
+        d_graph_costs = nullptr;
+        d_apf_bits_ = nullptr;
+        d_apf_tab_ = nullptr;
     }
 
     int W() const { return prm.W; }

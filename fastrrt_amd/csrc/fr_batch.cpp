@@ -78,6 +78,28 @@ class FrBatch {
     {
         if (max_plans <= 0 || max_cells <= 0)
             throw std::runtime_error("invalid batch capacity");
+        try {
+            allocate(max_plans);
+        } catch (...) {
+            /* a half-built batch runs no destructor */
+            (void)hipStreamSynchronize(eng_.stream());
+            release();
+            throw;
+        }
+        t0_ = now_ms();
+    }
+
+    ~FrBatch()
+    {
+        (void)hipStreamSynchronize(eng_.stream());
+        if (st2_)
+            (void)hipStreamSynchronize(st2_);
+        release();
+    }
+
+  private:
+    void allocate(int max_plans)
+    {
         for (int b = 0; b < 2; ++b) {
             FR_CHECK(hipHostMalloc((void **)&h_snap_[b], sizeof(FrCtl) * max_plans, hipHostMallocDefault));
             std::memset(h_snap_[b], 0, sizeof(FrCtl) * max_plans);
@@ -96,31 +118,48 @@ class FrBatch {
         FR_CHECK(hipEventCreateWithFlags(&fin_ev_, hipEventDisableTiming));
         eng_.upload_plans(); /* all-zero descriptors, including the padding plan's */
         FR_CHECK(hipStreamSynchronize(eng_.stream()));
-        t0_ = now_ms();
     }
 
-    ~FrBatch()
+    void release()
     {
-        (void)hipStreamSynchronize(eng_.stream());
-        (void)hipStreamSynchronize(st2_);
         drop_graphs();
         if (fin_exec_)
             (void)hipGraphExecDestroy(fin_exec_);
         if (fin_graph_)
             (void)hipGraphDestroy(fin_graph_);
-        (void)hipEventDestroy(fin_ev_);
-        (void)hipStreamDestroy(st2_);
+        if (fin_ev_)
+            (void)hipEventDestroy(fin_ev_);
+        if (st2_)
+            (void)hipStreamDestroy(st2_);
         for (int b = 0; b < 2; ++b) {
-            (void)hipHostFree(h_snap_[b]);
-            (void)hipEventDestroy(snap_ev_[b]);
+            if (h_snap_[b])
+                (void)hipHostFree(h_snap_[b]);
+            if (snap_ev_[b])
+                (void)hipEventDestroy(snap_ev_[b]);
         }
-        (void)hipHostFree(h_fin_list_);
-        (void)hipHostFree(h_idle_);
-        (void)hipHostFree(h_pack_);
-        (void)hipFree(d_list_);
-        (void)hipFree(fin_list_);
+        if (h_fin_list_)
+            (void)hipHostFree(h_fin_list_);
+        if (h_idle_)
+            (void)hipHostFree(h_idle_);
+        if (h_pack_)
+            (void)hipHostFree(h_pack_);
+        if (d_list_)
+            (void)hipFree(d_list_);
+        if (fin_list_)
+            (void)hipFree(fin_list_);
+        fin_exec_ = nullptr;
+        fin_graph_ = nullptr;
+        fin_ev_ = nullptr;
+        st2_ = nullptr;
+        h_snap_[0] = h_snap_[1] = nullptr;
+        snap_ev_[0] = snap_ev_[1] = nullptr;
+        h_fin_list_ = nullptr;
+        h_idle_ = nullptr;
+        h_pack_ = d_pack_ = nullptr;
+        d_list_ = fin_list_ = nullptr;
     }
 
+  public:
     int submit(const fastrrt_plan_request &r)
     {
         const FrFrame *f = static_cast<const FrFrame *>(r.frame);

@@ -248,6 +248,21 @@ FrFrame::~FrFrame()
 FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
     : max_slots_(max_slots), queue_cap_(queue_cap), st_(stream), own_stream_(false)
 {
+    try {
+        allocate();
+    } catch (...) {
+        /* a half-built engine runs no destructor */
+        if (st_)
+            (void)hipStreamSynchronize(st_);
+        release();
+        throw;
+    }
+}
+
+void FrEngine::allocate()
+{
+    const int max_slots = max_slots_;
+    const long queue_cap = queue_cap_;
     if (!st_) {
         FR_CHECK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
         own_stream_ = true;
@@ -298,13 +313,22 @@ FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
 
 FrEngine::~FrEngine()
 {
-    (void)hipStreamSynchronize(st_);
+    if (st_)
+        (void)hipStreamSynchronize(st_);
+    release();
+}
+
+/* Frees what allocate() made (hipFree / hipHostFree of null are no-ops);
+ * the events and the owned stream only when created. */
+void FrEngine::release()
+{
     (void)hipFree(L_.plans);
     (void)hipFree(L_.ctl);
     (void)hipFree(d_act_);
     (void)hipHostFree(h_act_stage_);
     for (auto e : act_ev_)
-        (void)hipEventDestroy(e);
+        if (e)
+            (void)hipEventDestroy(e);
     (void)hipFree(d_setup_slots_);
     (void)hipFree(L_.counters);
     (void)hipFree(L_.exp_q);
@@ -313,14 +337,16 @@ FrEngine::~FrEngine()
     (void)hipHostFree(h_plan_stage_);
     (void)hipHostFree(h_int_stage_);
     for (int k = 0; k < FR_PLAN_RING; ++k)
-        (void)hipEventDestroy(plan_ev_[k]);
+        if (plan_ev_[k])
+            (void)hipEventDestroy(plan_ev_[k]);
     for (auto e : int_ev_)
-        (void)hipEventDestroy(e);
+        if (e)
+            (void)hipEventDestroy(e);
     (void)hipFree(L_.pairs);
     (void)hipFree(L_.demand);
     (void)hipFree(L_.demand2);
     (void)hipFree(L_.demand2_geo);
-    if (own_stream_)
+    if (own_stream_ && st_)
         (void)hipStreamDestroy(st_);
 }
 

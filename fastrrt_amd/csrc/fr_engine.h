@@ -260,6 +260,8 @@ class FrEngine {
     int *d_scratch_int() { return d_scratch_; }
 
   private:
+    void allocate();
+    void release();
     int max_slots_;
     long queue_cap_;
     hipStream_t st_;

@@ -61,3 +61,11 @@ def test_library_reports_missing_gpu_loudly():
     h = lib.fastrrt_initialize(64, 64, 10.0, 10.0, 40.0, 5.4, -1, -1, -1, -1, None, -1.0)
     assert not h
     assert _lib.last_error() != ""
+    # the graph and batch constructors fail part-way and free what they made
+    for _ in range(3):
+        g = lib.cudagraph_initialize(64, 64, 10.0, 10.0, 40.0, 5.4, 22, 40, -1, -1, -1, -1, None, -1.0)
+        assert not g
+        assert _lib.last_error() != ""
+        b = lib.fastrrt_batch_create(4, 4 * 64 * 64)
+        assert not b
+        assert _lib.last_error() != ""
