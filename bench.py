@@ -350,7 +350,7 @@ def main():
         for pl in st:
             frame_of(pl.sc)
     B = len(steps[0])
-    batch = Batch(max_plans=B, max_cells=B * 256 * 256)
+    batch = Batch(max_plans=B, max_cells=B * 256 * 256, launch="auto")
     first = steps[args.warmup]
     if args.roofline_only:
         # warm-up on the same plans: both passes are the same deterministic work
@@ -417,7 +417,8 @@ def main():
                                "steer 40 deg, L=5.41; loop to goal, then path_optimize rounds",
                    "mode": args.mode, "plans_per_step": B, "budget_ms": args.budget_ms,
                    "plain_timeout_ms": args.plain_timeout_ms,
-                   "max_iter": args.max_iter, "opt_rounds": args.opt_rounds},
+                   "max_iter": args.max_iter, "opt_rounds": args.opt_rounds,
+                   "launch_grid_waves_tiles_items": list(batch.launch())},
         "p50_ms": round(batched["p50_ms"], 3), "p99_ms": round(batched["p99_ms"], 3),
         "success": round(batched["success"], 4),
         "batched": batched, "plain": plain,

@@ -123,6 +123,8 @@ SIGNATURES.update({
     "fastrrt_batch_poll": (c_bool, [c_void_p, ctypes.POINTER(PlanResult), fp, c_int]),
     "fastrrt_batch_drain": (None, [c_void_p]),
     "fastrrt_batch_clock_ms": (c_double, [c_void_p]),
+    "fastrrt_batch_set_launch": (c_bool, [c_void_p, c_int, c_int, c_int]),
+    "fastrrt_batch_get_launch": (c_bool, [c_void_p, ctypes.POINTER(c_int)]),
 })
 
 _lib = None

@@ -1,0 +1,174 @@
+"""Launch-shape autotuner for the batched iteration graph.
+
+The iteration graph's kernels take three launch shapes, in blocks of 256
+threads (`Batch.set_launch`): `grid_waves` for the one-wave-per-item kernels
+(expansion rollouts, goal-plane curves, goal connection; a multiple of 8,
+one share per XCD), `grid_tiles` for the 1024-cell tile scans and
+`grid_items` for the one-lane-per-item kernels.  Every kernel strides over
+its work, so a shape changes the time of an iteration, never its result.
+
+For each grid side the tuner builds a batch of seeded scenes (the 256 px
+entry uses the 5/15/30 % rectangle sweep of the default benchmark, the others
+`scenes.scene_sized`), runs the same deterministic pass (a fixed number of
+loop iterations, no deadline, no optimisation rounds) once per candidate and
+times the iteration graph with HIP events (`fastrrt_profile_*`).  The search
+is coordinate descent: grid_waves, then grid_tiles, then grid_items, each
+over its candidate list with the others fixed at the best so far (a shape
+replaces the best only when it is at least 1 % faster); every candidate runs
+`repeats` times and keeps its median.  Every pass's per-plan
+results (status, iterations, expansions, path) must equal the first pass's;
+a difference raises, since it would mean a kernel depends on its launch shape.
+
+The per-iteration expansion budget and the LDS tile sizes are not tuned: the
+reference expands every eligible node each iteration, so a budget would
+change the search, and the LDS buffers (rollout cell buffer, in-LDS erase
+capacity) are sized by the kernels' own structure.
+
+    python -m fastrrt_amd.autotune --sizes 128 256 384 512 --out table.json
+
+`lookup(side)` returns the tuned shape of the nearest tuned side from the
+table shipped in `fastrrt_amd/tuning/`.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import _lib, scenes
+from .batch import Batch, PlanSpec
+from .pyfastrrt import SearchFrame
+
+WAVES = (1024, 2048, 4096, 8192)
+TILES = (512, 1024, 2048, 4096)
+ITEMS = (256, 1024, 2048)
+# plans per batch by grid side: about 11 M cells, as the 168-plan 256 px batch
+PLANS = {128: 512, 256: 168, 384: 72, 512: 40}
+TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "launch_mi355x.json")
+
+
+def valid_shape(shape):
+    w, t, i = (int(v) for v in shape)
+    return 8 <= w <= 65535 and w % 8 == 0 and 1 <= t <= 65535 and 1 <= i <= 65535
+
+
+def nearest(entries, side):
+    """The entry whose side is nearest `side` on a log scale (None if empty)."""
+    best, bd = None, None
+    for e in entries:
+        d = abs(math.log(float(e["side"])) - math.log(float(side)))
+        if bd is None or d < bd:
+            best, bd = e, d
+    return best
+
+
+def lookup(side, path=TABLE):
+    """Tuned (grid_waves, grid_tiles, grid_items) for plans of `side` px, or
+    None when no table is present."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        tab = json.load(fh)
+    e = nearest(tab.get("entries", []), side)
+    if e is None:
+        return None
+    shape = tuple(e["best"])
+    if not valid_shape(shape):
+        raise ValueError(f"invalid launch shape {shape} in {path}")
+    return shape
+
+
+def _scenes(side, n, seed):
+    if side == 256:
+        per = max(1, n // 3)
+        pool = scenes.scenes_256_sweep(per, base_seed=seed)
+        return pool[:n]
+    rng = np.random.default_rng(seed)
+    return [scenes.scene_sized(int(seed * 1000 + k), side, float(rng.uniform(0.05, 0.30))) for k in range(n)]
+
+
+def _pass(batch, plans, iters):
+    """One deterministic pass; returns ({tag: result key}, ms per iteration)."""
+    L = _lib.load()
+    for tag, (sc, frame) in enumerate(plans):
+        batch.submit(PlanSpec(frame, sc.start, sc.goal, min_dist=sc.min_dist, ego_lb=sc.ego_lb, ego_ub=sc.ego_ub,
+                              max_iterations=iters, optimize_rounds=0, deadline_ms=0.0, seed=1, stream=900000 + tag,
+                              tag=tag))
+    L.fastrrt_profile_reset()
+    L.fastrrt_profile_enable(True)
+    res = batch.drain()
+    L.fastrrt_profile_enable(False)
+    n, ms = ctypes.c_long(), ctypes.c_double()
+    L.fastrrt_profile_read(3, ctypes.byref(n), ctypes.byref(ms))
+    if len(res) != len(plans) or n.value == 0:
+        raise RuntimeError("autotune pass did not complete")
+    keys = {r.tag: (r.status, r.iterations, r.expansions, r.path.tobytes()) for r in res}
+    return keys, ms.value / n.value
+
+
+def tune_side(side, n_plans=None, iters=24, repeats=3, seed=4242, log=print):
+    n_plans = n_plans or PLANS.get(side, max(8, (11 << 20) // (side * side)))
+    plans = [(sc, SearchFrame(sc.frame, sc.class_costs)) for sc in _scenes(side, n_plans, seed)]
+    batch = Batch(max_plans=len(plans), max_cells=len(plans) * side * side)
+    ref = None
+    times = {}
+
+    def measure(shape):
+        nonlocal ref
+        if shape in times:
+            return times[shape]
+        batch.set_launch(*shape)
+        ms = []
+        for _ in range(repeats):
+            keys, t = _pass(batch, plans, iters)
+            if ref is None:
+                ref = keys
+            elif keys != ref:
+                bad = sorted(k for k in ref if keys.get(k) != ref[k])[:5]
+                raise RuntimeError(f"launch shape {shape} changed the results of plans {bad}")
+            ms.append(t)
+        times[shape] = float(np.median(ms))
+        log(f"[autotune] side {side} plans {len(plans)} shape {shape}: {times[shape]:.4f} ms per iteration")
+        return times[shape]
+
+    default = batch.launch()
+    best = default
+    measure(best)
+    for axis, cands in ((0, WAVES), (1, TILES), (2, ITEMS)):
+        for c in cands:
+            shape = list(best)
+            shape[axis] = c
+            shape = tuple(shape)
+            if measure(shape) < 0.99 * times[best]:  # a gain under 1 % is noise
+                best = shape
+    return {"side": side, "plans": len(plans), "iterations": iters, "repeats": repeats, "best": list(best),
+            "best_ms": round(times[best], 5), "default": list(default), "default_ms": round(times[default], 5),
+            "measured": [{"shape": list(k), "ms": round(v, 5)} for k, v in sorted(times.items())]}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--sizes", type=int, nargs="+", default=[128, 256, 384, 512])
+    ap.add_argument("--iters", type=int, default=24)
+    ap.add_argument("--repeats", type=int, default=3)
+    ap.add_argument("--out", default=TABLE)
+    a = ap.parse_args()
+    t0 = time.time()
+    entries = []
+    for side in a.sizes:
+        entries.append(tune_side(side, iters=a.iters, repeats=a.repeats, log=lambda m: print(m, flush=True)))
+    tab = {"device": "MI355X (gfx950)", "unit": "ms per iteration graph launch (HIP events)",
+           "candidates": {"grid_waves": list(WAVES), "grid_tiles": list(TILES), "grid_items": list(ITEMS)},
+           "entries": entries, "seconds": round(time.time() - t0, 1)}
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    with open(a.out, "w") as fh:
+        json.dump(tab, fh, indent=1)
+    print(json.dumps({e["side"]: {"best": e["best"], "best_ms": e["best_ms"], "default_ms": e["default_ms"]}
+                      for e in entries}))
+
+
+if __name__ == "__main__":
+    main()

@@ -62,7 +62,12 @@ class Result:
 
 
 class Batch:
-    def __init__(self, max_plans=256, max_cells=256 * 256 * 256, path_cap=65536):
+    """launch: None keeps the engine's launch shapes; "auto" applies the
+    tuned shapes (fastrrt_amd.autotune) of the grid side max_cells/max_plans
+    corresponds to; a (grid_waves, grid_tiles, grid_items) tuple sets them.
+    Results do not depend on the choice."""
+
+    def __init__(self, max_plans=256, max_cells=256 * 256 * 256, path_cap=65536, launch=None):
         L = _lib.load()
         self._ptr = L.fastrrt_batch_create(int(max_plans), int(max_cells))
         if not self._ptr:
@@ -70,6 +75,11 @@ class Batch:
         self._keep = {}
         self._buf = np.zeros(3 * path_cap, np.float32)
         self._cap = path_cap
+        if launch == "auto":
+            from . import autotune
+            launch = autotune.lookup(math.sqrt(max(1, int(max_cells)) / max(1, int(max_plans))))
+        if launch is not None:
+            self.set_launch(*launch)
 
     def __del__(self):
         try:
@@ -114,3 +124,16 @@ class Batch:
 
     def clock_ms(self) -> float:
         return _lib.load().fastrrt_batch_clock_ms(self._ptr)
+
+    def set_launch(self, grid_waves, grid_tiles, grid_items):
+        """Launch shapes of the iteration graph, in blocks of 256 threads (the
+        results do not depend on them; fastrrt_amd.autotune measures them)."""
+        if not _lib.load().fastrrt_batch_set_launch(self._ptr, int(grid_waves), int(grid_tiles), int(grid_items)):
+            _lib.check("batch_set_launch")
+
+    def launch(self):
+        """(grid_waves, grid_tiles, grid_items) of the iteration graph"""
+        out = (ctypes.c_int * 3)()
+        if not _lib.load().fastrrt_batch_get_launch(self._ptr, out):
+            _lib.check("batch_get_launch")
+        return tuple(int(v) for v in out)

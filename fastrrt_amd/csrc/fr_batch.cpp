@@ -179,6 +179,23 @@ class FrBatch {
 
     int in_flight() const { return (int)queue_.size() + running_; }
 
+    /* New launch shapes for the iteration graph: the queued work drains, the
+     * graphs are captured again at the next step. */
+    void set_launch(int waves, int tiles, int items)
+    {
+        FR_CHECK(hipStreamSynchronize(eng_.stream()));
+        eng_.set_grids(waves, tiles, items);
+        drop_graphs();
+    }
+
+    void get_launch(int *out) const
+    {
+        const FrLaunch &L = eng_.launch();
+        out[0] = L.grid_waves;
+        out[1] = L.grid_tiles;
+        out[2] = L.grid_items;
+    }
+
     /* admit -> `iterations` loop iterations -> retire.  The host works one
      * step behind the device: after queueing this step's iterations and a
      * copy of the control blocks, it acts on the previous step's copy while
@@ -723,5 +740,23 @@ void fastrrt_batch_drain(void *batch)
 }
 
 double fastrrt_batch_clock_ms(void *batch) { FR_TRY(-1.0, { return B(batch)->clock_ms(); }) }
+
+bool fastrrt_batch_set_launch(void *batch, int grid_waves, int grid_tiles, int grid_items)
+{
+    FR_TRY(false, {
+        B(batch)->set_launch(grid_waves, grid_tiles, grid_items);
+        return true;
+    })
+}
+
+bool fastrrt_batch_get_launch(void *batch, int *grid_waves_tiles_items)
+{
+    FR_TRY(false, {
+        if (!grid_waves_tiles_items)
+            throw std::runtime_error("null output");
+        B(batch)->get_launch(grid_waves_tiles_items);
+        return true;
+    })
+}
 
 } /* extern "C" */

@@ -350,6 +350,18 @@ void FrEngine::release()
         (void)hipStreamDestroy(st_);
 }
 
+void FrEngine::set_grids(int waves, int tiles, int items)
+{
+    /* k_expand deals its queue to the eight XCDs in blocks of 8 */
+    if (waves < 8 || waves > 65535 || waves % 8 != 0)
+        throw std::runtime_error("grid_waves must be a multiple of 8 in [8, 65535]");
+    if (tiles < 1 || tiles > 65535 || items < 1 || items > 65535)
+        throw std::runtime_error("grid_tiles and grid_items must be in [1, 65535]");
+    L_.grid_waves = waves;
+    L_.grid_tiles = tiles;
+    L_.grid_items = items;
+}
+
 void FrEngine::bind(int slot, FrSlab &s, const FrPlanParams &p, const FrFrameView &frame, bool upload)
 {
     FrPlanDev d{};

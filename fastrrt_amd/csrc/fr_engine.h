@@ -231,6 +231,11 @@ class FrEngine {
 
     hipStream_t stream() const { return st_; }
     const FrLaunch &launch() const { return L_; }
+    /* Launch shapes (blocks of FR_BLOCK threads) of the iteration's kernels:
+     * the one-wave-per-item kernels, the tile scans, the one-lane-per-item
+     * kernels.  Every one of them strides over its work, so the results do
+     * not depend on these; a caller re-captures its graphs after a change. */
+    void set_grids(int waves, int tiles, int items);
     int max_slots() const { return max_slots_; }
 
     /* describe plan `slot` (slab + params + frame) and upload its descriptor */

@@ -66,7 +66,7 @@ def run_served(duration_s=10.0, rate=2000.0, sizes=SERVED_SIZES, deadlines=DEADL
                max_plans=256, max_cells=64 << 20, max_iter=200, opt_rounds=20, pool=None):
     rng = np.random.default_rng(seed)
     pool = pool or ScenePool(sizes, per_size, seed)
-    b = Batch(max_plans=max_plans, max_cells=max_cells)
+    b = Batch(max_plans=max_plans, max_cells=max_cells, launch="auto")
     # arrival schedule (ms from the start of the stream)
     arrivals, t = [], 0.0
     while True:
@@ -130,7 +130,7 @@ def run_batched(n_plans=512, sizes=BATCHED_SIZES, seed=12, max_iter=200, opt_rou
         sc = scenes.scene_sized(int(seed * 100000 + i), size, cov, min_dist=min_dist)
         plans.append((size, sc, SearchFrame(sc.frame, sc.class_costs)))
     cells = sum(s * s for s, _, _ in plans)
-    b = Batch(max_plans=n_plans, max_cells=max_cells or cells)
+    b = Batch(max_plans=n_plans, max_cells=max_cells or cells, launch="auto")
     t0 = time.perf_counter()
     for i, (size, sc, frame) in enumerate(plans):
         b.submit(_spec(sc, frame, i, 700000 + i, deadline_ms, max_iter, opt_rounds, seed))
@@ -202,7 +202,7 @@ def run_1024(n_plans=24, seed=13, timeout_ms=300.0, smart=True, region_side=32, 
                         "mean_iterations": float(np.mean(iters)),
                         "iteration_ms": loop_ms / max(1, sum(iters))}
     if "batched" in forms:
-        b = Batch(max_plans=n_plans, max_cells=n_plans * 1024 * 1024)
+        b = Batch(max_plans=n_plans, max_cells=n_plans * 1024 * 1024, launch="auto")
         res = None
         for rep in range(2):  # the first pass allocates the slabs
             t0 = time.perf_counter()

@@ -264,6 +264,17 @@ bool fastrrt_batch_poll(void *batch, fastrrt_plan_result *result, float *path, i
 void fastrrt_batch_drain(void *batch);
 /* Milliseconds since the batch was created (the clock of submit_ms/done_ms). */
 double fastrrt_batch_clock_ms(void *batch);
+/* Launch shapes, in blocks of 256 threads, of the batched iteration's
+ * kernels: grid_waves for the one-wave-per-item kernels (expansion rollouts,
+ * goal-plane curves, goal connection; a multiple of 8, one share per XCD),
+ * grid_tiles for the 1024-cell tile scans, grid_items for the
+ * one-lane-per-item kernels.  Results do not depend on them.  The defaults
+ * are 4096, and for grid_tiles and grid_items the batch's tile and queue
+ * capacity up to 2048; fastrrt_amd.autotune measures them per grid size.
+ * No counterpart in the reference, which launches one thread per cell.
+ * get_launch writes the three values in that order. */
+bool fastrrt_batch_set_launch(void *batch, int grid_waves, int grid_tiles, int grid_items);
+bool fastrrt_batch_get_launch(void *batch, int *grid_waves_tiles_items);
 
 #ifdef __cplusplus
 }
