@@ -481,7 +481,7 @@ class FrBatch {
                 found.push_back(i);
             else if (st == FR_FAILED)
                 failed.push_back(i);
-            else if (dl > 0 && t - slots_[i].p.t_submit + 2.0 * poll_ema_ + 0.2 > dl)
+            else if (dl > 0 && t - slots_[i].p.t_submit + 2.0 * poll_ema_ + 0.3 > dl)
                 failed.push_back(i); /* with a margin for a slower next iteration */
         }
         for (int sl : failed)
