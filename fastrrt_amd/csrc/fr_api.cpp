@@ -1147,7 +1147,11 @@ float *interpolate_planned_path_p(void *ptr, float *p, int size)
     FR_TRY(nullptr, {
         FrFastRRT *r = R(ptr);
         std::vector<Waypoint> pref;
-        for (int i = 0; i + 2 < size + 2 && i < size; i += 3)
+        if (size < 0 || (size > 0 && !p))
+            throw std::runtime_error("interpolate_planned_path_p: invalid buffer");
+        /* whole (x, z, heading) triples only: a trailing partial triple is
+         * never read past the caller's `size` floats */
+        for (int i = 0; i + 2 < size; i += 3)
             pref.push_back({(int)p[i], (int)p[i + 1], p[i + 2]});
         return convert_path(interpolate_path(pref, r->g.W(), r->g.H()));
     })
