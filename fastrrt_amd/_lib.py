@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libfastrrt.so")
 
 c_int, c_float, c_bool, c_void_p = ctypes.c_int, ctypes.c_float, ctypes.c_bool, ctypes.c_void_p
 c_char_p, c_u64, c_i64, c_double = ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int64, ctypes.c_double
+c_size_t = ctypes.c_size_t
 fp = ctypes.POINTER(ctypes.c_float)
 ip = ctypes.POINTER(ctypes.c_int)
 i32p = ctypes.POINTER(ctypes.c_int32)
@@ -60,6 +61,9 @@ SIGNATURES = {
     "can_connect_to_goal_using_hermite": (c_bool, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_float]),
     "cudagraph_can_connect_hermite": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_float, fp]),
     "compute_apf_repulsion": (None, [c_void_p, c_void_p, c_float, c_int]),
+    "compute_apf_repulsion_noalloc": (c_bool, [c_void_p, c_void_p, c_float, c_int]),
+    "fastrrt_apf_noalloc_scratch_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "fastrrt_apf_noalloc_scratch_capacity": (c_size_t, []),
     "compute_apf_attraction": (None, [c_void_p, c_void_p, c_float, c_int, c_int]),
     "get_intrinsic_costs": (fp, [c_void_p]),
     "destroy_intrinsic_costs_ptr": (None, [fp]),

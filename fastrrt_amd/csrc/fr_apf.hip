@@ -26,6 +26,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
 #include "fr_engine.h"
 #include "fr_math.h"
 
@@ -184,4 +188,79 @@ void fr_launch_apf_attraction(const FrApfArgs &a, float ka_half, int gx, int gz,
     const long b = (n + 255) / 256;
     hipLaunchKernelGGL(k_apf_attraction, dim3((unsigned)(b < 8192 ? b : 8192)), dim3(256), 0, st, a, ka_half, gx,
                        gz);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Allocation-free repulsion: the same three kernels, their scratch (edge   */
+/* bit plane, offset table, disc spans) carved from one file-scope device   */
+/* array instead of hipMalloc'd buffers.                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Each GPU holds its own copy of this array until the process ends.  Host
+ * code never passes its name to a kernel: the name is a host-side stand-in,
+ * the device address comes from hipGetSymbolAddress (fr_apf_scratch_ptr). */
+static __device__ __align__(256) unsigned char scratch[FR_APF_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_lookup_mu;
+void *g_scratch_ptr[kMaxDevices];
+/* calls on one device share its array: they are serialised here, and a
+ * call returns only after its kernels have finished */
+std::mutex g_scratch_use_mu[kMaxDevices];
+
+size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+} // namespace
+
+size_t fr_apf_scratch_bytes(int W, int H, int radius)
+{
+    const size_t words = (size_t)H * (size_t)((W + 63) / 64);
+    const size_t w = (size_t)fr_apf_window(radius, W, H);
+    return round256(8 * words) + round256(4 * (w + 1) * (w + 1)) + round256(4 * (w + 1));
+}
+
+/* Device address of `scratch` on the current device, looked up once per
+ * device (the first call on it) with that device current. */
+static unsigned char *fr_apf_scratch_ptr(int *dev_out)
+{
+    int dev = -1;
+    FR_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        throw std::runtime_error("compute_apf_repulsion_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_lookup_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw std::runtime_error(std::string("compute_apf_repulsion_noalloc: hipGetSymbolAddress failed: ") +
+                                     hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *dev_out = dev;
+    return static_cast<unsigned char *>(g_scratch_ptr[dev]);
+}
+
+void fr_launch_apf_repulsion_noalloc(const FrApfArgs &a, float kr_half, int radius, hipStream_t st)
+{
+    const size_t need = fr_apf_scratch_bytes(a.W, a.H, radius);
+    if (need > FR_APF_SCRATCH_BYTES)
+        throw std::runtime_error("compute_apf_repulsion_noalloc: the edge plane and offset table need " +
+                                 std::to_string(need) + " bytes of scratch, more than its " +
+                                 std::to_string((size_t)FR_APF_SCRATCH_BYTES) +
+                                 " (use compute_apf_repulsion for this grid and radius)");
+    int dev = -1;
+    unsigned char *base = fr_apf_scratch_ptr(&dev);
+    const size_t words = (size_t)a.H * (size_t)((a.W + 63) / 64);
+    const size_t w = (size_t)fr_apf_window(radius, a.W, a.H);
+    uint64_t *bits = reinterpret_cast<uint64_t *>(base);
+    float *tab = reinterpret_cast<float *>(base + round256(8 * words));
+    int *span = reinterpret_cast<int *>(base + round256(8 * words) + round256(4 * (w + 1) * (w + 1)));
+    /* every word, table entry and span the gather reads is written by the
+     * two kernels before it, so nothing left by an earlier call is read */
+    std::lock_guard<std::mutex> lk(g_scratch_use_mu[dev]);
+    fr_launch_apf_repulsion(a, kr_half, radius, bits, tab, span, st);
+    FR_CHECK(hipGetLastError());
+    FR_CHECK(hipStreamSynchronize(st));
 }

@@ -709,6 +709,17 @@ class FrGraph {
         dirty();
     }
 
+    /* the repulsion above with its scratch in the library's static device
+     * array (fr_apf.hip): no hipMalloc/hipFree on any call */
+    void apf_repulsion_noalloc(const FrFrame *frame, float kr, int radius)
+    {
+        const FrApfArgs a = apf_args(frame);
+        if (fr_apf_window(radius, prm.W, prm.H) > FR_APF_MAX_WINDOW)
+            throw std::runtime_error("repulsion window beyond 2048 px (radius and grid both larger)");
+        fr_launch_apf_repulsion_noalloc(a, (float)(kr * 0.5), radius, st());
+        dirty();
+    }
+
     void apf_attraction(const FrFrame *frame, float ka, int gx, int gz)
     {
         const FrApfArgs a = apf_args(frame);
@@ -1239,6 +1250,21 @@ void compute_apf_repulsion(void *ptr, void *frame, float kr, int radius)
 {
     FR_TRY_VOID({ G(ptr)->apf_repulsion(F(frame), kr, radius); })
 }
+
+bool compute_apf_repulsion_noalloc(void *ptr, void *frame, float kr, int radius)
+{
+    FR_TRY(false, {
+        G(ptr)->apf_repulsion_noalloc(F(frame), kr, radius);
+        return true;
+    })
+}
+
+size_t fastrrt_apf_noalloc_scratch_bytes(int width, int height, int radius)
+{
+    FR_TRY((size_t)0, { return fr_apf_scratch_bytes(width, height, radius); })
+}
+
+size_t fastrrt_apf_noalloc_scratch_capacity(void) { return (size_t)FR_APF_SCRATCH_BYTES; }
 
 void compute_apf_attraction(void *ptr, void *frame, float ka, int goal_x, int goal_z)
 {

@@ -120,6 +120,13 @@ int fr_apf_window(int radius, int W, int H);
 void fr_launch_apf_repulsion(const FrApfArgs &a, float kr_half, int radius, uint64_t *d_bits, float *d_tab,
                              int *d_span, hipStream_t st);
 void fr_launch_apf_attraction(const FrApfArgs &a, float ka_half, int gx, int gz, hipStream_t st);
+/* The repulsion with its scratch in a file-scope device array of
+ * FR_APF_SCRATCH_BYTES per GPU (no allocation); throws when
+ * fr_apf_scratch_bytes(W, H, radius) exceeds it.  Synchronous: returns after
+ * the kernels have finished; calls on one device are serialised. */
+#define FR_APF_SCRATCH_BYTES (4u << 20)
+size_t fr_apf_scratch_bytes(int W, int H, int radius);
+void fr_launch_apf_repulsion_noalloc(const FrApfArgs &a, float kr_half, int radius, hipStream_t st);
 
 /* Optional per-kernel timing with HIP events recorded on the launching
  * stream around the kernels named below. */

@@ -342,6 +342,13 @@ class CudaGraph:
         _lib.load().compute_apf_repulsion(self._ptr, frame.get_cuda_ptr(), kr, radius)
         _lib.check("compute_apf_repulsion")
 
+    def compute_apf_repulsion_noalloc(self, frame: SearchFrame, kr: float, radius: int) -> None:
+        """compute_apf_repulsion with its scratch in the library's static
+        device array: allocates nothing; raises when the grid and radius need
+        more scratch than the array holds."""
+        if not _lib.load().compute_apf_repulsion_noalloc(self._ptr, frame.get_cuda_ptr(), kr, radius):
+            raise RuntimeError(f"compute_apf_repulsion_noalloc: {_lib.last_error()}")
+
     def compute_apf_attraction(self, frame: SearchFrame, ka: float, goal_x: int, goal_z: int) -> None:
         """Potential-field attraction into the intrinsic costs (cuda_graph.py:280-282)."""
         _lib.load().compute_apf_attraction(self._ptr, frame.get_cuda_ptr(), ka, goal_x, goal_z)

@@ -20,6 +20,7 @@
 #define FASTRRT_C_ABI_H
 
 #include <stdbool.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -131,6 +132,23 @@ void cudagraph_destroy(void *ptr);
  * sums its contributions in ascending source order; a window wider than
  * 2048 px (radius and grid side both larger) is refused (last error set) */
 void compute_apf_repulsion(void *ptr, void *frame, float kr, int radius);
+/* No counterpart in the reference: compute_apf_repulsion with the same
+ * kernels and the same result, for callers that cannot allocate device
+ * memory (code under HIP graph capture on another stream, tight serving
+ * loops, harnesses that forbid hipMalloc).  Its scratch (edge bit plane,
+ * offset table, disc spans) lives in a file-scope device array of
+ * fastrrt_apf_noalloc_scratch_capacity() bytes (4 MiB) per GPU, held until
+ * the process ends; the call allocates and frees nothing.  A grid and radius
+ * needing more (fastrrt_apf_noalloc_scratch_bytes: any radius up to
+ * 512x512, radius <= 1006 at 1024x1024, <= 956 at 2048x2048) are refused:
+ * false, with fastrrt_last_error() saying so.
+ * Synchronous.  Every call on one GPU shares that GPU's array: calls from
+ * several host threads on one GPU are serialised inside the library, and
+ * the array keeps the last call's contents (nothing in it is read before
+ * the call writes it). */
+bool compute_apf_repulsion_noalloc(void *ptr, void *frame, float kr, int radius);
+size_t fastrrt_apf_noalloc_scratch_bytes(int width, int height, int radius);
+size_t fastrrt_apf_noalloc_scratch_capacity(void);
 /* :48-53 (apf.cu:100-121,140-155): potential-field attraction */
 void compute_apf_attraction(void *ptr, void *frame, float ka, int goal_x, int goal_z);
 /* :55-73: W*H intrinsic costs */
