@@ -305,8 +305,10 @@ int og_feasible(const og_graph *g, const float *frame, int x, int z, float angle
 /* Kinematic rollout (src/cuda/kinematics.cu:123-231)                       */
 /* ---------------------------------------------------------------------- */
 
-void og_kinematic_new_path(og_graph *g, const float *frame, int sx, int sz, float steer, float path_size,
-                           float v, float out[4])
+/* pose (optional): the final FP32 map pose (x, y, heading) and the number of
+ * steps executed, for the FP64 recomputation check of the test suite */
+static void og_rollout(og_graph *g, const float *frame, int sx, int sz, float steer, float path_size, float v,
+                       float out[4], double pose[4])
 {
     g->n_rollouts++;
     out[0] = -1; out[1] = -1; out[2] = 0; out[3] = 0;
@@ -337,9 +339,11 @@ void og_kinematic_new_path(og_graph *g, const float *frame, int sx, int sz, floa
     float node_cost = g->cost[start_pos];
     int loop_count = 0;
     const int loop_max = fr_trunc_f((float)(100 * max_size) / dt);
+    int steps = 0;
     while (size < max_size) {
         if (loop_count++ > loop_max)
             break;
+        steps++;
         x += ds * fr_cosf(heading + beta);
         y += ds * fr_sinf(heading + beta);
         heading += hif;
@@ -356,10 +360,22 @@ void og_kinematic_new_path(og_graph *g, const float *frame, int sx, int sz, floa
         node_cost += g->intrinsic[(long)cz * W + cx] + 1.0f;
         if (!og_feasible(g, frame, cx, cz, heading)) {
             out[0] = -1; out[1] = -1; out[2] = 0; out[3] = 0;
+            if (pose) {
+                pose[0] = x;
+                pose[1] = y;
+                pose[2] = heading;
+                pose[3] = steps;
+            }
             return;
         }
         last_x = cx;
         last_z = cz;
+    }
+    if (pose) {
+        pose[0] = x;
+        pose[1] = y;
+        pose[2] = heading;
+        pose[3] = steps;
     }
     if (size < 1)
         return;
@@ -367,6 +383,18 @@ void og_kinematic_new_path(og_graph *g, const float *frame, int sx, int sz, floa
     out[1] = (float)last_z;
     out[2] = node_cost;
     out[3] = heading;
+}
+
+void og_kinematic_new_path(og_graph *g, const float *frame, int sx, int sz, float steer, float path_size,
+                           float v, float out[4])
+{
+    og_rollout(g, frame, sx, sz, steer, path_size, v, out, NULL);
+}
+
+void og_kinematic_new_path_pose(og_graph *g, const float *frame, int sx, int sz, float steer, float path_size,
+                                float v, float out[4], double pose[4])
+{
+    og_rollout(g, frame, sx, sz, steer, path_size, v, out, pose);
 }
 
 /* expand_node, graph_expansion.cu:103-145, host sequential semantics
@@ -673,6 +701,14 @@ float og_check_direct_connection(og_graph *g, const float *frame, float max_curv
 {
     return og_check_direct_connection_impl(g, frame, max_curv, x, z, local_heading, gx, gz, gh, safe_zone,
                                            dist_proc, 0);
+}
+
+/* the same check with the host's float->int truncation (NaN -> INT_MIN), as
+ * path_optimize runs it on the host (graph_optimization.cu:92-110) */
+float og_check_direct_connection_host(og_graph *g, const float *frame, float max_curv, int x, int z,
+                                      float local_heading, int gx, int gz, float gh, int safe_zone)
+{
+    return og_check_direct_connection_impl(g, frame, max_curv, x, z, local_heading, gx, gz, gh, safe_zone, 0, 1);
 }
 
 /* graph_hermite_connection.cu:11-136 (cudaHermite with fillInterpolation and

@@ -31,6 +31,17 @@
  *   gtests, the straight-connection closed form, the pinned direct-connection
  *   check on the same curves and the curvature measure from finite
  *   differences of the curve.
+ *   path optimisation (og_rrt_path_optimize): pinned by hand-worked chains
+ *   whose candidate costs are closed forms (straight connections on a free
+ *   frame: last improving j per i, first i of maximal gain, strict cost
+ *   comparison, cost update of the spliced node) and by an enumeration of
+ *   the selection rules over solved scenes (tests/test_oracle_pins.py).
+ *   goal-window argmin (og_find_best_goal_direct_connection): pinned by a
+ *   brute-force enumeration of the half-open window on small grids and by
+ *   known answers for the window edges, the floor of 100 x total and both
+ *   tie rules.
+ *   FP32 rollout: within 1e-5 relative of a float64 recomputation with libm
+ *   (og_kinematic_new_path_pose).
  */
 #ifndef FR_ORACLE_H
 #define FR_ORACLE_H
@@ -94,6 +105,10 @@ unsigned og_count_all(og_graph *g);
 int og_feasible(const og_graph *g, const float *frame, int x, int z, float angle);
 void og_kinematic_new_path(og_graph *g, const float *frame, int sx, int sz, float steer, float path_size,
                            float v, float out[4]);
+/* as og_kinematic_new_path, also returning the final FP32 map pose (x, y,
+ * heading) and the steps executed (test hook of the FP64 recomputation) */
+void og_kinematic_new_path_pose(og_graph *g, const float *frame, int sx, int sz, float steer, float path_size,
+                                float v, float out[4], double pose[4]);
 void og_derivate_node(og_graph *g, const float *frame, float steer, float path_size, float v, int x, int z,
                       int out[2]);
 void og_accept_derived_nodes(og_graph *g);
@@ -106,6 +121,8 @@ void og_expand(og_graph *g, const float *frame, float max_path, float v, int fro
 
 float og_check_direct_connection(og_graph *g, const float *frame, float max_curv, int x, int z,
                                  float local_heading, int gx, int gz, float gh, int safe_zone, int dist_proc);
+float og_check_direct_connection_host(og_graph *g, const float *frame, float max_curv, int x, int z,
+                                      float local_heading, int gx, int gz, float gh, int safe_zone);
 void og_process_direct_goal_connection(og_graph *g, const float *frame, int gx, int gz, float gh,
                                        float max_curv);
 int og_find_best_goal_direct_connection(og_graph *g, const float *frame, float radius, int safe_zone);

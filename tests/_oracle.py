@@ -10,7 +10,9 @@ import subprocess
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+# FR_ORACLE_LIB points the suite at another build of the oracle (e.g. a
+# deliberately broken copy, to check that the pins catch the mistake)
+LIB_PATH = os.environ.get("FR_ORACLE_LIB") or os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 
 NULL, NODE, TEMP, PROCESSING, COLLISION, CONNECT_TO_GOAL = range(6)
 
@@ -79,6 +81,8 @@ def lib():
         "og_count_all": (ctypes.c_uint, [P]),
         "og_feasible": (c_int, [P, fp, c_int, c_int, c_float]),
         "og_kinematic_new_path": (None, [P, fp, c_int, c_int, c_float, c_float, c_float, fp]),
+        "og_kinematic_new_path_pose": (None, [P, fp, c_int, c_int, c_float, c_float, c_float, fp,
+                                               ctypes.POINTER(c_double)]),
         "og_derivate_node": (None, [P, fp, c_float, c_float, c_float, c_int, c_int, ip]),
         "og_accept_derived_nodes": (None, [P]),
         "og_solve_collisions": (None, [P]),
@@ -86,6 +90,8 @@ def lib():
         "og_expand": (None, [P, fp, c_float, c_float, c_int, c_int, c_int, u64, u64, u32]),
         "og_check_direct_connection": (c_float, [P, fp, c_float, c_int, c_int, c_float, c_int, c_int, c_float,
                                                  c_int, c_int]),
+        "og_check_direct_connection_host": (c_float, [P, fp, c_float, c_int, c_int, c_float, c_int, c_int,
+                                                      c_float, c_int]),
         "og_process_direct_goal_connection": (None, [P, fp, c_int, c_int, c_float, c_float]),
         "og_find_best_goal_direct_connection": (c_int, [P, fp, c_float, c_int]),
         "og_check_goal_reached": (c_int, [P, c_int, c_int, c_float, c_float, c_float]),
