@@ -9,7 +9,9 @@ import numpy as np
 import pytest
 
 import _oracle as O
-from _parity import GpuPlanner, OraclePlanner, Params, diff_goal_planes, diff_states
+from _parity import GpuPlanner, OraclePlanner, Params, diff_goal_planes, diff_states, run_side_by_side
+
+_run_side_by_side = run_side_by_side
 
 from fastrrt_amd import scenes
 
@@ -23,36 +25,6 @@ def _small_scene(seed, size=128, coverage=0.10):
 
 SMALL = Params(min_dist=(10, 16), max_path=30.0, dist_tol=12.0)
 FULL = Params()
-
-
-def _run_side_by_side(sc, prm, seed, stream, max_iter, smart=False, goal_planes_first=False, region_side=None):
-    """Oracle and GPU planner in lock step, grids compared bit for bit after
-    every loop().  The handle evaluates the goal-connection planes lazily; they
-    are compared before the search (which finishes the pass up front) or after
-    it (the search ran on lazily evaluated cells)."""
-    print(f"[parity] seed={seed} size={sc.width} cov={sc.coverage} smart={smart}", flush=True)
-    o = OraclePlanner(sc, prm, seed, stream, region_side)
-    g = GpuPlanner(sc, prm, seed, stream, region_side)
-    if goal_planes_first:
-        bad = diff_goal_planes(o.goal_planes(), g.goal_planes())
-        assert not bad, f"goal-connection planes differ: {bad}"
-    o.search_init()
-    g.search_init()
-    assert o.goal_reached() == g.goal_reached()
-    it = 0
-    while True:
-        go = (not o.goal_reached()) and o.loop(smart)
-        gg = (not g.goal_reached()) and g.loop(smart)
-        it += 1
-        bad = diff_states(o.state(), g.state())
-        assert not bad, f"iteration {it}: grid differs {bad}"
-        assert go == gg, f"iteration {it}: loop() returned {go} (oracle) vs {gg} (gpu)"
-        if not go or it >= max_iter:
-            break
-    if not goal_planes_first:
-        bad = diff_goal_planes(o.goal_planes(), g.goal_planes())
-        assert not bad, f"goal-connection planes differ after the search: {bad}"
-    return o, g, it
 
 
 def _odd_scene(W, H, seed, coverage=0.08, min_dist=(10, 16)):
