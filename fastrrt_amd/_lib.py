@@ -76,6 +76,9 @@ SIGNATURES = {
     "clear": (None, [c_void_p]),
     "list_all": (fp, [c_void_p]),
     "free_list_all": (None, [fp]),
+    "cudagraph_check_feasible_connection": (c_bool, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int]),
+    "cudagraph_check_feasible_connections": (c_int, [c_void_p, c_void_p, i32p, c_int, c_int, u8p,
+                                                     ctypes.POINTER(c_double), fp]),
     "solve_collisions": (None, [c_void_p]),
     "process_direct_goal_connection": (None, [c_void_p, c_void_p, c_int, c_int, c_float, c_float]),
     "is_directly_connected_to_goal": (c_bool, [c_void_p, c_int, c_int]),

@@ -173,6 +173,10 @@ class FrGraph {
             (void)hipFree(d_apf_bits_);
         if (d_apf_tab_)
             (void)hipFree(d_apf_tab_);
+        if (d_fc_)
+            (void)hipFree(d_fc_);
+        d_fc_ = nullptr;
+        fc_cap_ = 0;
         d_tmp = nullptr;
         d_region = nullptr;
         d_graph_costs = nullptr;
@@ -408,6 +412,46 @@ class FrGraph {
             node_collision = true;
         dirty();
         return {h[0], h[1]};
+    }
+
+    /* CudaGraph::checkFeasibleConnection (kinematics.cu:308-326) for n
+     * queries (start x, start z, end x, end z): the graph's class costs,
+     * footprint, ego box and headings, as there.  Outputs per query: the
+     * verdict, and check_kinematic_connection_start_end's final heading and
+     * path cost (either may be null). */
+    void feasible_connections(const FrFrame *frame, const int32_t *q, int n, int velocity, uint8_t *ok, double *fh,
+                              float *pc)
+    {
+        if (!has_physical)
+            throw std::runtime_error("physical parameters are not set");
+        if (n < 0 || (n > 0 && (!q || !ok)))
+            throw std::runtime_error("check_feasible_connections: invalid query or output buffer");
+        if (n == 0)
+            return;
+        prep_frame(frame, false);
+        const size_t need = 32 * (size_t)n; /* int4 query + double + float + byte, 16/8/4/1 (+3 spare) */
+        if (need > fc_cap_) {
+            eng.sync();
+            if (d_fc_)
+                (void)hipFree(d_fc_);
+            d_fc_ = nullptr;
+            fc_cap_ = 0;
+            FR_CHECK(hipMalloc(&d_fc_, need));
+            fc_cap_ = need;
+        }
+        int4 *d_q = reinterpret_cast<int4 *>(d_fc_);
+        double *d_h = reinterpret_cast<double *>(d_fc_ + 16 * (size_t)n);
+        float *d_c = reinterpret_cast<float *>(d_fc_ + 24 * (size_t)n);
+        uint8_t *d_ok = d_fc_ + 28 * (size_t)n;
+        FR_CHECK(hipMemcpyAsync(d_q, q, 16 * (size_t)n, hipMemcpyHostToDevice, st()));
+        fr_launch_feasible_connections(eng.launch(), 0, d_q, n, velocity, d_ok, d_h, d_c, st());
+        FR_CHECK(hipGetLastError());
+        FR_CHECK(hipMemcpyAsync(ok, d_ok, (size_t)n, hipMemcpyDeviceToHost, st()));
+        if (fh)
+            FR_CHECK(hipMemcpyAsync(fh, d_h, 8 * (size_t)n, hipMemcpyDeviceToHost, st()));
+        if (pc)
+            FR_CHECK(hipMemcpyAsync(pc, d_c, 4 * (size_t)n, hipMemcpyDeviceToHost, st()));
+        eng.sync();
     }
 
     /* graph_expansion.cu:246-261 (the frame's class costs; the footprint and
@@ -765,6 +809,9 @@ class FrGraph {
     uint64_t *d_apf_bits_ = nullptr;
     float *d_apf_tab_ = nullptr;
     size_t apf_words_ = 0, apf_tab_ = 0;
+    /* feasible-connection queries and results (grown on demand) */
+    uint8_t *d_fc_ = nullptr;
+    size_t fc_cap_ = 0;
 };
 
 /* frames carry a generation counter so a re-uploaded frame is re-prepared */
@@ -1353,6 +1400,30 @@ int cudagraph_can_connect_hermite(void *ptr, void *frame, int x, int z, int goal
         if (result)
             *result = r.second;
         return r.first ? 1 : 0;
+    })
+}
+
+bool cudagraph_check_feasible_connection(void *ptr, void *frame, int start_x, int start_z, int end_x, int end_z,
+                                         int velocity_m_s)
+{
+    FR_TRY(false, {
+        int32_t q[4];
+        q[0] = start_x;
+        q[1] = start_z;
+        q[2] = end_x;
+        q[3] = end_z;
+        uint8_t ok = 0;
+        G(ptr)->feasible_connections(F(frame), q, 1, velocity_m_s, &ok, nullptr, nullptr);
+        return ok != 0;
+    })
+}
+
+int cudagraph_check_feasible_connections(void *ptr, void *frame, const int32_t *queries, int n, int velocity_m_s,
+                                         uint8_t *feasible, double *final_heading, float *path_cost)
+{
+    FR_TRY(-1, {
+        G(ptr)->feasible_connections(F(frame), queries, n, velocity_m_s, feasible, final_heading, path_cost);
+        return n;
     })
 }
 

@@ -97,6 +97,10 @@ void fr_launch_derivate(const FrLaunch &L, int slot, float steer, float path_siz
 /* CudaGraph::canConnectToGoal: d_out[0] = curve result, d_out[1] = goal pixel blocked */
 void fr_launch_hermite_probe(const FrLaunch &L, int slot, int x, int z, int gx, int gz, int goal_heading,
                              float max_curv, float *d_out, hipStream_t st);
+/* check_kinematic_connection_start_end (kinematics.cu:233-306) for n queries
+ * (start x, start z, end x, end z), one lane each */
+void fr_launch_feasible_connections(const FrLaunch &L, int slot, const int4 *d_q, int n, int velocity,
+                                    uint8_t *d_ok, double *d_heading, float *d_cost, hipStream_t st);
 void fr_launch_accept(const FrLaunch &L, int slot, int N, hipStream_t st);
 void fr_launch_solve_collisions(const FrLaunch &L, int slot, int N, int nodes, hipStream_t st);
 void fr_launch_region_count(const FrLaunch &L, int slot, int N, uint32_t *d_out, hipStream_t st);

@@ -3216,6 +3216,136 @@ __global__ void __launch_bounds__(64) k_op_hermite(const FrPlanDev *plans, int s
         out[0] = node_cost;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Path feasibility: check_kinematic_connection_start_end                   */
+/* (kinematics.cu:233-306) behind CudaGraph::checkFeasibleConnection        */
+/* (:308-326).  FP64 throughout, one lane per (start, end) query.           */
+/* ------------------------------------------------------------------------ */
+
+/* compute_path_heading, kinematics.cu:75-92 */
+__device__ __forceinline__ double dev_path_heading(double p1x, double p1y, double p2x, double p2y)
+{
+    const double dy = p2y - p1y, dx = p2x - p1x;
+    if (dy >= 0 && dx > 0)
+        return atan(dy / dx);
+    else if (dy >= 0 && dx < 0)
+        return FR_PI_D - atan(dy / fabs(dx));
+    else if (dy < 0 && dx > 0)
+        return -atan(fabs(dy) / dx);
+    else if (dy < 0 && dx < 0)
+        return atan(dy / dx) - FR_PI_D;
+    else if (dx == 0 && dy > 0)
+        return FR_HALF_PI_D;
+    else if (dx == 0 && dy < 0)
+        return -FR_HALF_PI_D;
+    return 0.0;
+}
+
+/* clip, kinematics.cu:113-120 */
+__device__ __forceinline__ double dev_clip(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* The footprint test at any centre.  The walk below can leave the grid;
+ * there the reference still tests the footprint of the previous cell
+ * (path_feasible.cu:18-78: out-of-grid samples are skipped).  In the grid
+ * this is dev_feasible; outside it every sample is tested against the
+ * frame (ego box skipped, a class outside the table is an obstacle), and
+ * the all-heading bit of a centre outside the grid counts as clear (the
+ * reference reads it past the frame). */
+__device__ bool dev_feasible_anywhere(const FrPlanDev &P, int x, int z, float h)
+{
+    if (x >= 0 && x < P.W && z >= 0 && z < P.H)
+        return dev_feasible(P, x, z, h);
+    float s, c;
+    fr_sincosf(h, &s, &c);
+    for (int i = -P.mz; i <= P.mz; ++i)
+        for (int j = -P.mx; j <= P.mx; ++j) {
+            const int xl = fr_trunc_f((float)j * c - (float)i * s + (float)x);
+            const int zl = fr_trunc_f((float)j * s + (float)i * c + (float)z);
+            if (xl < 0 || xl >= P.W || zl < 0 || zl >= P.H)
+                continue;
+            if (xl >= P.lbx && xl <= P.ubx && zl >= P.ubz && zl <= P.lbz)
+                continue;
+            const int cls = fr_trunc_f(P.frame[3 * ((long)zl * P.W + xl)]);
+            if (cls < 0 || cls >= P.n_classes || P.class_costs[cls] < 0)
+                return false;
+        }
+    return true;
+}
+
+/* kinematics.cu:233-306.  The start heading is the grid's heading at the
+ * start cell; velocity is the reference's int argument (:308).  Outputs:
+ * the verdict, and the heading and cost reached when the walk stopped on
+ * moving away from the end (both 0 otherwise, as in the reference).  The
+ * intrinsic cost of a cell outside the grid counts as 0 (the reference
+ * reads past the grid there). */
+__device__ bool dev_feasible_connection(const FrPlanDev &P, int sx, int sz, int ex, int ez, int velocity,
+                                        double *final_heading, float *path_cost)
+{
+    *final_heading = 0;
+    *path_cost = 0;
+    const double ddx0 = (double)(ex - sx), ddz0 = (double)(ez - sz);
+    const double distance = sqrt(ddx0 * ddx0 + ddz0 * ddz0);
+    const double lr = P.lr;
+    const double max_steer = P.max_steer_rad;
+    const double smx = (double)(P.og_y - (float)sz), smy = (double)((float)sx - P.og_x);
+    const double emx = (double)(P.og_y - (float)ez), emy = (double)((float)ex - P.og_x);
+    const double dt = 0.1;
+    double heading = (double)P.heading[sz * P.W + sx];
+    double steer_a = dev_clip(dev_path_heading(smx, smy, emx, emy) - heading, -max_steer, max_steer);
+    const double ds = (double)(float)velocity * dt;
+    const int total_steps = fr_trunc_d(distance / ds);
+    double nx = smx, ny = smy;
+    int lx = sx, lz = sz;
+    double best_end = distance;
+    float curr_cost = 0;
+    for (int i = 0; i < total_steps; i++) {
+        const double steer = tan(steer_a);
+        const double beta = atan(steer / lr);
+        nx += ds * cos(heading + beta);
+        ny += ds * sin(heading + beta);
+        heading += ds * cos(beta) * steer / (2 * lr);
+        steer_a = dev_clip(dev_path_heading(nx, ny, emx, emy) - heading, -max_steer, max_steer);
+        const double dx = emx - nx, dy = emy - ny;
+        const double dist = sqrt(dx * dx + dy * dy);
+        const int cx = fr_trunc_d((double)P.og_x + ny), cz = fr_trunc_d((double)P.og_y - nx);
+        if (cx == lx && cz == lz)
+            continue;
+        const bool in = cx >= 0 && cx < P.W && cz >= 0 && cz < P.H;
+        curr_cost += ((in && P.intrinsic) ? P.intrinsic[cz * P.W + cx] : 0.0f) + 1.0f;
+        if (best_end < dist) {
+            *final_heading = heading;
+            *path_cost = curr_cost;
+            return best_end <= 2;
+        }
+        if (!dev_feasible_anywhere(P, lx, lz, (float)heading))
+            return false;
+        lx = cx;
+        lz = cz;
+        best_end = dist;
+    }
+    return true;
+}
+
+/* One lane per query q = (start x, start z, end x, end z); a start outside
+ * the grid answers false. */
+__global__ void __launch_bounds__(FR_BLOCK) k_op_feasible_connection(const FrPlanDev *plans, int slot, const int4 *q,
+                                                                     int n, int velocity, uint8_t *ok,
+                                                                     double *final_heading, float *path_cost)
+{
+    const FrPlanDev &P = plans[slot];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int4 a = q[k];
+        double fh = 0;
+        float pc = 0;
+        bool r = false;
+        if (a.x >= 0 && a.x < P.W && a.y >= 0 && a.y < P.H)
+            r = dev_feasible_connection(P, a.x, a.y, a.z, a.w, velocity, &fh, &pc);
+        ok[k] = r ? 1 : 0;
+        final_heading[k] = fh;
+        path_cost[k] = pc;
+    }
+}
+
 /* CudaGraph::derivateNode -> expand_node, host sequential semantics
  * (graph_expansion.cu:103-145,224-244).  out = end cell or (-1, -1). */
 __global__ void __launch_bounds__(FR_BLOCK) k_op_derivate(const FrPlanDev *plans, int slot, float steer, float path_size, float v, int x, int z,
@@ -3356,6 +3486,15 @@ void fr_launch_hermite_probe(const FrLaunch &L, int slot, int x, int z, int gx, 
 {
     hipLaunchKernelGGL(k_op_hermite, dim3(1), dim3(64), 0, st, L.plans, slot, x, z, gx, gz, goal_heading, max_curv,
                        d_out);
+}
+
+void fr_launch_feasible_connections(const FrLaunch &L, int slot, const int4 *d_q, int n, int velocity,
+                                    uint8_t *d_ok, double *d_heading, float *d_cost, hipStream_t st)
+{
+    if (n <= 0)
+        return;
+    hipLaunchKernelGGL(k_op_feasible_connection, dim3(fr_grid_for(n, FR_BLOCK, 4096)), dim3(FR_BLOCK), 0, st, L.plans,
+                       slot, d_q, n, velocity, d_ok, d_heading, d_cost);
 }
 
 void fr_launch_accept(const FrLaunch &L, int slot, int N, hipStream_t st)

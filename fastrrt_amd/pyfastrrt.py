@@ -381,6 +381,29 @@ class CudaGraph:
         _lib.check("derivate_node")
         return (out[0], out[1])
 
+    def check_feasible_connection(self, frame: SearchFrame, start, end, velocity: int) -> bool:
+        """CudaGraph::checkFeasibleConnection (kinematics.cu:308-326)."""
+        r = _lib.load().cudagraph_check_feasible_connection(self._ptr, frame.get_cuda_ptr(), int(start[0]),
+                                                            int(start[1]), int(end[0]), int(end[1]), int(velocity))
+        _lib.check("cudagraph_check_feasible_connection")
+        return bool(r)
+
+    def check_feasible_connections(self, frame: SearchFrame, queries, velocity: int):
+        """Many (start x, start z, end x, end z) queries in one launch; returns
+        (feasible bool[n], final heading float64[n], path cost float32[n])."""
+        q = np.ascontiguousarray(np.asarray(queries, np.int32).reshape(-1, 4))
+        n = q.shape[0]
+        ok = np.zeros(n, np.uint8)
+        fh = np.zeros(n, np.float64)
+        pc = np.zeros(n, np.float32)
+        r = _lib.load().cudagraph_check_feasible_connections(
+            self._ptr, frame.get_cuda_ptr(), q.ctypes.data_as(_lib.i32p), n, int(velocity),
+            ok.ctypes.data_as(_lib.u8p), fh.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            pc.ctypes.data_as(_lib.fp))
+        if r != n:
+            raise RuntimeError(f"check_feasible_connections: {_lib.last_error()}")
+        return ok.astype(bool), fh, pc
+
     def accept_derived_nodes(self) -> None:
         _lib.load().accept_derived_nodes(self._ptr, 0, 0, 0.0)
 

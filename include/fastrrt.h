@@ -181,6 +181,22 @@ void free_list_all(float *p);
  * reference does (a failed curve, -1 or -2, answers true) */
 bool can_connect_to_goal_using_hermite(void *ptr, void *frame, int x, int z, int goal_x, int goal_z,
                                        float goal_heading);
+/* New (the reference keeps it on the C++ class, used by
+ * unittests/src/test_kinematics.cpp:51-94): CudaGraph::checkFeasibleConnection
+ * (kinematics.cu:308-326) -> check_kinematic_connection_start_end (:233-306),
+ * an FP64 walk that steers toward the end cell and tests the footprint of
+ * every cell it passes; true when it gets within 2 px of the end.  The start
+ * heading is the graph's heading at the start cell; velocity is an int there.
+ * The graph's class costs, footprint and ego box apply; the frame is read. */
+bool cudagraph_check_feasible_connection(void *ptr, void *frame, int start_x, int start_z, int end_x, int end_z,
+                                         int velocity_m_s);
+/* The same for n queries (start x, start z, end x, end z per query, 4n ints)
+ * in one launch, one lane per query: feasible[k] = 0/1 and, if the buffers
+ * are not NULL, the walk's final heading and path cost (the reference's out
+ * parameters, 0 unless the walk stopped on moving away from the end).
+ * Returns n, or -1 on error (fastrrt_last_error). */
+int cudagraph_check_feasible_connections(void *ptr, void *frame, const int32_t *queries, int n, int velocity_m_s,
+                                         uint8_t *feasible, double *final_heading, float *path_cost);
 /* :155-159 */
 void solve_collisions(void *ptr);
 /* :161-167 */

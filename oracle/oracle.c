@@ -276,7 +276,9 @@ int og_feasible(const og_graph *g, const float *frame, int x, int z, float angle
     ((og_graph *)g)->n_footprint_checks++;
     const int W = g->W, H = g->H;
     long pos = og_pos(g, x, z);
-    if (fr_trunc_f(frame[3 * pos + 2]) & 1)
+    /* a centre outside the grid (reached by the feasible-connection walk):
+     * the reference reads its all-heading bit past the frame; taken as clear */
+    if (og_in(g, x, z) && (fr_trunc_f(frame[3 * pos + 2]) & 1))
         return 1;
     const int lbx = g->lb_x, lbz = g->lb_z, ubx = g->ub_x, ubz = g->ub_z;
     const int mx = g->min_dist_x, mz = g->min_dist_z;
@@ -1054,9 +1056,18 @@ static double og_path_heading(double p1x, double p1y, double p2x, double p2y)
     return 0.0;
 }
 
-/* check_kinematic_connection_start_end, kinematics.cu:233-306 (FP64, host) */
-int og_check_feasible_connection(og_graph *g, const float *frame, int sx, int sz, int ex, int ez, float v)
+/* check_kinematic_connection_start_end, kinematics.cu:233-306 (FP64, host).
+ * final_heading / path_cost: the reference's out parameters (set when the
+ * walk stops on moving away from the end, 0 otherwise); the intrinsic cost
+ * of a cell outside the grid counts as 0 (the reference reads past the
+ * grid).  A start outside the grid answers 0. */
+int og_check_feasible_connection_ex(og_graph *g, const float *frame, int sx, int sz, int ex, int ez, float v,
+                                    double *final_heading, float *path_cost)
 {
+    *final_heading = 0;
+    *path_cost = 0;
+    if (!og_in(g, sx, sz))
+        return 0;
     const double ddx0 = (double)(ex - sx), ddz0 = (double)(ez - sz);
     const double distance = sqrt(ddx0 * ddx0 + ddz0 * ddz0);
     const double lr = g->lr;
@@ -1074,6 +1085,7 @@ int og_check_feasible_connection(og_graph *g, const float *frame, int sx, int sz
     double nx = smx, ny = smy;
     int lx = sx, lz = sz;
     double best_end = distance;
+    float curr_cost = 0;
     for (int i = 0; i < total_steps; i++) {
         double steer = tan(steer_a);
         double beta = atan(steer / lr);
@@ -1089,8 +1101,12 @@ int og_check_feasible_connection(og_graph *g, const float *frame, int sx, int sz
         const int cx = fr_trunc_d((double)g->og_x + ny), cz = fr_trunc_d((double)g->og_y - nx);
         if (cx == lx && cz == lz)
             continue;
-        if (best_end < dist)
+        curr_cost += (og_in(g, cx, cz) ? g->intrinsic[og_pos(g, cx, cz)] : 0.0f) + 1.0f;
+        if (best_end < dist) {
+            *final_heading = heading;
+            *path_cost = curr_cost;
             return best_end <= 2;
+        }
         if (!og_feasible(g, frame, lx, lz, (float)heading))
             return 0;
         lx = cx;
@@ -1098,6 +1114,13 @@ int og_check_feasible_connection(og_graph *g, const float *frame, int sx, int sz
         best_end = dist;
     }
     return 1;
+}
+
+int og_check_feasible_connection(og_graph *g, const float *frame, int sx, int sz, int ex, int ez, float v)
+{
+    double fh;
+    float pc;
+    return og_check_feasible_connection_ex(g, frame, sx, sz, ex, ez, v, &fh, &pc);
 }
 
 /* interpolateHermiteCurve, src/interpolation.cpp:36-110 (FP32, host).

@@ -135,6 +135,8 @@ float og_can_connect_hermite(og_graph *g, const float *frame, float max_curv, in
 void og_apf_repulsion(og_graph *g, const float *frame, float kr, int radius);
 void og_apf_attraction(og_graph *g, const float *frame, float ka, int gx, int gz);
 int og_check_feasible_connection(og_graph *g, const float *frame, int sx, int sz, int ex, int ez, float v);
+int og_check_feasible_connection_ex(og_graph *g, const float *frame, int sx, int sz, int ex, int ez, float v,
+                                    double *final_heading, float *path_cost);
 int og_interpolate_hermite(int W, int H, int x1, int z1, float h1, int x2, int z2, float h2, float *out,
                            int cap);
 

@@ -101,6 +101,8 @@ def lib():
         "og_apf_repulsion": (None, [P, fp, c_float, c_int]),
         "og_apf_attraction": (None, [P, fp, c_float, c_int, c_int]),
         "og_check_feasible_connection": (c_int, [P, fp, c_int, c_int, c_int, c_int, c_float]),
+        "og_check_feasible_connection_ex": (c_int, [P, fp, c_int, c_int, c_int, c_int, c_float,
+                                                    ctypes.POINTER(c_double), fp]),
         "og_interpolate_hermite": (c_int, [c_int, c_int, c_int, c_int, c_float, c_int, c_int, c_float, fp, c_int]),
         "og_dump_graph": (None, [P, ctypes.c_char_p]),
         "og_read_dump": (None, [P, ctypes.c_char_p]),

@@ -453,3 +453,57 @@ def test_rollout_fp32_within_1e5_of_fp64():
     print(f"[fp64] {n} rollouts: worst position {worst_pos:.3g} (relative to |pose|), worst heading "
           f"{worst_h:.3g} (relative to 2 pi); {over_rel_h} headings beyond 1e-5 relative to |h|; "
           f"{flips} end-cell flips at cell edges")
+
+
+# ---------------------------------------------------------------------------
+# path feasibility: check_kinematic_connection_start_end (kinematics.cu:233-306)
+# ---------------------------------------------------------------------------
+
+def _feasible(g, frame, s, e, v):
+    fh = ctypes.c_double()
+    pc = ctypes.c_float()
+    r = O.lib().og_check_feasible_connection_ex(g.g, O.fptr(frame), s[0], s[1], e[0], e[1], v, ctypes.byref(fh),
+                                                ctypes.byref(pc))
+    return r, fh.value, pc.value
+
+
+def test_feasible_connection_closed_forms():
+    """Heading 0 points to -z.  An end straight ahead gives path heading =
+    heading, so the steering stays 0 and the walk is the straight column: it
+    reaches the end after distance / ds steps without moving away (true,
+    out parameters untouched).  An obstacle row across that column rejects
+    it.  An end behind the start: the first new cell is farther from the end
+    than the start was, so the walk stops there (false: 10 px > 2 px), with
+    cost 1 and the heading after that step."""
+    W, H = 256, 256
+    frame = O.empty_frame(W, H, cls=1.0)
+    g = O.Graph(W, H)
+    g.physical(256, 256, 40.0, LEN)
+    g.costs([0, 0, 2, 3, 4, -1])
+    g.search(4, 4)
+    g.add(128, 200, 0.0, -1, -1, 0)
+    assert _feasible(g, frame, (128, 200), (128, 150), 1.0) == (1, 0.0, 0.0)
+    assert _feasible(g, frame, (128, 200), (128, 199), 1.0) == (1, 0.0, 0.0)
+    blocked = frame.copy()
+    blocked[170, :, 0] = 5.0
+    assert _feasible(g, blocked, (128, 200), (128, 150), 1.0)[0] == 0
+    r, fh, pc = _feasible(g, frame, (128, 200), (128, 210), 1.0)
+    assert r == 0 and pc == 1.0 and abs(fh) <= 0.1 * math.tan(math.radians(40)) / LEN * 20
+    # a start outside the grid answers false
+    assert _feasible(g, frame, (-1, 200), (128, 150), 1.0)[0] == 0
+
+
+def test_feasible_connection_gtest():
+    """unittests/src/test_kinematics.cpp:51-94: a node derived at 20 deg,
+    P = 50, v = 2 from (128, 128, 0) is reachable from the start."""
+    L = O.lib()
+    frame = O.empty_frame(256, 256)
+    g = O.Graph(256, 256)
+    g.physical(256, 256, 40.0, LEN)
+    g.costs([0, 1, 2, 3, 4, 5])
+    g.search(0, 0)
+    g.add(128, 128, 0.0, -1, -1, 0)
+    out = (ctypes.c_int * 2)()
+    L.og_derivate_node(g.g, O.fptr(frame), float(np.float32(math.radians(20))), 50.0, 2.0, 128, 128, out)
+    assert out[0] > 128 and out[1] < 128
+    assert _feasible(g, frame, (128, 128), (out[0], out[1]), 2.0)[0] == 1
