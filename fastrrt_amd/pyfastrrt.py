@@ -243,6 +243,13 @@ class FastRRT:
         ptr = L.interpolate_planned_path(self._ptr) if interpolate else L.get_planned_path(self._ptr)
         return _path_from_ptr(L, ptr)
 
+    def interpolate_planned_path_p(self, path) -> list:
+        """Hermite resampling of a caller's path (python_bind.cpp:157-171):
+        `path` is an (n, 3) array of (x, z, heading)."""
+        L = _lib.load()
+        flat = np.ascontiguousarray(np.asarray(path, np.float32).reshape(-1))
+        return _path_from_ptr(L, L.interpolate_planned_path_p(self._ptr, flat.ctypes.data_as(_lib.fp), flat.size))
+
     def build_ideal_curve(self, goal_x: int, goal_z: int, goal_heading: float):
         L = _lib.load()
         return _path_from_ptr(L, L.ideal_curve(self._ptr, goal_x, goal_z, goal_heading))

@@ -1295,6 +1295,13 @@ void og_rrt_search_init(og_rrt *r)
     r->iter = 0;
 }
 
+/* fastrrt.cpp:293-297 */
+void og_rrt_load_graph_state(og_rrt *r, const char *path)
+{
+    og_read_dump(r->g, path);
+    r->last_count = (int)og_count(r->g, FR_TYPE_TEMP);
+}
+
 /* fastrrt.cpp:198-202 */
 int og_rrt_goal_reached(og_rrt *r)
 {

@@ -156,6 +156,7 @@ void og_rrt_set_plan_data(og_rrt *r, const float *frame, const float *class_cost
 void og_rrt_search_init(og_rrt *r);
 int og_rrt_loop(og_rrt *r, int smart);
 int og_rrt_goal_reached(og_rrt *r);
+void og_rrt_load_graph_state(og_rrt *r, const char *path);
 int og_rrt_get_planned_path(og_rrt *r, float *out, int cap);
 int og_rrt_path_optimize(og_rrt *r);
 int og_rrt_interpolate_planned_path(og_rrt *r, float *out, int cap);

@@ -116,6 +116,7 @@ def lib():
         "og_rrt_search_init": (None, [ctypes.c_void_p]),
         "og_rrt_loop": (c_int, [ctypes.c_void_p, c_int]),
         "og_rrt_goal_reached": (c_int, [ctypes.c_void_p]),
+        "og_rrt_load_graph_state": (None, [ctypes.c_void_p, ctypes.c_char_p]),
         "og_rrt_get_planned_path": (c_int, [ctypes.c_void_p, fp, c_int]),
         "og_rrt_path_optimize": (c_int, [ctypes.c_void_p]),
         "og_rrt_interpolate_planned_path": (c_int, [ctypes.c_void_p, fp, c_int]),
