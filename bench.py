@@ -52,6 +52,7 @@ Prints one JSON line.  One GPU only: --gpus N > 1 exits with a message.
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -86,6 +87,11 @@ def parse():
                     help="timed steps whose plans also run plain (success compared plan by plan)")
     ap.add_argument("--cpu-sample", type=int, default=2, help="plans timed on the CPU oracle, one thread (0 = skip)")
     ap.add_argument("--cpu-workers", type=int, default=16, help="processes of the all-core CPU oracle run")
+    ap.add_argument("--c1-plans", type=int, default=32,
+                    help="C1 plans (10 %% rectangles, goal (128,0,0)) compared between the CPU oracle and the GPU")
+    ap.add_argument("--c1-timeout-ms", type=int, default=100, help="the reference's timeout of the C1 comparison")
+    ap.add_argument("--plain-profile-plans", type=int, default=48,
+                    help="plans of a separate plain pass timed kernel by kernel with HIP events")
     ap.add_argument("--iters-per-poll", type=int, default=1)
     ap.add_argument("--roofline-iters", type=int, default=32)
     ap.add_argument("--roofline-only", action="store_true", help="run only the deterministic roofline pass")
@@ -121,11 +127,12 @@ def make_steps(args, scenes):
     return steps
 
 
-def plain_plan(P, rrt, frame, pl, args):
+def plain_plan(P, rrt, frame, pl, args, timeout_ms=None):
     """set_plan_data -> search_init -> loop() until goal/timeout -> path_optimize
-    rounds -> get_planned_path, through the FastRRT handle."""
+    rounds -> get_planned_path, through the FastRRT handle.  Returns the
+    latency, the path (None when none), the iterations and the phase times."""
     sc = pl.sc
-    budget = args.plain_timeout_ms
+    budget = args.plain_timeout_ms if timeout_ms is None else timeout_ms
     sp = P.SearchParams(frame, P.Waypoint(sc.start[0], sc.start[1], P.angle(sc.start[2])),
                         P.Waypoint(sc.goal[0], sc.goal[1], P.angle(sc.goal[2])), 1.0, sc.min_dist, budget, 40.0,
                         20.0, P.angle.new_deg(10))
@@ -133,39 +140,88 @@ def plain_plan(P, rrt, frame, pl, args):
     t0 = time.perf_counter()
     rrt.set_plan_data(sp)
     rrt.search_init()
+    t_setup = time.perf_counter()
     it = 0
     while not rrt.goal_reached() and rrt.loop(False):
         it += 1
         if it >= args.max_iter:
             break
     ok = rrt.goal_reached()
+    t_loop = time.perf_counter()
     if ok:
         r = 0
         while r < args.opt_rounds and rrt.path_optimize():
             r += 1
+    t_opt = time.perf_counter()
     path = rrt.get_planned_path() if ok else None
     t1 = time.perf_counter()
-    return (t1 - t0) * 1e3, bool(ok and path), it
+    phases = ((t_setup - t0) * 1e3, (t_loop - t_setup) * 1e3, (t_opt - t_loop) * 1e3, (t1 - t_opt) * 1e3)
+    arr = np.array([[w.x, w.z, w.heading.rad()] for w in path], np.float32) if (ok and path) else None
+    return (t1 - t0) * 1e3, arr, it, phases
 
 
-def run_plain(P, plans, args, frame_of):
+def plain_path_cost(L, rrt, path):
+    """cost field of the path's last node (read after the plan is timed)"""
+    cost = np.zeros((256, 256), np.float32)
+    L.fastrrt_export_state(rrt._ptr, None, None, None, None, None, cost.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                           None, None, None)
+    return float(cost[int(path[-1, 1]), int(path[-1, 0])])
+
+
+def run_plain(P, L, plans, args, frame_of, timeout_ms=None):
+    """The plans one at a time through one FastRRT handle.  Latency covers
+    set_plan_data to the extracted path; the path cost is read after the
+    latency clock stops, and its read time is left out of plans/s."""
     sc0 = plans[0].sc
     ego = P.EgoParams(256, 256, sc0.perception_m, sc0.perception_m, 40.0, 5.412658773, sc0.ego_lb, sc0.ego_ub, -1.0,
                       sc0.class_costs)
     rrt = P.FastRRT(ego)
     for pl in plans[:2]:  # warm-up
-        plain_plan(P, rrt, frame_of(pl.sc), pl, args)
-    lat, oks, iters = [], {}, []
+        plain_plan(P, rrt, frame_of(pl.sc), pl, args, timeout_ms)
+    lat, paths, iters, phases, costs = [], {}, [], [], {}
+    extra = 0.0
     t0 = time.perf_counter()
     for pl in plans:
-        ms, ok, it = plain_plan(P, rrt, frame_of(pl.sc), pl, args)
+        ms, path, it, ph = plain_plan(P, rrt, frame_of(pl.sc), pl, args, timeout_ms)
         lat.append(ms)
-        oks[pl.tag] = ok
+        paths[pl.tag] = path
         iters.append(it)
+        phases.append(ph)
+        if path is not None:
+            tc = time.perf_counter()
+            costs[pl.tag] = plain_path_cost(L, rrt, path)
+            extra += time.perf_counter() - tc
     t1 = time.perf_counter()
-    return {"plans_per_s": len(plans) / (t1 - t0), "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)), "success": float(np.mean(list(oks.values()))),
-            "mean_iterations": float(np.mean(iters)), "plans": len(plans)}, oks
+    wall = t1 - t0 - extra
+    ph = np.array(phases)
+    phase_ms = {name: [round(float(np.percentile(ph[:, k], q)), 3) for q in (50, 99, 100)]
+                for k, name in enumerate(("setup", "loop", "optimize", "path"))}
+    found = sum(1 for v in paths.values() if v is not None)
+    out = {"plans_per_s": len(plans) / wall, "found_plans_per_s": found / wall,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "max_ms": float(np.max(lat)), "success": found / len(plans),
+           "mean_iterations": float(np.mean(iters)), "plans": len(plans),
+           "mean_path_cost": float(np.mean(list(costs.values()))) if costs else None,
+           "phase_p50_p99_max_ms": phase_ms}
+    return out, paths, costs
+
+
+def validate_paths(items):
+    """items: (scene, path) of every found plan -> counts of valid plans and
+    of each failed check (fastrrt_amd.validate)."""
+    from fastrrt_amd import validate
+    fails, n_ok, worst, short_last = {}, 0, 0.0, 0
+    for sc, path in items:
+        short_last += math.hypot(float(path[-1, 0] - path[-2, 0]), float(path[-1, 1] - path[-2, 1])) < 2.0
+        f, rate = validate.check_path(sc.frame, sc.class_costs, path, sc.start, sc.goal, sc.min_dist,
+                                      (sc.ego_lb, sc.ego_ub), 20.0, math.radians(10))
+        n_ok += not f
+        for k in f:
+            fails[k] = fails.get(k, 0) + 1
+        if rate.size:
+            worst = max(worst, float(rate[:-1].max()) if rate.size > 1 else 0.0)
+    return {"paths": len(items), "valid": n_ok, "failed_checks": fails,
+            "max_inner_turning_rate_per_px": round(worst, 4), "last_edges_under_2px": short_last}
 
 
 def run_batched(b, plans, args, frame_of, max_iter=None, opt_rounds=None, deadline=None):
@@ -294,6 +350,38 @@ def _cpu_plan(item):
     return time.perf_counter() - t0
 
 
+def c1_scenes(n):
+    """C1: 256x256, 10 % rectangles, start (128,128,0), goal (128,0,0)."""
+    from fastrrt_amd import scenes
+    return [Plan(scenes.scene_256(20000 + k, 0.10, 128, 0.0), 300000 + k, 900000 + k) for k in range(n)]
+
+
+def _cpu_c1_plan(item):
+    """One C1 plan through the CPU oracle with the reference's protocol:
+    set_plan_data, search_init (the timeout clock starts, fastrrt.cpp:91),
+    loop() until the goal or the timeout, path_optimize rounds until the
+    timeout or no gain.  Returns (found, path cost, iterations, seconds)."""
+    sc, stream, timeout_ms, opt_rounds = item
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _parity import OraclePlanner, Params
+    t0 = time.perf_counter()
+    o = OraclePlanner(sc, Params(), SEED, stream, timeout_ms=timeout_ms)
+    o.search_init()
+    it = 0
+    while not o.goal_reached() and o.loop():
+        it += 1
+    cost = None
+    if o.goal_reached():
+        r = 0
+        while r < opt_rounds and o.path_optimize():
+            r += 1
+        path = o.path()
+        if path.shape[0]:
+            x, z = int(path[-1, 0]), int(path[-1, 1])
+            cost = float(o.L.og_get_cost(o.L.og_rrt_graph(o.r), x, z))
+    return cost is not None, cost, it, time.perf_counter() - t0
+
+
 def cpu_baseline(steps, args):
     """The CPU oracle on the host cores, run before the GPU is touched: one
     thread on `--cpu-sample` plans, then one process per core (at most
@@ -309,6 +397,19 @@ def cpu_baseline(steps, args):
     with mp.get_context("fork").Pool(workers) as pool:
         pool.map(_cpu_plan, [(pl.sc, pl.stream, args.max_iter, args.opt_rounds) for pl in many])
     wall = time.perf_counter() - t0
+    c1 = None
+    if args.c1_plans > 0:
+        plans = c1_scenes(args.c1_plans)
+        t0 = time.perf_counter()
+        with mp.get_context("fork").Pool(workers) as pool:
+            got = pool.map(_cpu_c1_plan, [(pl.sc, pl.stream, args.c1_timeout_ms, args.opt_rounds) for pl in plans])
+        c1_wall = time.perf_counter() - t0
+        costs = [g[1] for g in got if g[0]]
+        c1 = {"plans": len(plans), "success": round(float(np.mean([g[0] for g in got])), 4),
+              "mean_path_cost": round(float(np.mean(costs)), 2) if costs else None,
+              "mean_iterations": round(float(np.mean([g[2] for g in got])), 1),
+              "per_plan": {pl.tag: (g[0], g[1]) for pl, g in zip(plans, got)},
+              "cores": workers, "wall_s": round(c1_wall, 1)}
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -320,7 +421,38 @@ def cpu_baseline(steps, args):
             "sample": f"{len(many)} plans of the timed workload (one in {stride} of the first timed step, all "
                       f"coverages) through the CPU oracle, one process per core, {wall:.1f} s wall",
             "single_thread": {"value": round(len(single) / t_single, 4), "cores": 1,
-                              "sample": f"the first {len(single)} plans (5% coverage), {t_single:.1f} s"}}
+                              "sample": f"the first {len(single)} plans (5% coverage), {t_single:.1f} s"},
+            "c1": c1}
+
+
+def c1_compare(P, L, Batch, cpu_c1, args, frame_of):
+    """C1 on the GPU beside the CPU oracle's run: the same plans (same
+    Philox keys, so the same searches) through the FastRRT handle with the
+    reference's timeout, and batched with that timeout as each plan's
+    deadline.  A plan's search is the same on both sides, so a plan the CPU
+    solves within its timeout is one the GPU solves too unless the GPU ran
+    out of time first: the stated bound is GPU success >= CPU success on the
+    same plans, with no plan solved only by the CPU."""
+    plans = c1_scenes(cpu_c1["plans"])
+    plain, paths, costs = run_plain(P, L, plans, args, frame_of, timeout_ms=args.c1_timeout_ms)
+    b = Batch(max_plans=len(plans), max_cells=len(plans) * 256 * 256, launch="auto")
+    res, _ = run_batched(b, plans, args, frame_of, deadline=float(args.c1_timeout_ms))
+    ok_b = {r.tag: r.ok for r in res}
+    cost_b = [r.path_cost for r in res if r.ok]
+    cpu_ok = {int(t): v[0] for t, v in cpu_c1["per_plan"].items()}
+    only_cpu_plain = sum(1 for pl in plans if cpu_ok[pl.tag] and paths.get(pl.tag) is None)
+    only_cpu_batched = sum(1 for pl in plans if cpu_ok[pl.tag] and not ok_b.get(pl.tag, False))
+    return {"workload": "256x256, 10% rectangles 4-30 px, start (128,128,0), goal (128,0,0), min_dist 22x40, "
+                        f"P=40, r=20, timeout {args.c1_timeout_ms} ms (loop and path_optimize), then path_optimize",
+            "plans": len(plans),
+            "cpu": {k: cpu_c1[k] for k in ("success", "mean_path_cost", "mean_iterations", "cores")},
+            "gpu_plain": {"success": round(plain["success"], 4), "mean_path_cost": plain["mean_path_cost"],
+                          "mean_iterations": round(plain["mean_iterations"], 1), "p99_ms": round(plain["p99_ms"], 2),
+                          "solved_by_cpu_not_gpu": only_cpu_plain},
+            "gpu_batched": {"success": round(float(np.mean(list(ok_b.values()))), 4),
+                            "mean_path_cost": float(np.mean(cost_b)) if cost_b else None,
+                            "solved_by_cpu_not_gpu": only_cpu_batched},
+            "bound": "GPU success >= CPU success on the same plans, no plan solved only by the CPU"}
 
 
 def main():
@@ -378,34 +510,51 @@ def main():
             walls.append(wall)
         t1 = time.perf_counter()
     allres = [r for rs in results for r in rs]
+    scene_of = {pl.tag: pl.sc for st in steps for pl in st}
     lat = np.array([r.latency_ms for r in allres])
     n_b = len(allres)
     lat_ok = [r.latency_ms for r in allres if r.ok]
     lat_fail = [r.latency_ms for r in allres if not r.ok]
-    batched = {"plans_per_s": n_b / (t1 - t0), "p50_ms": float(np.percentile(lat, 50)),
+    found_b = [r for r in allres if r.ok]
+    batched = {"plans_per_s": n_b / (t1 - t0), "found_plans_per_s": len(found_b) / (t1 - t0),
+               "p50_ms": float(np.percentile(lat, 50)),
                "p99_ms": float(np.percentile(lat, 99)), "success": float(np.mean([r.ok for r in allres])),
                "mean_iterations": float(np.mean([r.iterations for r in allres])), "plans": n_b,
+               "mean_path_cost": float(np.mean([r.path_cost for r in found_b])) if found_b else None,
                "wall_ms_per_step": float(np.mean(walls)),
                "found_p50_p99_max_ms": [round(float(np.percentile(lat_ok, q)), 3) for q in (50, 99, 100)] if lat_ok else None,
-               "failed_p50_p99_max_ms": [round(float(np.percentile(lat_fail, q)), 3) for q in (50, 99, 100)] if lat_fail else None}
-    # plain form on the plans of the first timed step, with kernel timing
+               "failed_p50_p99_max_ms": [round(float(np.percentile(lat_fail, q)), 3) for q in (50, 99, 100)] if lat_fail else None,
+               "validation": validate_paths([(scene_of[r.tag], r.path) for r in found_b])}
+    # plain form on the plans of the timed steps (no kernel timing: the
+    # comparison's latencies are the handle's own)
     plain_set = [pl for k in range(args.warmup, args.warmup + min(max(args.plain_steps, 1), args.steps))
                  for pl in steps[k]]
     if args.plain_plans >= 0:
         plain_set = plain_set[:args.plain_plans]
-    plain, oks = None, {}
+    plain, paths_p = None, {}
     prof_p = None
     if plain_set:
+        plain, paths_p, costs_p = run_plain(P, L, plain_set, args, frame_of)
+        plain["validation"] = validate_paths([(scene_of[t], p) for t, p in paths_p.items() if p is not None])
+        same = {r.tag: r.ok for rs in results for r in rs if r.tag in paths_p}
+        if same:
+            oks = {t: paths_p[t] is not None for t in same}
+            plain["success_same_plans_batched"] = float(np.mean(list(same.values())))
+            plain["success_same_plans_plain"] = float(np.mean(list(oks.values())))
+            plain["success_gap_pp"] = round(100.0 * (plain["success_same_plans_plain"] -
+                                                     plain["success_same_plans_batched"]), 3)
+            plain["plans_solved_plain_not_batched"] = int(sum(1 for t in same if oks[t] and not same[t]))
+            plain["plans_solved_batched_not_plain"] = int(sum(1 for t in same if same[t] and not oks[t]))
+        # kernel timing of the plain loop, on a separate pass
         L.fastrrt_profile_reset()
         L.fastrrt_profile_enable(True)
-        plain, oks = run_plain(P, plain_set, args, frame_of)
+        run_plain(P, L, plain_set[:max(2, args.plain_profile_plans)], args, frame_of)
         L.fastrrt_profile_enable(False)
         prof_p = read_prof(L)
-    same = {r.tag: r.ok for rs in results for r in rs if r.tag in oks}
-    if same:
-        plain["success_same_plans_batched"] = float(np.mean(list(same.values())))
-        plain["plans_solved_plain_not_batched"] = int(sum(1 for t in same if oks[t] and not same[t]))
-        plain["plans_solved_batched_not_plain"] = int(sum(1 for t in same if same[t] and not oks[t]))
+    c1 = None
+    if cpu and cpu.get("c1"):
+        c1 = c1_compare(P, L, Batch, cpu["c1"], args, frame_of)
+        cpu["c1"].pop("per_plan", None)
     roof = roofline_pass(L, batch, first, args, frame_of)
     out = {
         "metric": "plans/s at p99<=20 ms (served/batched vs plain loop), 256^2 grid, success %",
@@ -424,6 +573,7 @@ def main():
         "batched": batched, "plain": plain,
         "batched_vs_plain": (round(batched["plans_per_s"] / plain["plans_per_s"], 3) if plain else None),
         "kernel_ms_plain": prof_json(prof_p) if prof_p else None,
+        "c1": c1,
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

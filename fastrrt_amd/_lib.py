@@ -119,7 +119,7 @@ class PlanRequest(ctypes.Structure):
 
 class PlanResult(ctypes.Structure):
     _fields_ = [("user_tag", c_i64), ("status", c_int), ("iterations", c_int), ("path_len", c_int),
-                ("expansions", c_int), ("submit_ms", c_double), ("done_ms", c_double)]
+                ("expansions", c_int), ("submit_ms", c_double), ("done_ms", c_double), ("path_cost", c_float)]
 
 
 SIGNATURES.update({

@@ -50,7 +50,7 @@ class PlanSpec:
 
 
 class Result:
-    __slots__ = ("tag", "status", "iterations", "expansions", "path", "submit_ms", "done_ms")
+    __slots__ = ("tag", "status", "iterations", "expansions", "path", "submit_ms", "done_ms", "path_cost")
 
     @property
     def ok(self):
@@ -112,6 +112,7 @@ class Batch:
             n = res.path_len
             r.path = self._buf[:3 * n].reshape(n, 3).copy() if n > 0 else np.zeros((0, 3), np.float32)
             r.submit_ms, r.done_ms = res.submit_ms, res.done_ms
+            r.path_cost = float(res.path_cost)
             self._keep.pop(r.tag, None)
             out.append(r)
         _lib.check("batch_poll")

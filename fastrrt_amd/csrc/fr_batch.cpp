@@ -606,7 +606,8 @@ class FrBatch {
             slots_[fin_slots_[q]].finalizing = false;
             /* the goal test (graph_best_node.cu:182-222) does not test the
              * footprint, findBestNode does: a reached goal can leave no path */
-            retire(fin_slots_[q], len > 0 ? FR_FOUND : FR_RESULT_NO_PATH, trip.data(), len, c, false);
+            retire(fin_slots_[q], len > 0 ? FR_FOUND : FR_RESULT_NO_PATH, trip.data(), len, c, false,
+                   len > 0 ? p[len - 1].w : 0.0f);
         }
         finalizing_ -= nf;
         fin_slots_.clear();
@@ -615,11 +616,13 @@ class FrBatch {
 
     /* The slot's control block goes back to IDLE on the iteration stream
      * (from a constant pinned block), ahead of any later admission. */
-    void retire(int s, int status, const float *path, int len, const FrCtl &c, bool reset_ctl = true)
+    void retire(int s, int status, const float *path, int len, const FrCtl &c, bool reset_ctl = true,
+                float path_cost = 0.0f)
     {
         Slot &sl = slots_[s];
         Finished f;
         f.res.user_tag = sl.p.req.user_tag;
+        f.res.path_cost = path_cost;
         f.res.status = status;
         f.res.iterations = c.iter;
         f.res.expansions = c.total_expansions;

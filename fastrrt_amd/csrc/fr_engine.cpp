@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <mutex>
 #include <stdexcept>
 
@@ -21,7 +22,7 @@ namespace {
 struct ProfState {
     std::mutex mu;
     bool on = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> done[FR_PROF_N];
+    std::deque<std::pair<hipEvent_t, hipEvent_t>> done[FR_PROF_N];
     hipEvent_t open_ev[FR_PROF_N] = {};
     std::vector<hipEvent_t> pool;
     long launches[FR_PROF_N] = {};
@@ -43,18 +44,36 @@ hipEvent_t prof_event(ProfState &p)
     FR_CHECK(hipEventCreate(&e));
     return e;
 }
+void prof_take(ProfState &p, int k)
+{
+    const auto pr = p.done[k].front();
+    p.done[k].pop_front();
+    float ms = 0;
+    FR_CHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+    p.launches[k] += 1;
+    p.total_ms[k] += ms;
+    p.pool.push_back(pr.first);
+    p.pool.push_back(pr.second);
+}
+/* every recorded pair, waiting for the last ones (reads and resets) */
 void prof_drain(ProfState &p, int k)
 {
-    for (auto &pr : p.done[k]) {
-        FR_CHECK(hipEventSynchronize(pr.second));
-        float ms = 0;
-        FR_CHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
-        p.launches[k] += 1;
-        p.total_ms[k] += ms;
-        p.pool.push_back(pr.first);
-        p.pool.push_back(pr.second);
+    while (!p.done[k].empty()) {
+        FR_CHECK(hipEventSynchronize(p.done[k].front().second));
+        prof_take(p, k);
     }
-    p.done[k].clear();
+}
+/* the pairs whose end event has completed, without waiting: the list stays
+ * short and no launch pays for a long drain */
+void prof_collect(ProfState &p, int k)
+{
+    while (!p.done[k].empty()) {
+        const hipError_t q = hipEventQuery(p.done[k].front().second);
+        if (q == hipErrorNotReady)
+            return;
+        FR_CHECK(q);
+        prof_take(p, k);
+    }
 }
 } // namespace
 
@@ -86,8 +105,7 @@ void fr_prof_end(int k, hipStream_t st)
     FR_CHECK(hipEventRecord(e, st));
     p.done[k].push_back({p.open_ev[k], e});
     p.open_ev[k] = nullptr;
-    if (p.done[k].size() > 4096)
-        prof_drain(p, k);
+    prof_collect(p, k);
 }
 void fr_prof_read(int k, long *launches, double *total_ms)
 {

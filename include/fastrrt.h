@@ -278,6 +278,8 @@ typedef struct fastrrt_plan_result {
     int path_len;             /* waypoints written (-1: the caller's buffer is too small) */
     int expansions;           /* rollouts the search ran */
     double submit_ms, done_ms; /* batch clock */
+    float path_cost;          /* cost of the path's last node (cells traversed plus their intrinsic
+                                 costs, the grid's cost field; 0 when no path) */
 } fastrrt_plan_result;
 
 /* A batch of up to max_plans concurrent plans whose grids total at most

@@ -25,7 +25,7 @@ class Params:
 
 
 class OraclePlanner:
-    def __init__(self, sc, prm, seed, stream, region_side=None):
+    def __init__(self, sc, prm, seed, stream, region_side=None, timeout_ms=-1):
         self.L = O.lib()
         self.sc = sc
         self.r = self.L.og_rrt_new(sc.width, sc.height, sc.perception_m, sc.perception_m, prm.steer_deg,
@@ -36,7 +36,8 @@ class OraclePlanner:
         self.costs = np.ascontiguousarray(sc.class_costs, np.float32)
         self.L.og_rrt_set_plan_data(self.r, O.fptr(self.frame), O.fptr(self.costs), len(self.costs), sc.start[0],
                                     sc.start[1], sc.start[2], sc.goal[0], sc.goal[1], sc.goal[2], prm.v,
-                                    prm.min_dist[0], prm.min_dist[1], -1, prm.max_path, prm.dist_tol, prm.herr)
+                                    prm.min_dist[0], prm.min_dist[1], timeout_ms, prm.max_path, prm.dist_tol,
+                                    prm.herr)
         if region_side:
             self.L.og_set_region_side(self.L.og_rrt_graph(self.r), region_side)
 

@@ -1,0 +1,82 @@
+"""fastrrt_amd.validate (the bench's path checker) against the oracle: the
+footprint verdict equals og_feasible (path_feasible.cu:13-78) on random
+poses of benchmark frames, oracle-planned paths pass every check, and
+corrupted paths fail the check they break."""
+import math
+
+import numpy as np
+
+import _oracle as O
+from _parity import OraclePlanner, Params
+from fastrrt_amd import scenes
+from fastrrt_amd import validate as V
+
+
+def test_footprint_equals_oracle():
+    L = O.lib()
+    rng = np.random.default_rng(8)
+    n = 0
+    for seed, cov in ((1, 0.1), (2, 0.3)):
+        sc = scenes.scene_256(seed, cov)
+        frame = np.ascontiguousarray(sc.frame, np.float32)
+        frame[100:104, 100:110, 2] = 1.0  # pixels feasible at every heading
+        g = O.Graph(256, 256)
+        g.search(22, 40, sc.ego_lb[0], sc.ego_lb[1], sc.ego_ub[0], sc.ego_ub[1])
+        g.costs(sc.class_costs)
+        obst = V.obstacle_mask(frame, sc.class_costs)
+        trav = (np.trunc(frame[:, :, 2]).astype(np.int64) & 1) != 0
+        for _ in range(1500):
+            x, z = int(rng.integers(0, 256)), int(rng.integers(0, 256))
+            h = np.float32(rng.uniform(-math.pi, math.pi))
+            exp = L.og_feasible(g.g, O.fptr(frame), x, z, float(h)) == 1
+            got = V.footprint_free(obst, trav, x, z, h, 11, 20, (sc.ego_lb, sc.ego_ub))
+            assert got == exp, (seed, x, z, h)
+            n += exp
+    assert 100 < n < 2900
+
+
+def test_oracle_paths_pass_and_corruptions_fail():
+    checked = 0
+    for seed in (3, 11, 19, 29):
+        sc = scenes.scene_sized(seed, 160, 0.08, min_dist=(10, 16))
+        prm = Params(min_dist=(10, 16), max_path=14.0, dist_tol=6.0)
+        o = OraclePlanner(sc, prm, 7 + seed, seed)
+        o.search_init()
+        for _ in range(200):
+            if o.goal_reached() or not o.loop():
+                break
+        if not o.goal_reached():
+            continue
+        for r in range(3):
+            path = o.path()
+            fails, rate = V.check_path(sc.frame, sc.class_costs, path, sc.start, sc.goal, (10, 16), None,
+                                       6.0, float(prm.herr))
+            assert not fails, (seed, r, fails)
+            checked += 1
+            if not o.path_optimize():
+                break
+        blocked = np.array(sc.frame, copy=True)
+        blocked[int(path[1, 1]), int(path[1, 0]), 0] = scenes.OBSTACLE_CLASS
+        assert "footprint" in V.check_path(blocked, sc.class_costs, path, sc.start, sc.goal, (10, 16), None, 6.0,
+                                           float(prm.herr))[0]
+        far = np.array(sc.goal, dtype=float)
+        far[1] += 30
+        assert "goal_distance" in V.check_path(sc.frame, sc.class_costs, path, sc.start, far, (10, 16), None, 6.0,
+                                               float(prm.herr))[0]
+    assert checked >= 4
+
+
+def test_heading_continuity_on_a_straight_path():
+    """Waypoints 6 px apart in a column, heading 0, on a free frame: valid;
+    a middle heading turned by 2 rad gives its edges a turning rate of
+    2 sin(1) / (6 - sqrt 2) = 0.37, above the 0.28 an inner edge allows."""
+    frame = np.zeros((128, 64, 3), np.float32)
+    frame[:, :, 0] = scenes.FREE_CLASS
+    path = np.array([[32, 100 - 6 * k, 0.0] for k in range(6)], np.float32)
+    goal = (32, 70, 0.0)
+    fails, rate = V.check_path(frame, scenes.SEGMENTATION_CLASS_COST, path, (32, 100), goal, (10, 16))
+    assert not fails and rate.max() == 0.0
+    bad = path.copy()
+    bad[2, 2] = 2.0
+    fails, rate = V.check_path(frame, scenes.SEGMENTATION_CLASS_COST, bad, (32, 100), goal, (10, 16))
+    assert fails == ["heading_continuity"] and abs(rate[1] - 2 * math.sin(1.0) / (6 - math.sqrt(2))) < 1e-12
