@@ -75,7 +75,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=168, help="plans per step")
+    ap.add_argument("--batch", type=int, default=120, help="plans per step")
     ap.add_argument("--budget-ms", type=float, default=20.0, help="per-plan latency budget (batched deadline)")
     ap.add_argument("--plain-timeout-ms", type=int, default=-1,
                     help="FastRRT handle timeout of the plain form (-1: budget - 2 ms, so that its p99 stays "
@@ -207,21 +207,25 @@ def run_plain(P, L, plans, args, frame_of, timeout_ms=None):
 
 
 def validate_paths(items):
-    """items: (scene, path) of every found plan -> counts of valid plans and
-    of each failed check (fastrrt_amd.validate)."""
+    """items: (tag, scene, path) of every found plan -> counts of valid plans
+    and of each failed check (fastrrt_amd.validate), with the first failing
+    plans' tags."""
     from fastrrt_amd import validate
-    fails, n_ok, worst, short_last = {}, 0, 0.0, 0
-    for sc, path in items:
+    fails, n_ok, worst, short_last, bad = {}, 0, 0.0, 0, []
+    for tag, sc, path in items:
         short_last += math.hypot(float(path[-1, 0] - path[-2, 0]), float(path[-1, 1] - path[-2, 1])) < 2.0
         f, rate = validate.check_path(sc.frame, sc.class_costs, path, sc.start, sc.goal, sc.min_dist,
                                       (sc.ego_lb, sc.ego_ub), 20.0, math.radians(10))
         n_ok += not f
+        if f and len(bad) < 8:
+            bad.append([int(tag)] + f)
         for k in f:
             fails[k] = fails.get(k, 0) + 1
         if rate.size:
             worst = max(worst, float(rate[:-1].max()) if rate.size > 1 else 0.0)
     return {"paths": len(items), "valid": n_ok, "failed_checks": fails,
-            "max_inner_turning_rate_per_px": round(worst, 4), "last_edges_under_2px": short_last}
+            "max_inner_turning_rate_per_px": round(worst, 4), "last_edges_under_2px": short_last,
+            "failed_plans": bad}
 
 
 def run_batched(b, plans, args, frame_of, max_iter=None, opt_rounds=None, deadline=None):
@@ -524,7 +528,7 @@ def main():
                "wall_ms_per_step": float(np.mean(walls)),
                "found_p50_p99_max_ms": [round(float(np.percentile(lat_ok, q)), 3) for q in (50, 99, 100)] if lat_ok else None,
                "failed_p50_p99_max_ms": [round(float(np.percentile(lat_fail, q)), 3) for q in (50, 99, 100)] if lat_fail else None,
-               "validation": validate_paths([(scene_of[r.tag], r.path) for r in found_b])}
+               "validation": validate_paths([(r.tag, scene_of[r.tag], r.path) for r in found_b])}
     # plain form on the plans of the timed steps (no kernel timing: the
     # comparison's latencies are the handle's own)
     plain_set = [pl for k in range(args.warmup, args.warmup + min(max(args.plain_steps, 1), args.steps))
@@ -535,7 +539,7 @@ def main():
     prof_p = None
     if plain_set:
         plain, paths_p, costs_p = run_plain(P, L, plain_set, args, frame_of)
-        plain["validation"] = validate_paths([(scene_of[t], p) for t, p in paths_p.items() if p is not None])
+        plain["validation"] = validate_paths([(t, scene_of[t], p) for t, p in paths_p.items() if p is not None])
         same = {r.tag: r.ok for rs in results for r in rs if r.tag in paths_p}
         if same:
             oks = {t: paths_p[t] is not None for t in same}

@@ -7,7 +7,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libfastrrt.so")
+# FASTRRT_LIB names another build of the library (an instrumented or
+# experimental one); the default is the in-tree build
+LIB_PATH = os.environ.get("FASTRRT_LIB") or os.path.join(HERE, "lib", "libfastrrt.so")
 
 c_int, c_float, c_bool, c_void_p = ctypes.c_int, ctypes.c_float, ctypes.c_bool, ctypes.c_void_p
 c_char_p, c_u64, c_i64, c_double = ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int64, ctypes.c_double

@@ -763,7 +763,15 @@ __device__ __forceinline__ float dev_wave_curve(const FrPlanDev &P, const FrHerm
             FrOpen o;
             if (acc) {
                 add = (P.intrinsic ? P.intrinsic[cz * W + cx] : 0.0f) + 1.0f;
-                if (!skip_fp) {
+                /* a point whose centre distance certifies every heading (or
+                 * whose pixel is feasible at every heading) needs no heading:
+                 * the derivative, atan2 and sincos run only for the others.
+                 * With one point (t = 0/0) the heading is NaN, which the
+                 * certificate does not cover. */
+                const bool cert = !skip_fp && num_points > 1 &&
+                                  ((P.trav[cz * W + cx] & 1) ||
+                                   P.dist[fr_dist_at(P.TW, cx + P.pad, cz + P.pad)] >= P.pad);
+                if (!skip_fp && !cert) {
                     double ddx, ddz;
                     dev_curve_d1(hm, t, &ddx, &ddz);
                     const float hd = (float)((double)fr_atan2f((float)ddz, (float)ddx) + FR_HALF_PI_D);
@@ -1254,7 +1262,9 @@ __device__ bool dev_roll_flush(const FrPlanDev &P, const int *bx, const int *bz,
     if (lane < m) {
         const int cx = bx[lane], cz = bz[lane];
         add = (P.intrinsic ? P.intrinsic[cz * P.W + cx] : 0.0f) + 1.0f;
-        pr = dev_lane_probe(P, cx, cz, bh[lane], &o);
+        /* certified cells skip the sincos and probe loads */
+        if (!dev_fp_certified(P, cx, cz, bh[lane]))
+            pr = dev_lane_probe(P, cx, cz, bh[lane], &o);
         set.x[lane] = cx;
         set.z[lane] = cz;
         set.s[lane] = o.s;

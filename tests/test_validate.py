@@ -80,3 +80,35 @@ def test_heading_continuity_on_a_straight_path():
     bad[2, 2] = 2.0
     fails, rate = V.check_path(frame, scenes.SEGMENTATION_CLASS_COST, bad, (32, 100), goal, (10, 16))
     assert fails == ["heading_continuity"] and abs(rate[1] - 2 * math.sin(1.0) / (6 - math.sqrt(2))) < 1e-12
+
+
+def test_hermite_edge_equals_oracle_connection():
+    """validate.hermite_edge_ok restates checkDirectConnectionToGoal
+    (graph_direct_goal_connection.cu:16-167): its verdict equals the
+    oracle's og_check_direct_connection (cost >= 0) on random pairs of a
+    benchmark frame, curvature bounds 0.8 and the physical one."""
+    L = O.lib()
+    rng = np.random.default_rng(21)
+    sc = scenes.scene_256(4, 0.12)
+    frame = np.ascontiguousarray(sc.frame, np.float32)
+    g = O.Graph(256, 256)
+    g.physical(34.64, 34.64, 40.0, 5.412658773)
+    g.search(22, 40, sc.ego_lb[0], sc.ego_lb[1], sc.ego_ub[0], sc.ego_ub[1])
+    g.costs(sc.class_costs)
+    obst = V.obstacle_mask(frame, sc.class_costs)
+    trav = (np.trunc(frame[:, :, 2]).astype(np.int64) & 1) != 0
+    inner, last = V.max_turning_rates()
+    n_ok = 0
+    for k in range(600):
+        x, z = int(rng.integers(0, 256)), int(rng.integers(0, 256))
+        gx = int(np.clip(x + rng.integers(-40, 41), 0, 255))
+        gz = int(np.clip(z + rng.integers(-40, 41), 0, 255))
+        h1 = float(np.float32(rng.uniform(-math.pi, math.pi)))
+        h2 = float(np.float32(rng.uniform(-math.pi, math.pi))) if k % 2 else h1
+        mc = float(np.float32(last if k % 3 else inner))
+        g.add(x, z, h1, -1, -1, 0)
+        c = L.og_check_direct_connection(g.g, O.fptr(frame), mc, x, z, h1, gx, gz, h2, 0, 0)
+        got = V.hermite_edge_ok(obst, trav, (x, z, h1), (gx, gz, h2), mc, 11, 20, (sc.ego_lb, sc.ego_ub))
+        assert got == (c >= 0), (x, z, h1, gx, gz, h2, mc, c)
+        n_ok += got
+    assert 20 < n_ok < 580
