@@ -56,6 +56,21 @@ class ScenePool:
         return lst[int(rng.integers(0, len(lst)))]
 
 
+def validate_all(items, dist_tol=20.0):
+    """(scene, path) of every found plan -> {"paths", "valid", "failed"}:
+    each path checked by fastrrt_amd.validate (chain, goal tolerance, every
+    edge's footprints or Hermite connection, heading continuity)."""
+    from . import validate
+    failed, n_ok = [], 0
+    for sc, path in items:
+        f, _ = validate.check_path(sc.frame, sc.class_costs, path, sc.start, sc.goal, sc.min_dist,
+                                   (sc.ego_lb, sc.ego_ub), dist_tol, math.radians(10))
+        n_ok += not f
+        if f and len(failed) < 8:
+            failed.append(f)
+    return {"paths": len(items), "valid": n_ok, "failed": failed}
+
+
 def _spec(sc, frame, tag, stream, deadline_ms, max_iter, opt_rounds, seed):
     return PlanSpec(frame, sc.start, sc.goal, min_dist=sc.min_dist, ego_lb=sc.ego_lb, ego_ub=sc.ego_ub,
                     max_iterations=max_iter, optimize_rounds=opt_rounds, deadline_ms=deadline_ms, seed=seed,
@@ -110,7 +125,9 @@ def run_served(duration_s=10.0, rate=2000.0, sizes=SERVED_SIZES, deadlines=DEADL
         d[1] += int(r.ok)
         d[2] += int(miss[-1])
     lat = np.array(lat)
-    return {"mode": "served", "requests": n, "duration_s": duration_s, "offered_rate": rate,
+    valid = validate_all([(reqs[t][2], r.path) for t, r in results.items() if r.ok])
+    return {"mode": "served", "requests": n, "returned": len(results), "validation": valid,
+            "duration_s": duration_s, "offered_rate": rate,
             "completed_per_s": n / wall, "found_per_s": float(np.sum(ok)) / wall,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "deadline_miss_rate": float(np.mean(miss)), "success_rate": float(np.mean(ok)),
@@ -143,7 +160,10 @@ def run_batched(n_plans=512, sizes=BATCHED_SIZES, seed=12, max_iter=200, opt_rou
         d[0] += 1
         d[1] += int(r.ok)
         d[2] += r.iterations
-    return {"mode": "batched", "plans": n_plans, "cells": cells, "wall_ms": wall * 1e3, "plans_per_s": n_plans / wall,
+    valid = validate_all([(plans[r.tag][1], r.path) for r in res if r.ok])
+    return {"mode": "batched", "plans": n_plans, "returned": len(res), "validation": valid,
+            "statuses": sorted(set(int(r.status) for r in res)),
+            "cells": cells, "wall_ms": wall * 1e3, "plans_per_s": n_plans / wall,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "success_rate": float(np.mean([r.ok for r in res])),
             "mean_iterations": float(np.mean([r.iterations for r in res])),
@@ -170,7 +190,7 @@ def run_1024(n_plans=24, seed=13, timeout_ms=300.0, smart=True, region_side=32, 
                           -1.0, sc0.class_costs)
         rrt = P.FastRRT(ego)
         rrt.set_density_region(region_side)
-        lat, ok, iters, loop_ms = [], [], [], 0.0
+        lat, ok, iters, loop_ms, paths = [], [], [], 0.0, []
         t_all = time.perf_counter()
         for i, sc in enumerate(scs):
             sp = P.SearchParams(frames[i], P.Waypoint(sc.start[0], sc.start[1], P.angle(sc.start[2])),
@@ -194,13 +214,16 @@ def run_1024(n_plans=24, seed=13, timeout_ms=300.0, smart=True, region_side=32, 
             t3 = time.perf_counter()
             lat.append((t3 - t0) * 1e3)
             ok.append(bool(good and path))
+            if good and path:
+                paths.append((sc, np.array([[w.x, w.z, w.heading.rad()] for w in path], np.float32)))
             iters.append(it)
             loop_ms += (t2 - t1) * 1e3
         wall = time.perf_counter() - t_all
         out["plain"] = {"plans_per_s": n_plans / wall, "p50_ms": float(np.percentile(lat, 50)),
                         "p99_ms": float(np.percentile(lat, 99)), "success": float(np.mean(ok)),
                         "mean_iterations": float(np.mean(iters)),
-                        "iteration_ms": loop_ms / max(1, sum(iters))}
+                        "iteration_ms": loop_ms / max(1, sum(iters)),
+                        "validation": validate_all(paths)}
     if "batched" in forms:
         b = Batch(max_plans=n_plans, max_cells=n_plans * 1024 * 1024, launch="auto")
         res = None
@@ -216,7 +239,8 @@ def run_1024(n_plans=24, seed=13, timeout_ms=300.0, smart=True, region_side=32, 
         lat = np.array([r.latency_ms for r in res])
         out["batched"] = {"plans_per_s": n_plans / wall, "p50_ms": float(np.percentile(lat, 50)),
                           "p99_ms": float(np.percentile(lat, 99)), "success": float(np.mean([r.ok for r in res])),
-                          "mean_iterations": float(np.mean([r.iterations for r in res]))}
+                          "mean_iterations": float(np.mean([r.iterations for r in res])),
+                          "validation": validate_all([(scs[r.tag], r.path) for r in res if r.ok])}
     return out
 
 
