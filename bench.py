@@ -528,7 +528,8 @@ def main():
                "wall_ms_per_step": float(np.mean(walls)),
                "found_p50_p99_max_ms": [round(float(np.percentile(lat_ok, q)), 3) for q in (50, 99, 100)] if lat_ok else None,
                "failed_p50_p99_max_ms": [round(float(np.percentile(lat_fail, q)), 3) for q in (50, 99, 100)] if lat_fail else None,
-               "validation": validate_paths([(r.tag, scene_of[r.tag], r.path) for r in found_b])}
+               "validation": validate_paths([(r.tag, scene_of[r.tag], r.path) for r in found_b]),
+               "hbm": batch.memory()}
     # plain form on the plans of the timed steps (no kernel timing: the
     # comparison's latencies are the handle's own)
     plain_set = [pl for k in range(args.warmup, args.warmup + min(max(args.plain_steps, 1), args.steps))

@@ -119,6 +119,14 @@ class PlanRequest(ctypes.Structure):
     ]
 
 
+class BatchHbm(ctypes.Structure):
+    _fields_ = [("total_bytes", ctypes.c_uint64), ("free_bytes_at_create", ctypes.c_uint64),
+                ("free_bytes_last", ctypes.c_uint64), ("reserve_bytes", ctypes.c_uint64),
+                ("engine_bytes", ctypes.c_uint64), ("slab_bytes", ctypes.c_uint64),
+                ("requested_cells", c_i64), ("cell_budget", c_i64), ("cells_in_use", c_i64),
+                ("memory_stalls", c_i64), ("running", ctypes.c_int32), ("queued", ctypes.c_int32)]
+
+
 class PlanResult(ctypes.Structure):
     _fields_ = [("user_tag", c_i64), ("status", c_int), ("iterations", c_int), ("path_len", c_int),
                 ("expansions", c_int), ("submit_ms", c_double), ("done_ms", c_double), ("path_cost", c_float)]
@@ -134,6 +142,8 @@ SIGNATURES.update({
     "fastrrt_batch_clock_ms": (c_double, [c_void_p]),
     "fastrrt_batch_set_launch": (c_bool, [c_void_p, c_int, c_int, c_int]),
     "fastrrt_batch_get_launch": (c_bool, [c_void_p, ctypes.POINTER(c_int)]),
+    "fastrrt_batch_memory": (c_bool, [c_void_p, ctypes.c_void_p]),
+    "fastrrt_batch_set_hbm_reserve": (c_bool, [c_void_p, ctypes.c_uint64]),
 })
 
 _lib = None

@@ -132,6 +132,21 @@ class Batch:
         if not _lib.load().fastrrt_batch_set_launch(self._ptr, int(grid_waves), int(grid_tiles), int(grid_items)):
             _lib.check("batch_set_launch")
 
+    def memory(self):
+        """The batch's device memory as measured (fastrrt_batch_memory): free
+        and total bytes, the reserve, what the batch holds, the cell budget
+        and the admissions held back by memory."""
+        h = _lib.BatchHbm()
+        if not _lib.load().fastrrt_batch_memory(self._ptr, ctypes.byref(h)):
+            _lib.check("batch_memory")
+        return {name: int(getattr(h, name)) for name, _ in h._fields_}
+
+    def set_hbm_reserve(self, nbytes):
+        """Bytes left free beside the batch when new plans' planes are
+        allocated (default 2 GiB, at most a 16th of the device)."""
+        if not _lib.load().fastrrt_batch_set_hbm_reserve(self._ptr, int(nbytes)):
+            _lib.check("batch_set_hbm_reserve")
+
     def launch(self):
         """(grid_waves, grid_tiles, grid_items) of the iteration graph"""
         out = (ctypes.c_int * 3)()

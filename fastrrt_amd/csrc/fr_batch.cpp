@@ -24,6 +24,7 @@
 #include <map>
 #include <memory>
 #include <stdexcept>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -64,6 +65,42 @@ struct Slot {
     long cells = 0;
 };
 
+/* Device memory as the batch measured it (hipMemGetInfo) and what it holds. */
+struct Hbm {
+    size_t total = 0, free_at_create = 0, free_last = 0, reserve = 0, engine = 0;
+    long requested = 0, budget = 0, stalls = 0;
+};
+
+/* Memory left to the rest of the process by default: 2 GiB, at most a 16th
+ * of the device. */
+const size_t kHbmReserve = 2ull << 30;
+
+/* The cell budget from the measured free memory: the engine's fixed part
+ * first, then cells at the bytes a cell costs in a plan's planes (a 256 x
+ * 256 grid with a 22 x 40 footprint: its distance planes and path buffers
+ * included) plus the engine's per-cell queues; at most `max_cells`. */
+Hbm measure_hbm(int max_plans, long max_cells)
+{
+    if (max_plans <= 0 || max_cells <= 0)
+        throw std::runtime_error("invalid batch capacity");
+    Hbm h;
+    FR_CHECK(hipMemGetInfo(&h.free_at_create, &h.total));
+    h.free_last = h.free_at_create;
+    h.reserve = std::min<size_t>(kHbmReserve, h.total / 16);
+    h.requested = max_cells;
+    const double slab_pc = (double)FrSlab::bytes_for(256, 256, 11, 20, 256 * 256 + 1, false) / 65536.0;
+    const size_t e1 = FrEngine::bytes_for(max_plans + 1, 2);
+    const double eng_pc = (double)(FrEngine::bytes_for(max_plans + 1, (1L << 20) + 2) - e1) / (double)(1L << 20);
+    const double avail = (double)h.free_at_create - (double)h.reserve - (double)e1;
+    const long fit = avail > 0 ? (long)(avail / (slab_pc + eng_pc)) : 0;
+    h.budget = std::min(max_cells, fit);
+    if (h.budget < 1)
+        throw std::runtime_error("device memory: " + std::to_string(h.free_at_create) + " bytes free, " +
+                                 std::to_string(h.reserve) + " reserved: no room for a batch");
+    h.engine = FrEngine::bytes_for(max_plans + 1, h.budget + 1);
+    return h;
+}
+
 } // namespace
 
 static_assert(sizeof(FrCtl) <= FR_PACK_PATH_OFF, "control block must fit before the path of a pinned record");
@@ -72,12 +109,12 @@ class FrBatch {
   public:
     /* The engine has one slot more than the batch: a plan that never runs,
      * whose control block stays IDLE, pads the finalisation list. */
+    /* The cell budget is max_cells or what the measured free device memory
+     * holds, whichever is smaller (measure_hbm). */
     FrBatch(int max_plans, long max_cells)
-        : eng_(max_plans + 1, std::max<long>(max_cells, 1) + 1), max_cells_(max_cells), slots_(max_plans),
-          dummy_(max_plans), fin_max_(std::min(max_plans, 64))
+        : hbm_(measure_hbm(max_plans, max_cells)), eng_(max_plans + 1, hbm_.budget + 1), max_cells_(hbm_.budget),
+          slots_(max_plans), dummy_(max_plans), fin_max_(std::min(max_plans, 64))
     {
-        if (max_plans <= 0 || max_cells <= 0)
-            throw std::runtime_error("invalid batch capacity");
         try {
             allocate(max_plans);
         } catch (...) {
@@ -166,7 +203,8 @@ class FrBatch {
         if (!f)
             throw std::runtime_error("request without a frame");
         if ((long)f->W * f->H > max_cells_)
-            throw std::runtime_error("request grid exceeds the batch cell budget");
+            throw std::runtime_error("request grid exceeds the batch cell budget (" + std::to_string(max_cells_) +
+                                     " cells)");
         if (r.region_side != 0 && r.region_side < 16)
             throw std::runtime_error("density region side must be 0 (128 px) or at least 16 px");
         Pending p;
@@ -186,6 +224,28 @@ class FrBatch {
         FR_CHECK(hipStreamSynchronize(eng_.stream()));
         eng_.set_grids(waves, tiles, items);
         drop_graphs();
+    }
+
+    void set_hbm_reserve(size_t bytes) { hbm_.reserve = bytes; }
+
+    void memory(fastrrt_batch_hbm *o)
+    {
+        size_t fr = 0, tot = 0;
+        FR_CHECK(hipMemGetInfo(&fr, &tot));
+        hbm_.free_last = fr;
+        std::memset(o, 0, sizeof(*o));
+        o->total_bytes = hbm_.total;
+        o->free_bytes_at_create = hbm_.free_at_create;
+        o->free_bytes_last = hbm_.free_last;
+        o->reserve_bytes = hbm_.reserve;
+        o->engine_bytes = hbm_.engine;
+        o->slab_bytes = slab_bytes_;
+        o->requested_cells = hbm_.requested;
+        o->cell_budget = hbm_.budget;
+        o->cells_in_use = cells_in_use_;
+        o->memory_stalls = hbm_.stalls;
+        o->running = running_;
+        o->queued = (int)queue_.size();
     }
 
     void get_launch(int *out) const
@@ -299,6 +359,10 @@ class FrBatch {
         p.region_side = r.region_side > 0 ? r.region_side : 128;
     }
 
+    /* A plan's planes: a pooled slab of its size class, or a new one when
+     * the measured free memory holds it beside the reserve (pooled slabs of
+     * other classes are freed first if it does not).  Null when it does not
+     * fit: the plan waits until a running plan's slab is given back. */
     std::unique_ptr<FrSlab> take_slab(int W, int H, int mx, int mz)
     {
         auto key = std::make_tuple(W, H, mx, mz);
@@ -307,11 +371,47 @@ class FrBatch {
         if (it != pool_.end() && !it->second.empty()) {
             s = std::move(it->second.back());
             it->second.pop_back(); /* reset on the device at admission (k_slab_reset) */
-        } else {
-            s.reset(new FrSlab());
-            s->allocate(W, H, mx, mz, path_cap(W, H), false, eng_.stream());
+            return s;
         }
+        const size_t need = FrSlab::bytes_for(W, H, mx, mz, path_cap(W, H), false);
+        if (!hbm_fits(need)) {
+            if (!release_pools())
+                return nullptr;
+            if (!hbm_fits(need))
+                return nullptr;
+        }
+        s.reset(new FrSlab());
+        s->allocate(W, H, mx, mz, path_cap(W, H), false, eng_.stream());
+        slab_bytes_ += need;
         return s;
+    }
+
+    bool hbm_fits(size_t need)
+    {
+        size_t fr = 0, tot = 0;
+        FR_CHECK(hipMemGetInfo(&fr, &tot));
+        hbm_.free_last = fr;
+        return fr >= need + hbm_.reserve;
+    }
+
+    /* Frees every pooled slab (after the queued device work, which may still
+     * read a slab retired since); false when the pool was empty. */
+    bool release_pools()
+    {
+        bool any = false;
+        for (auto &kv : pool_)
+            any = any || !kv.second.empty();
+        if (!any)
+            return false;
+        FR_CHECK(hipStreamSynchronize(eng_.stream()));
+        FR_CHECK(hipStreamSynchronize(st2_));
+        for (auto &kv : pool_) {
+            for (auto &sl : kv.second)
+                slab_bytes_ -= std::min(slab_bytes_, FrSlab::bytes_for(sl->W, sl->H, sl->mx, sl->mz, sl->path_cap,
+                                                                        sl->with_intrinsic));
+            kv.second.clear();
+        }
+        return true;
     }
 
     static int path_cap(int W, int H) { return std::min(W * H + 1, 1 << 16); }
@@ -322,8 +422,12 @@ class FrBatch {
         /* keep every slab of the batch's size classes: freeing device memory
          * synchronises the device, and the cell budget bounds the pool */
         auto &v = pool_[key];
-        if ((long)v.size() * s->W * s->H < max_cells_)
+        if ((long)v.size() * s->W * s->H < max_cells_) {
             v.push_back(std::move(s));
+            return;
+        }
+        slab_bytes_ -= std::min(slab_bytes_, FrSlab::bytes_for(s->W, s->H, s->mx, s->mz, s->path_cap,
+                                                                s->with_intrinsic));
     }
 
     void admit()
@@ -343,14 +447,26 @@ class FrBatch {
             const long cells = (long)f->W * f->H;
             if (cells_in_use_ + cells > max_cells_)
                 break;
+            const int mx = fr_trunc_f((float)p.req.min_dist_x / 2.0f);
+            const int mz = fr_trunc_f((float)p.req.min_dist_z / 2.0f);
+            std::unique_ptr<FrSlab> slab = take_slab(f->W, f->H, mx, mz);
+            if (!slab) {
+                hbm_.stalls += 1;
+                if (running_ == 0 && admitted.empty()) {
+                    /* nothing running will give memory back */
+                    const int W = f->W, H = f->H;
+                    queue_.pop_front();
+                    throw std::runtime_error("device memory: a " + std::to_string(W) + " x " + std::to_string(H) +
+                                             " plan does not fit in the measured free memory beside the reserve");
+                }
+                break;
+            }
             Slot &s = slots_[free_slot];
             FrPlanParams prm;
             fill_params(prm, p.req, f);
             prm.gc_lazy = 1;
             prm.opt_rounds = p.req.optimize_rounds;
-            const int mx = fr_trunc_f((float)p.req.min_dist_x / 2.0f);
-            const int mz = fr_trunc_f((float)p.req.min_dist_z / 2.0f);
-            s.slab = take_slab(f->W, f->H, mx, mz);
+            s.slab = std::move(slab);
             FrFrameView v;
             v.xyz = f->d_xyz;
             v.costs = f->d_costs;
@@ -642,9 +758,11 @@ class FrBatch {
         running_ -= 1;
     }
 
+    Hbm hbm_;
     FrEngine eng_;
     long max_cells_;
     long cells_in_use_ = 0;
+    size_t slab_bytes_ = 0; /* plans' planes allocated, in use or pooled */
     std::vector<Slot> slots_;
     std::deque<Pending> queue_;
     std::deque<Finished> done_;
@@ -748,6 +866,24 @@ bool fastrrt_batch_set_launch(void *batch, int grid_waves, int grid_tiles, int g
 {
     FR_TRY(false, {
         B(batch)->set_launch(grid_waves, grid_tiles, grid_items);
+        return true;
+    })
+}
+
+bool fastrrt_batch_memory(void *batch, fastrrt_batch_hbm *out)
+{
+    FR_TRY(false, {
+        if (!out)
+            throw std::runtime_error("null output");
+        B(batch)->memory(out);
+        return true;
+    })
+}
+
+bool fastrrt_batch_set_hbm_reserve(void *batch, uint64_t bytes)
+{
+    FR_TRY(false, {
+        B(batch)->set_hbm_reserve((size_t)bytes);
         return true;
     })
 }

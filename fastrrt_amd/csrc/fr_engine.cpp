@@ -155,6 +155,23 @@ void FrSlab::release()
 /* The padded distance planes depend on the footprint; they are reallocated
  * alone so that the node grid survives a footprint change, as the reference's
  * grid survives set_plan_data. */
+/* Distance planes of a footprint: the 8 x 8-tiled plane and its untiled
+ * scratch, over the grid padded by the footprint's reach. */
+static void fr_dist_geometry(int W, int H, int mx, int mz, int *pad, int *PW, int *PH, int *TW, size_t *tiled,
+                             size_t *pwh)
+{
+    const double hyp = std::sqrt((double)mx * mx + (double)mz * mz);
+    *pad = (int)std::ceil(hyp) + 2;
+    *PW = W + 2 * *pad;
+    *PH = H + 2 * *pad;
+    *TW = (*PW + 7) / 8;
+    *pwh = (size_t)*PW * *PH;
+    *tiled = (size_t)*TW * ((*PH + 7) / 8) * 64; /* whole 8 x 8 tiles */
+}
+
+/* The padded distance planes depend on the footprint; they are reallocated
+ * alone so that the node grid survives a footprint change, as the reference's
+ * grid survives set_plan_data. */
 void FrSlab::set_footprint(int mx_, int mz_, hipStream_t st)
 {
     if (dist_base_ && mx_ == mx && mz_ == mz)
@@ -164,17 +181,64 @@ void FrSlab::set_footprint(int mx_, int mz_, hipStream_t st)
     dist_base_ = nullptr;
     mx = mx_;
     mz = mz_;
-    const double hyp = std::sqrt((double)mx * mx + (double)mz * mz);
-    pad = (int)std::ceil(hyp) + 2;
-    PW = W + 2 * pad;
-    PH = H + 2 * pad;
-    TW = (PW + 7) / 8;
-    const size_t PWH = (size_t)PW * PH;
-    const size_t tiled = (size_t)TW * ((PH + 7) / 8) * 64; /* whole 8 x 8 tiles */
+    size_t tiled = 0, PWH = 0;
+    fr_dist_geometry(W, H, mx, mz, &pad, &PW, &PH, &TW, &tiled, &PWH);
     FR_CHECK(hipMalloc(&dist_base_, fr_align(tiled) + fr_align(PWH)));
     dist = (uint8_t *)dist_base_;
     dist_tmp = (uint8_t *)dist_base_ + fr_align(tiled);
     FR_CHECK(hipMemsetAsync(dist_base_, 0, fr_align(tiled) + fr_align(PWH), st));
+}
+
+/* Offsets of the planes in the slab's one allocation (a plane per field,
+ * each 256-byte aligned). */
+struct FrSlabLayout {
+    size_t type, derive, px, pz, heading, cost, intr, direct, gcf, gch, gcc, done, bits, mark, trav, region;
+    size_t zero_end, path, pd, claim, escr, bytes;
+};
+
+static FrSlabLayout fr_slab_layout(int W, int H, int path_cap, bool with_intrinsic)
+{
+    const size_t N = (size_t)W * H;
+    const int dens = (W / 16 + 1) * (H / 16 + 1); /* fits every region side >= 16 */
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        size_t o = off;
+        off += fr_align(b);
+        return o;
+    };
+    FrSlabLayout l;
+    l.type = take(N);
+    l.derive = take(4 * N);
+    l.px = take(4 * N);
+    l.pz = take(4 * N);
+    l.heading = take(4 * N);
+    l.cost = take(4 * N);
+    l.intr = with_intrinsic ? take(4 * N) : 0;
+    l.direct = with_intrinsic ? take(4 * N) : 0;
+    l.gcf = take(N);
+    l.gch = take(4 * N);
+    l.gcc = take(4 * N);
+    l.done = take(4 * ((N + 31) / 32));
+    l.bits = take(4 * ((N + 31) / 32));
+    l.mark = take(N);
+    l.trav = take(N);
+    l.region = take(4 * (size_t)2 * dens);
+    l.zero_end = off; /* everything above starts at zero */
+    l.path = take(16 * (size_t)path_cap); /* written before read */
+    l.pd = take(16 * (size_t)path_cap);
+    l.claim = take(8 * N);
+    l.escr = take(16 * N); /* erase scratch, written before read */
+    l.bytes = off;
+    return l;
+}
+
+size_t FrSlab::bytes_for(int W, int H, int mx, int mz, int path_cap, bool with_intrinsic)
+{
+    const int cap = path_cap > 0 ? path_cap : W * H + 1;
+    int pad, PW, PH, TW;
+    size_t tiled = 0, PWH = 0;
+    fr_dist_geometry(W, H, mx, mz, &pad, &PW, &PH, &TW, &tiled, &PWH);
+    return fr_slab_layout(W, H, cap, with_intrinsic).bytes + fr_align(tiled) + fr_align(PWH);
 }
 
 void FrSlab::allocate(int W_, int H_, int mx_, int mz_, int path_cap_, bool with_intrinsic_, hipStream_t st)
@@ -186,59 +250,31 @@ void FrSlab::allocate(int W_, int H_, int mx_, int mz_, int path_cap_, bool with
     mz = mz_;
     with_intrinsic = with_intrinsic_;
     path_cap = path_cap_ > 0 ? path_cap_ : W * H + 1;
-    const size_t N = (size_t)W * H;
-    const int dens = (W / 16 + 1) * (H / 16 + 1); /* fits every region side >= 16 */
-    size_t off = 0;
-    auto take = [&](size_t b) {
-        size_t o = off;
-        off += fr_align(b);
-        return o;
-    };
-    const size_t o_type = take(N);
-    const size_t o_derive = take(4 * N);
-    const size_t o_px = take(4 * N);
-    const size_t o_pz = take(4 * N);
-    const size_t o_heading = take(4 * N);
-    const size_t o_cost = take(4 * N);
-    const size_t o_intr = with_intrinsic ? take(4 * N) : 0;
-    const size_t o_direct = with_intrinsic ? take(4 * N) : 0;
-    const size_t o_gcf = take(N);
-    const size_t o_gch = take(4 * N);
-    const size_t o_gcc = take(4 * N);
-    const size_t o_done = take(4 * ((N + 31) / 32));
-    const size_t o_bits = take(4 * ((N + 31) / 32));
-    const size_t o_mark = take(N);
-    const size_t o_trav = take(N);
-    const size_t o_region = take(4 * (size_t)2 * dens);
-    const size_t zero_end = off; /* everything above starts at zero */
-    const size_t o_path = take(16 * (size_t)path_cap); /* written before read */
-    const size_t o_pd = take(16 * (size_t)path_cap);
-    const size_t o_claim = take(8 * N);
-    const size_t o_escr = take(16 * N); /* erase scratch, written before read */
-    bytes_ = off;
+    const FrSlabLayout l = fr_slab_layout(W, H, path_cap, with_intrinsic);
+    bytes_ = l.bytes;
     FR_CHECK(hipMalloc(&base_, bytes_));
     char *b = (char *)base_;
-    type = (uint8_t *)(b + o_type);
-    derive = (int32_t *)(b + o_derive);
-    px = (int32_t *)(b + o_px);
-    pz = (int32_t *)(b + o_pz);
-    heading = (float *)(b + o_heading);
-    cost = (float *)(b + o_cost);
-    intrinsic = with_intrinsic ? (float *)(b + o_intr) : nullptr;
-    direct = with_intrinsic ? (float *)(b + o_direct) : nullptr;
-    gc_flag = (uint8_t *)(b + o_gcf);
-    gc_heading = (float *)(b + o_gch);
-    gc_cost = (float *)(b + o_gcc);
-    gc_done = (uint32_t *)(b + o_done);
-    gc_bits = (uint32_t *)(b + o_bits);
-    mark = (uint8_t *)(b + o_mark);
-    trav = (uint8_t *)(b + o_trav);
-    region = (uint32_t *)(b + o_region);
-    path = (float4 *)(b + o_path);
-    pd = (float4 *)(b + o_pd);
-    claim = (unsigned long long *)(b + o_claim);
-    escr = (int32_t *)(b + o_escr);
-    zero_end_ = zero_end;
+    type = (uint8_t *)(b + l.type);
+    derive = (int32_t *)(b + l.derive);
+    px = (int32_t *)(b + l.px);
+    pz = (int32_t *)(b + l.pz);
+    heading = (float *)(b + l.heading);
+    cost = (float *)(b + l.cost);
+    intrinsic = with_intrinsic ? (float *)(b + l.intr) : nullptr;
+    direct = with_intrinsic ? (float *)(b + l.direct) : nullptr;
+    gc_flag = (uint8_t *)(b + l.gcf);
+    gc_heading = (float *)(b + l.gch);
+    gc_cost = (float *)(b + l.gcc);
+    gc_done = (uint32_t *)(b + l.done);
+    gc_bits = (uint32_t *)(b + l.bits);
+    mark = (uint8_t *)(b + l.mark);
+    trav = (uint8_t *)(b + l.trav);
+    region = (uint32_t *)(b + l.region);
+    path = (float4 *)(b + l.path);
+    pd = (float4 *)(b + l.pd);
+    claim = (unsigned long long *)(b + l.claim);
+    escr = (int32_t *)(b + l.escr);
+    zero_end_ = l.zero_end;
     reset(st);
     mx = -1;
     mz = -1;
@@ -277,10 +313,36 @@ FrEngine::FrEngine(int max_slots, long queue_cap, hipStream_t stream)
     }
 }
 
+/* Device buffers of an engine: descriptors, control blocks, the active
+ * table and the per-iteration queues (sized by the cell capacity). */
+struct FrEngineSizes {
+    long max_tiles, pair_cap, demand_cap;
+    size_t act_tiles_off, act_bytes, bytes;
+};
+
+static FrEngineSizes fr_engine_sizes(int max_slots, long queue_cap)
+{
+    FrEngineSizes z;
+    z.max_tiles = queue_cap / FR_TILE + max_slots + 1;
+    z.act_tiles_off = sizeof(int) * (4 + (size_t)(max_slots + 1) / 2 * 2);
+    z.act_bytes = z.act_tiles_off + sizeof(int2) * (size_t)z.max_tiles;
+    z.pair_cap = std::min<long>(std::max<long>(16 * queue_cap, 1L << 16), 1L << 26);
+    z.demand_cap = queue_cap + 1;
+    z.bytes = sizeof(FrPlanDev) * (size_t)max_slots + sizeof(FrCtl) * (size_t)max_slots + z.act_bytes +
+              sizeof(int) * (size_t)max_slots + sizeof(int) * FR_CNT_N + sizeof(FrExp) * (size_t)(queue_cap + 1) +
+              sizeof(FrFresh) * (size_t)(2 * queue_cap + max_slots + 1) + sizeof(int) * 64 +
+              sizeof(FrPair) * (size_t)z.pair_cap + 2 * sizeof(int2) * (size_t)z.demand_cap +
+              sizeof(FrGcGeo) * (size_t)z.demand_cap;
+    return z;
+}
+
+size_t FrEngine::bytes_for(int max_slots, long queue_cap) { return fr_engine_sizes(max_slots, queue_cap).bytes; }
+
 void FrEngine::allocate()
 {
     const int max_slots = max_slots_;
     const long queue_cap = queue_cap_;
+    const FrEngineSizes z = fr_engine_sizes(max_slots, queue_cap);
     if (!st_) {
         FR_CHECK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
         own_stream_ = true;
@@ -294,12 +356,12 @@ void FrEngine::allocate()
         FR_CHECK(hipEventCreateWithFlags(&plan_ev_[k], hipEventDisableTiming));
     for (auto &e : int_ev_)
         FR_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    const long max_tiles = queue_cap / FR_TILE + max_slots + 1;
+    const long max_tiles = z.max_tiles;
     FR_CHECK(hipMalloc(&L_.plans, sizeof(FrPlanDev) * max_slots));
     FR_CHECK(hipMalloc(&L_.ctl, sizeof(FrCtl) * max_slots));
     act_max_tiles_ = max_tiles;
-    act_tiles_off_ = sizeof(int) * (4 + (size_t)(max_slots + 1) / 2 * 2);
-    act_bytes_ = act_tiles_off_ + sizeof(int2) * (size_t)max_tiles;
+    act_tiles_off_ = z.act_tiles_off;
+    act_bytes_ = z.act_bytes;
     FR_CHECK(hipMalloc(&d_act_, act_bytes_));
     FR_CHECK(hipHostMalloc((void **)&h_act_stage_, act_bytes_ * FR_ACT_RING, hipHostMallocDefault));
     for (auto &e : act_ev_)
@@ -314,9 +376,9 @@ void FrEngine::allocate()
     FR_CHECK(hipMalloc(&L_.exp_q, sizeof(FrExp) * (queue_cap + 1)));
     FR_CHECK(hipMalloc(&L_.fresh_q, sizeof(FrFresh) * (2 * queue_cap + max_slots + 1)));
     FR_CHECK(hipMalloc(&d_scratch_, sizeof(int) * 64));
-    L_.pair_cap = std::min<long>(std::max<long>(16 * queue_cap, 1L << 16), 1L << 26);
+    L_.pair_cap = z.pair_cap;
     FR_CHECK(hipMalloc(&L_.pairs, sizeof(FrPair) * L_.pair_cap));
-    L_.demand_cap = queue_cap + 1;
+    L_.demand_cap = z.demand_cap;
     FR_CHECK(hipMalloc(&L_.demand, sizeof(int2) * L_.demand_cap));
     FR_CHECK(hipMalloc(&L_.demand2, sizeof(int2) * L_.demand_cap));
     FR_CHECK(hipMalloc(&L_.demand2_geo, sizeof(FrGcGeo) * L_.demand_cap));

@@ -189,6 +189,8 @@ class FrSlab {
     void allocate(int W, int H, int mx, int mz, int path_cap, bool with_intrinsic, hipStream_t st);
     void release();
     void set_footprint(int mx, int mz, hipStream_t st);
+    /* device bytes allocate() takes for these arguments */
+    static size_t bytes_for(int W, int H, int mx, int mz, int path_cap, bool with_intrinsic);
     /* back to the state allocate() leaves: planes zero, claims all-ones */
     void reset(hipStream_t st);
     size_t bytes() const { return bytes_; }
@@ -248,6 +250,8 @@ class FrEngine {
      * not depend on these; a caller re-captures its graphs after a change. */
     void set_grids(int waves, int tiles, int items);
     int max_slots() const { return max_slots_; }
+    /* device bytes an engine of these capacities allocates */
+    static size_t bytes_for(int max_slots, long queue_cap);
 
     /* describe plan `slot` (slab + params + frame) and upload its descriptor */
     void bind(int slot, FrSlab &slab, const FrPlanParams &p, const FrFrameView &frame, bool upload = true);

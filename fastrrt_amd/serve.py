@@ -134,7 +134,8 @@ def run_served(duration_s=10.0, rate=2000.0, sizes=SERVED_SIZES, deadlines=DEADL
             "late_rate": float(np.mean(lat > np.array([reqs[t][4] for t in results]))),
             "by_size": {str(k): {"requests": v[0], "success": v[1] / v[0], "deadline_miss": v[2] / v[0]}
                         for k, v in sorted(by_size.items())},
-            "sizes": list(sizes), "deadlines_ms": list(deadlines), "max_plans": max_plans, "max_cells": max_cells}
+            "sizes": list(sizes), "deadlines_ms": list(deadlines), "max_plans": max_plans, "max_cells": max_cells,
+            "hbm": b.memory()}
 
 
 def run_batched(n_plans=512, sizes=BATCHED_SIZES, seed=12, max_iter=200, opt_rounds=20, deadline_ms=0.0,

@@ -312,6 +312,31 @@ double fastrrt_batch_clock_ms(void *batch);
 bool fastrrt_batch_set_launch(void *batch, int grid_waves, int grid_tiles, int grid_items);
 bool fastrrt_batch_get_launch(void *batch, int *grid_waves_tiles_items);
 
+/* Device memory of a batch, measured with hipMemGetInfo.  At creation the
+ * cell budget is max_cells or what the free memory holds beside the reserve
+ * (the engine's queues first, then cells at the bytes a cell of a plan's
+ * planes and of the queues takes), whichever is smaller; a request larger
+ * than the budget is refused at submission.  Before a new plan's planes are
+ * allocated (a plan of a size class with no pooled planes) the free memory
+ * is measured again: when it cannot hold them beside the reserve, pooled
+ * planes of other sizes are freed, and if it still cannot, the plan waits
+ * in the queue (memory_stalls counts such holds) until a running plan gives
+ * its planes back; with no plan running, fastrrt_batch_step fails with an
+ * error instead.  No counterpart in the reference (one plan per process). */
+typedef struct fastrrt_batch_hbm {
+    uint64_t total_bytes, free_bytes_at_create, free_bytes_last, reserve_bytes;
+    uint64_t engine_bytes;    /* queues and tables of the iteration */
+    uint64_t slab_bytes;      /* plans' planes, in use and pooled */
+    int64_t requested_cells, cell_budget, cells_in_use;
+    int64_t memory_stalls;
+    int32_t running, queued;
+} fastrrt_batch_hbm;
+/* Fills `out`; free_bytes_last is measured by this call. */
+bool fastrrt_batch_memory(void *batch, fastrrt_batch_hbm *out);
+/* Bytes left free for the rest of the process when planes are allocated
+ * (default 2 GiB, at most a 16th of the device). */
+bool fastrrt_batch_set_hbm_reserve(void *batch, uint64_t bytes);
+
 #ifdef __cplusplus
 }
 #endif
