@@ -116,6 +116,7 @@ class PlanRequest(ctypes.Structure):
         ("max_path_size", c_float), ("dist_to_goal_tolerance", c_float), ("heading_error_tolerance_rad", c_float),
         ("max_iterations", c_int), ("optimize_rounds", c_int), ("smart", c_int), ("region_side", c_int),
         ("deadline_ms", c_double), ("seed", c_u64), ("stream", c_u64), ("user_tag", c_i64),
+        ("expansion_budget", c_int),
     ]
 
 
@@ -142,6 +143,8 @@ SIGNATURES.update({
     "fastrrt_batch_clock_ms": (c_double, [c_void_p]),
     "fastrrt_batch_set_launch": (c_bool, [c_void_p, c_int, c_int, c_int]),
     "fastrrt_batch_get_launch": (c_bool, [c_void_p, ctypes.POINTER(c_int)]),
+    "fastrrt_batch_set_block_threads": (c_bool, [c_void_p, c_int]),
+    "fastrrt_batch_get_block_threads": (c_int, [c_void_p]),
     "fastrrt_batch_memory": (c_bool, [c_void_p, ctypes.c_void_p]),
     "fastrrt_batch_set_hbm_reserve": (c_bool, [c_void_p, ctypes.c_uint64]),
 })

@@ -1,11 +1,13 @@
 """Launch-shape autotuner for the batched iteration graph.
 
-The iteration graph's kernels take three launch shapes, in blocks of 256
-threads (`Batch.set_launch`): `grid_waves` for the one-wave-per-item kernels
+The iteration graph's kernels take three launch shapes, in blocks
+(`Batch.set_launch`): `grid_waves` for the one-wave-per-item kernels
 (expansion rollouts, goal-plane curves, goal connection; a multiple of 8,
 one share per XCD), `grid_tiles` for the 1024-cell tile scans and
-`grid_items` for the one-lane-per-item kernels.  Every kernel strides over
-its work, so a shape changes the time of an iteration, never its result.
+`grid_items` for the one-lane-per-item kernels, and the threads per block
+of all of them (64, 128 or 256: a smaller block puts fewer waves behind one
+workgroup slot and LDS allocation).  Every kernel strides over its work, so a
+shape changes the time of an iteration, never its result.
 
 For each grid side the tuner builds a batch of seeded scenes (the 256 px
 entry uses the 5/15/30 % rectangle sweep of the default benchmark, the others
@@ -15,14 +17,21 @@ times the iteration graph with HIP events (`fastrrt_profile_*`).  The search
 is coordinate descent: grid_waves, then grid_tiles, then grid_items, each
 over its candidate list with the others fixed at the best so far (a shape
 replaces the best only when it is at least 1 % faster); every candidate runs
-`repeats` times and keeps its median.  Every pass's per-plan
-results (status, iterations, expansions, path) must equal the first pass's;
-a difference raises, since it would mean a kernel depends on its launch shape.
+`repeats` times and keeps its median.  The search is coordinate descent:
+threads per block, grid_waves, grid_tiles, then grid_items.  Every pass's
+per-plan results (status, iterations, expansions, path) must equal the first
+pass's; a difference raises, since it would mean a kernel depends on its
+launch shape.
 
-The per-iteration expansion budget and the LDS tile sizes are not tuned: the
-reference expands every eligible node each iteration, so a budget would
-change the search, and the LDS buffers (rollout cell buffer, in-LDS erase
-capacity) are sized by the kernels' own structure.
+The per-iteration expansion budget (`PlanSpec(expansion_budget=...)`) is not
+a launch shape: it changes the search (the reference expands every eligible
+node each iteration), so it is off by default and `measure_budget` only
+reports its effect on success within a deadline, on the same plans as the
+default search.  The LDS buffers (the rollout cell buffer and the footprint
+group sets, 128 and 64 entries per wave; the erase's in-LDS ancestor
+capacity) are compile-time sizes fixed by the kernels' structure: a wave
+flushes its cell buffer 64 cells at a time, so 128 entries hold a flush and
+the next chunk of the walk.
 
     python -m fastrrt_amd.autotune --sizes 128 256 384 512 --out table.json
 
@@ -45,14 +54,18 @@ from .pyfastrrt import SearchFrame
 WAVES = (1024, 2048, 4096, 8192)
 TILES = (512, 1024, 2048, 4096)
 ITEMS = (256, 1024, 2048)
+BLOCKS = (64, 128, 256)
 # plans per batch by grid side: about 11 M cells, as the 168-plan 256 px batch
 PLANS = {128: 512, 256: 168, 384: 72, 512: 40}
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "launch_mi355x.json")
 
 
 def valid_shape(shape):
-    w, t, i = (int(v) for v in shape)
-    return 8 <= w <= 65535 and w % 8 == 0 and 1 <= t <= 65535 and 1 <= i <= 65535
+    if len(shape) not in (3, 4):
+        return False
+    w, t, i = (int(v) for v in shape[:3])
+    b = int(shape[3]) if len(shape) == 4 else 256
+    return (8 <= w <= 65535 and w % 8 == 0 and 1 <= t <= 65535 and 1 <= i <= 65535 and b in (64, 128, 192, 256))
 
 
 def nearest(entries, side):
@@ -66,8 +79,8 @@ def nearest(entries, side):
 
 
 def lookup(side, path=TABLE):
-    """Tuned (grid_waves, grid_tiles, grid_items) for plans of `side` px, or
-    None when no table is present."""
+    """Tuned (grid_waves, grid_tiles, grid_items[, threads per block]) for
+    plans of `side` px, or None when no table is present."""
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -134,10 +147,10 @@ def tune_side(side, n_plans=None, iters=24, repeats=3, seed=4242, log=print):
         log(f"[autotune] side {side} plans {len(plans)} shape {shape}: {times[shape]:.4f} ms per iteration")
         return times[shape]
 
-    default = batch.launch()
+    default = tuple(batch.launch()) + (batch.block(),)
     best = default
     measure(best)
-    for axis, cands in ((0, WAVES), (1, TILES), (2, ITEMS)):
+    for axis, cands in ((3, BLOCKS), (0, WAVES), (1, TILES), (2, ITEMS)):
         for c in cands:
             shape = list(best)
             shape[axis] = c
@@ -149,20 +162,66 @@ def tune_side(side, n_plans=None, iters=24, repeats=3, seed=4242, log=print):
             "measured": [{"shape": list(k), "ms": round(v, 5)} for k, v in sorted(times.items())]}
 
 
+def measure_budget(budgets, n_plans=120, steps=10, deadline_ms=20.0, seed=4343, log=print):
+    """The expansion budget's effect at the default benchmark's workload
+    (256 px, 5/15/30 % rectangles, 22 x 40 footprint, `n_plans` plans per
+    batch, each with a `deadline_ms` deadline): the same `steps` batches of
+    plans run once per budget (0 first, the reference's search), and for each
+    the success rate, the iterations and rollouts per plan, the latency and
+    the plans solved only with or only without the budget."""
+    pool = scenes.scenes_256_sweep(max(1, (n_plans * steps + 2) // 3), base_seed=seed)[:n_plans * steps]
+    frames = [SearchFrame(sc.frame, sc.class_costs) for sc in pool]
+    batch = Batch(max_plans=n_plans, max_cells=n_plans * 256 * 256, launch="auto")
+    out, base = [], None
+    for budget in [0] + [int(b) for b in budgets if int(b) > 0]:
+        res = {}
+        t0 = time.perf_counter()
+        for s in range(steps):
+            for k in range(s * n_plans, (s + 1) * n_plans):
+                sc = pool[k]
+                batch.submit(PlanSpec(frames[k], sc.start, sc.goal, min_dist=sc.min_dist, ego_lb=sc.ego_lb,
+                                      ego_ub=sc.ego_ub, max_iterations=200, optimize_rounds=20,
+                                      deadline_ms=deadline_ms, seed=1234, stream=700000 + k, tag=k,
+                                      expansion_budget=budget))
+            for r in batch.drain():
+                res[r.tag] = r
+        wall = time.perf_counter() - t0
+        ok = {t: r.ok for t, r in res.items()}
+        if base is None:
+            base = ok
+        lat = np.array([r.latency_ms for r in res.values()])
+        e = {"budget": budget, "plans": len(res), "success": round(float(np.mean(list(ok.values()))), 4),
+             "only_with_budget": int(sum(1 for t in ok if ok[t] and not base[t])),
+             "only_without_budget": int(sum(1 for t in ok if base[t] and not ok[t])),
+             "mean_iterations": round(float(np.mean([r.iterations for r in res.values()])), 1),
+             "rollouts_per_iteration": round(float(sum(r.expansions for r in res.values()) /
+                                                   max(1, sum(r.iterations for r in res.values()))), 2),
+             "p99_ms": round(float(np.percentile(lat, 99)), 3), "plans_per_s": round(len(res) / wall, 1)}
+        log(f"[autotune] expansion budget {e}")
+        out.append(e)
+    return {"side": 256, "plans_per_batch": n_plans, "deadline_ms": deadline_ms, "default": 0, "measured": out}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--sizes", type=int, nargs="+", default=[128, 256, 384, 512])
     ap.add_argument("--iters", type=int, default=24)
     ap.add_argument("--repeats", type=int, default=3)
     ap.add_argument("--out", default=TABLE)
+    ap.add_argument("--budgets", type=int, nargs="*", default=[8, 16, 32],
+                    help="expansion budgets whose effect measure_budget reports (none: skip)")
     a = ap.parse_args()
     t0 = time.time()
     entries = []
     for side in a.sizes:
         entries.append(tune_side(side, iters=a.iters, repeats=a.repeats, log=lambda m: print(m, flush=True)))
     tab = {"device": "MI355X (gfx950)", "unit": "ms per iteration graph launch (HIP events)",
-           "candidates": {"grid_waves": list(WAVES), "grid_tiles": list(TILES), "grid_items": list(ITEMS)},
+           "shape": ["grid_waves", "grid_tiles", "grid_items", "threads_per_block"],
+           "candidates": {"grid_waves": list(WAVES), "grid_tiles": list(TILES), "grid_items": list(ITEMS),
+                          "threads_per_block": list(BLOCKS)},
            "entries": entries, "seconds": round(time.time() - t0, 1)}
+    if a.budgets:
+        tab["expansion_budget"] = measure_budget(a.budgets, log=lambda m: print(m, flush=True))
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     with open(a.out, "w") as fh:
         json.dump(tab, fh, indent=1)

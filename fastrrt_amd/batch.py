@@ -16,13 +16,16 @@ from .pyfastrrt import SearchFrame
 
 class PlanSpec:
     """One plan request (the parameters set_plan_data takes, plus batch-only
-    fields: iteration cap, optimisation rounds, deadline, RNG key)."""
+    fields: iteration cap, optimisation rounds, deadline, RNG key, and the
+    optional expansion budget: 0 expands every eligible node each iteration,
+    as the reference; > 0 keeps each eligible node with probability budget /
+    (eligible nodes of the previous iteration), a different search)."""
 
     def __init__(self, frame: SearchFrame, start, goal, min_dist=(22, 40), ego_lb=(-1, -1), ego_ub=(-1, -1),
                  velocity=1.0, max_steering_deg=40.0, vehicle_length=5.412658773, max_curvature=-1.0,
                  max_path_size=40.0, dist_to_goal_tolerance=20.0, heading_error_tolerance=math.radians(10),
                  max_iterations=200, optimize_rounds=20, smart=False, deadline_ms=0.0, seed=0, stream=0, tag=0,
-                 region_side=0):
+                 region_side=0, expansion_budget=0):
         self.frame = frame
         r = _lib.PlanRequest()
         r.frame = frame.get_cuda_ptr()
@@ -46,6 +49,7 @@ class PlanSpec:
         r.seed = int(seed) & ((1 << 64) - 1)
         r.stream = int(stream) & ((1 << 64) - 1)
         r.user_tag = int(tag)
+        r.expansion_budget = int(expansion_budget)
         self.req = r
 
 
@@ -64,8 +68,8 @@ class Result:
 class Batch:
     """launch: None keeps the engine's launch shapes; "auto" applies the
     tuned shapes (fastrrt_amd.autotune) of the grid side max_cells/max_plans
-    corresponds to; a (grid_waves, grid_tiles, grid_items) tuple sets them.
-    Results do not depend on the choice."""
+    corresponds to; a (grid_waves, grid_tiles, grid_items[, threads per
+    block]) tuple sets them.  Results do not depend on the choice."""
 
     def __init__(self, max_plans=256, max_cells=256 * 256 * 256, path_cap=65536, launch=None):
         L = _lib.load()
@@ -126,11 +130,19 @@ class Batch:
     def clock_ms(self) -> float:
         return _lib.load().fastrrt_batch_clock_ms(self._ptr)
 
-    def set_launch(self, grid_waves, grid_tiles, grid_items):
-        """Launch shapes of the iteration graph, in blocks of 256 threads (the
-        results do not depend on them; fastrrt_amd.autotune measures them)."""
-        if not _lib.load().fastrrt_batch_set_launch(self._ptr, int(grid_waves), int(grid_tiles), int(grid_items)):
+    def set_launch(self, grid_waves, grid_tiles, grid_items, block=None):
+        """Launch shapes of the iteration graph, in blocks, and optionally the
+        threads per block (64, 128, 192 or 256; the results do not depend on
+        them; fastrrt_amd.autotune measures them)."""
+        L = _lib.load()
+        if block is not None and not L.fastrrt_batch_set_block_threads(self._ptr, int(block)):
+            _lib.check("batch_set_block_threads")
+        if not L.fastrrt_batch_set_launch(self._ptr, int(grid_waves), int(grid_tiles), int(grid_items)):
             _lib.check("batch_set_launch")
+
+    def block(self):
+        """threads per block of the iteration kernels"""
+        return int(_lib.load().fastrrt_batch_get_block_threads(self._ptr))
 
     def memory(self):
         """The batch's device memory as measured (fastrrt_batch_memory): free

@@ -228,6 +228,15 @@ class FrBatch {
 
     void set_hbm_reserve(size_t bytes) { hbm_.reserve = bytes; }
 
+    void set_block(int threads)
+    {
+        FR_CHECK(hipStreamSynchronize(eng_.stream()));
+        eng_.set_block(threads);
+        drop_graphs();
+    }
+
+    int block() const { return eng_.launch().block; }
+
     void memory(fastrrt_batch_hbm *o)
     {
         size_t fr = 0, tot = 0;
@@ -357,6 +366,7 @@ class FrBatch {
         p.seed = r.seed;
         p.stream = r.stream;
         p.region_side = r.region_side > 0 ? r.region_side : 128;
+        p.exp_budget = r.expansion_budget > 0 ? r.expansion_budget : 0;
     }
 
     /* A plan's planes: a pooled slab of its size class, or a new one when
@@ -887,6 +897,16 @@ bool fastrrt_batch_set_hbm_reserve(void *batch, uint64_t bytes)
         return true;
     })
 }
+
+bool fastrrt_batch_set_block_threads(void *batch, int threads)
+{
+    FR_TRY(false, {
+        B(batch)->set_block(threads);
+        return true;
+    })
+}
+
+int fastrrt_batch_get_block_threads(void *batch) { FR_TRY(-1, { return B(batch)->block(); }) }
 
 bool fastrrt_batch_get_launch(void *batch, int *grid_waves_tiles_items)
 {

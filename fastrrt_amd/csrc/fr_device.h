@@ -47,6 +47,7 @@ struct FrPlanDev {
     int region_side, dens_w, dens_h;
     int smart;
     int max_iter;
+    int exp_budget;           /* > 0: eligible nodes thinned to about this many rollouts per iteration */
     uint64_t seed, stream;
     /* node grid */
     uint8_t *type;
@@ -129,6 +130,8 @@ struct FrCtl {
     int total_expansions;     /* rollouts since the search started */
     int chain_only;           /* the grid holds only the rebuilt path chain (after an optimisation round) */
     int opt_active;           /* batch finalisation: optimisation rounds pending */
+    int n_eligible;           /* nodes the expansion gate passed this iteration (before any thinning) */
+    int prev_eligible;        /* the same, previous iteration */
 };
 
 /* One queued expansion (one rollout). */

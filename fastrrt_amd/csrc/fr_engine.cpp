@@ -388,6 +388,7 @@ void FrEngine::allocate()
     L_.grid_items = (int)std::max<long>(1, std::min<long>((queue_cap + FR_BLOCK - 1) / FR_BLOCK, 2048));
     L_.grid_slots = 1;
     L_.grid_waves = 4096; /* blocks of 4 one-item waves; k_expand needs a multiple of 8 blocks */
+    L_.block = FR_BLOCK;
     L_.max_slots = max_slots;
 }
 
@@ -442,6 +443,13 @@ void FrEngine::set_grids(int waves, int tiles, int items)
     L_.grid_items = items;
 }
 
+void FrEngine::set_block(int threads)
+{
+    if (threads < 64 || threads > FR_BLOCK || threads % 64 != 0)
+        throw std::runtime_error("threads per block must be 64, 128, 192 or 256");
+    L_.block = threads;
+}
+
 void FrEngine::bind(int slot, FrSlab &s, const FrPlanParams &p, const FrFrameView &frame, bool upload)
 {
     FrPlanDev d{};
@@ -479,6 +487,7 @@ void FrEngine::bind(int slot, FrSlab &s, const FrPlanParams &p, const FrFrameVie
     d.dens_h = s.H / p.region_side + 1;
     d.smart = p.smart;
     d.max_iter = p.max_iter;
+    d.exp_budget = p.exp_budget;
     d.seed = p.seed;
     d.stream = p.stream;
     d.type = s.type;

@@ -49,6 +49,7 @@ struct FrLaunch {
     long demand_cap;          /* of both */
     int *n_slots;             /* device: entries of `slots` */
     int grid_tiles, grid_items, grid_slots, grid_waves;
+    int block;                /* threads per block of the iteration kernels (k_erase_mark aside) */
     int max_slots;
 };
 
@@ -172,6 +173,7 @@ struct FrPlanParams {
     int region_side = 128;
     int smart = 0;
     int max_iter = 0;
+    int exp_budget = 0;           /* batch: expected rollouts per iteration (0: every eligible node) */
     uint64_t seed = 0, stream = 0;
     int path_cap = 0;             /* 0: W*H + 1 */
     bool with_intrinsic = false;
@@ -249,6 +251,9 @@ class FrEngine {
      * kernels.  Every one of them strides over its work, so the results do
      * not depend on these; a caller re-captures its graphs after a change. */
     void set_grids(int waves, int tiles, int items);
+    /* threads per block of the same kernels: 64, 128, 192 or 256 (every one
+     * of them works in whole waves and sizes its LDS for 256 threads) */
+    void set_block(int threads);
     int max_slots() const { return max_slots_; }
     /* device bytes an engine of these capacities allocates */
     static size_t bytes_for(int max_slots, long queue_cap);

@@ -1088,6 +1088,8 @@ __global__ void __launch_bounds__(FR_BLOCK) k_begin(const FrPlanDev *plans, FrCt
         return;
     C.frontier = C.last_temp >= 100;
     C.force = C.last_temp == 0;
+    C.prev_eligible = C.n_eligible;
+    C.n_eligible = 0;
     C.collision = 0;
     C.n_node_pre = 0;
     C.n_conv = 0;
@@ -1210,6 +1212,27 @@ __global__ void __launch_bounds__(FR_BLOCK) k_select(const FrPlanDev *plans, FrC
             }
             if (ok)
                 ok_m |= 1u << i;
+        }
+        if (P.exp_budget > 0) {
+            /* Optional thinning (no counterpart in the reference, which
+             * expands every eligible node): each eligible node is kept with
+             * probability budget / (eligible nodes of the previous
+             * iteration), drawn from the third word of its expansion's
+             * Philox block, so the plan stays independent of the batch. */
+            const int elig = dev_bcast_i(dev_wave_incl_scan(__popc(ok_m)), 63);
+            if (lane == 0 && elig)
+                atomicAdd(&C.n_eligible, elig);
+            const int prev = C.prev_eligible;
+            if (prev > P.exp_budget) {
+                const uint32_t thr = (uint32_t)((double)P.exp_budget / (double)prev * 4294967296.0);
+                uint32_t keep = 0;
+                for (uint32_t m = ok_m; m; m &= m - 1) {
+                    const int i = __ffs((int)m) - 1;
+                    if (fr_expansion_gate(P.seed, P.stream, (uint32_t)C.iter, (uint32_t)(base + 16 * lane + i)) < thr)
+                        keep |= 1u << i;
+                }
+                ok_m = keep;
+            }
         }
         int total;
         int w = dev_wave_reserve(&counters[FR_CNT_EXP], __popc(ok_m), &total);
@@ -2856,8 +2879,8 @@ void fr_launch_goal_complete(const FrLaunch &L, const int *d_slots, int n, int m
  * that captures graphs can time k_expand with events between them. */
 void fr_launch_iteration_pre(const FrLaunch &L, hipStream_t st)
 {
-    dim3 b(FR_BLOCK);
-    hipLaunchKernelGGL(k_begin, dim3(fr_grid_for(L.max_slots, FR_BLOCK / 64, 65535)), b, 0, st, L.plans, L.ctl, L.slots,
+    dim3 b(L.block);
+    hipLaunchKernelGGL(k_begin, dim3(fr_grid_for(L.max_slots, L.block / 64, 65535)), b, 0, st, L.plans, L.ctl, L.slots,
                        L.n_slots, L.counters);
     hipLaunchKernelGGL(k_select, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles, L.exp_q,
                        L.counters);
@@ -2865,12 +2888,12 @@ void fr_launch_iteration_pre(const FrLaunch &L, hipStream_t st)
 
 void fr_launch_iteration_expand(const FrLaunch &L, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_expand, dim3(L.grid_waves), dim3(FR_BLOCK), 0, st, L.plans, L.ctl, L.exp_q, L.counters);
+    hipLaunchKernelGGL(k_expand, dim3(L.grid_waves), dim3(L.block), 0, st, L.plans, L.ctl, L.exp_q, L.counters);
 }
 
 void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st)
 {
-    dim3 b(FR_BLOCK);
+    dim3 b(L.block);
     hipLaunchKernelGGL(k_commit, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.exp_q, L.counters);
     const int dens_blocks = (L.grid_tiles + FR_ERASE_BLOCK / FR_BLOCK - 1) / (FR_ERASE_BLOCK / FR_BLOCK);
     hipLaunchKernelGGL(k_erase_mark, dim3(L.max_slots + dens_blocks), dim3(FR_ERASE_BLOCK), 0, st, L.plans, L.ctl,
@@ -2893,7 +2916,7 @@ void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st)
                        L.pair_cap, L.counters);
     fr_prof_end(FR_PROF_CONNECT, st);
     hipLaunchKernelGGL(k_connect_pick, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);
-    hipLaunchKernelGGL(k_finish, dim3(fr_grid_for(L.max_slots, FR_BLOCK / 64, 65535)), b, 0, st, L.plans, L.ctl,
+    hipLaunchKernelGGL(k_finish, dim3(fr_grid_for(L.max_slots, L.block / 64, 65535)), b, 0, st, L.plans, L.ctl,
                        L.slots, L.n_slots, L.fresh_q, L.pairs);
 }
 

@@ -298,4 +298,16 @@ FR_HD void fr_expansion_draws(uint64_t seed, uint64_t stream, uint32_t iter, uin
     *u_len = fr_u01(r.v[1]);
 }
 
+/* Third word of the same Philox block: the draw of the optional expansion
+ * thinning (unused when every eligible node expands). */
+FR_HD uint32_t fr_expansion_gate(uint64_t seed, uint64_t stream, uint32_t iter, uint32_t cell)
+{
+    fr_u32x4 c;
+    c.v[0] = cell;
+    c.v[1] = iter;
+    c.v[2] = (uint32_t)stream;
+    c.v[3] = (uint32_t)(stream >> 32);
+    return fr_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32)).v[2];
+}
+
 #endif /* FR_MATH_H */

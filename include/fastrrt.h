@@ -268,6 +268,9 @@ typedef struct fastrrt_plan_request {
     double deadline_ms;       /* from submission; the plan fails when it passes (<= 0: none) */
     uint64_t seed, stream;    /* Philox key and stream */
     int64_t user_tag;
+    int expansion_budget;     /* 0: every eligible node expands each iteration, as the reference;
+                                 > 0: eligible nodes kept with probability budget / (eligible nodes of
+                                 the previous iteration), a search the reference does not run */
 } fastrrt_plan_request;
 
 typedef struct fastrrt_plan_result {
@@ -311,6 +314,11 @@ double fastrrt_batch_clock_ms(void *batch);
  * get_launch writes the three values in that order. */
 bool fastrrt_batch_set_launch(void *batch, int grid_waves, int grid_tiles, int grid_items);
 bool fastrrt_batch_get_launch(void *batch, int *grid_waves_tiles_items);
+/* Threads per block of the same kernels (the erase's marking kernel aside):
+ * 64, 128, 192 or 256 (the default).  The grids stay counted in blocks.
+ * Results do not depend on it; get returns -1 on a null handle. */
+bool fastrrt_batch_set_block_threads(void *batch, int threads);
+int fastrrt_batch_get_block_threads(void *batch);
 
 /* Device memory of a batch, measured with hipMemGetInfo.  At creation the
  * cell budget is max_cells or what the free memory holds beside the reserve

@@ -14,6 +14,11 @@ def test_valid_shape():
     assert not autotune.valid_shape((0, 1, 1))
     assert not autotune.valid_shape((8, 0, 1))
     assert not autotune.valid_shape((8, 1, 65536))
+    assert autotune.valid_shape((4096, 2048, 2048, 64))
+    assert autotune.valid_shape((4096, 2048, 2048, 192))
+    assert not autotune.valid_shape((4096, 2048, 2048, 96))
+    assert not autotune.valid_shape((4096, 2048, 2048, 512))
+    assert not autotune.valid_shape((4096, 2048))
 
 
 def test_lookup_takes_the_nearest_side_on_a_log_scale(tmp_path):

@@ -161,18 +161,20 @@ def test_launch_shapes_leave_results_unchanged():
     frames = [P.SearchFrame(sc.frame, sc.class_costs) for sc in scs]
     b = Batch(max_plans=len(scs), max_cells=sum(s.width * s.height for s in scs))
     default = b.launch()
-    assert default[0] == 4096 and 1 <= default[1] <= 2048 and 1 <= default[2] <= 2048
+    assert default[0] == 4096 and 1 <= default[1] <= 2048 and 1 <= default[2] <= 2048 and b.block() == 256
     runs = []
-    for shape in [default, (8, 1, 1), (1024, 512, 256), (8192, 4096, 1024)]:
+    for shape in [default + (256,), (8, 1, 1, 256), (1024, 512, 256, 64), (8192, 4096, 1024, 128),
+                  (2048, 1024, 512, 192), (8, 1, 1, 64)]:
         print(f"[batch] launch shape {shape}", flush=True)
         b.set_launch(*shape)
-        assert b.launch() == shape
+        assert b.launch() + (b.block(),) == shape
         for i, sc in enumerate(scs):
             b.submit(_spec(sc, frames[i], i))
         runs.append({r.tag: _key(r) for r in b.drain()})
     assert all(r == runs[0] for r in runs[1:])
     assert sum(1 for k in runs[0].values() if k[0] == 2) >= 2
-    for bad in [(12, 2048, 2048), (0, 2048, 2048), (4096, 0, 2048), (4096, 2048, 70000)]:
+    for bad in [(12, 2048, 2048), (0, 2048, 2048), (4096, 0, 2048), (4096, 2048, 70000), (4096, 2048, 2048, 96),
+                (4096, 2048, 2048, 512)]:
         with pytest.raises(RuntimeError):
             b.set_launch(*bad)
-    assert b.launch() == (8192, 4096, 1024)
+    assert b.launch() == (8, 1, 1) and b.block() == 64
