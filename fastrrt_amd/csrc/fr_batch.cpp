@@ -649,11 +649,24 @@ class FrBatch {
         if (!fin_exec_ || rounds > fin_rounds_)
             capture_finalize(std::max(rounds, fin_rounds_));
         const double t = now_ms() - t0_;
+        /* A plan that reaches its goal while this graph runs waits for it:
+         * the graph's rounds end in time for the earliest deadline of a plan
+         * still searching or waiting (one round and the path extraction
+         * after them; a plan past that point finalises with no rounds). */
+        int cap = 1 << 30;
+        for (int i = 0; i < (int)slots_.size(); ++i) {
+            const Slot &sl = slots_[i];
+            if (!sl.busy || sl.finalizing || sl.p.req.deadline_ms <= 0 ||
+                std::find(fin_slots_.begin(), fin_slots_.end(), i) != fin_slots_.end())
+                continue;
+            const double left = sl.p.req.deadline_ms - (t - sl.p.t_submit) - 2.0 * poll_ema_ - 0.3;
+            cap = std::min(cap, std::max(0, (int)std::floor(left / std::max(fin_round_ms_, 1e-3)) - 2));
+        }
         fin_used_rounds_ = 0;
         for (int q = 0; q < fin_max_; ++q) {
             const bool real = q < (int)fin_slots_.size();
             h_fin_list_[q] = real ? fin_slots_[q] : dummy_;
-            h_fin_list_[fin_max_ + q] = real ? round_limit(fin_slots_[q], t) : 0;
+            h_fin_list_[fin_max_ + q] = real ? std::min(cap, round_limit(fin_slots_[q], t)) : 0;
             if (real)
                 fin_used_rounds_ = std::max(fin_used_rounds_, h_fin_list_[fin_max_ + q]);
         }

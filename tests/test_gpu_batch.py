@@ -99,6 +99,31 @@ def test_deadline_retires_plans():
         assert r.latency_ms < 30.0 + 200.0  # retired at the first poll past the deadline
 
 
+def test_deadline_latency_at_the_benchmark_workload():
+    """20 ms deadlines, no iteration cap, 20 optimisation rounds, 256 px
+    scenes with the 22 x 40 footprint, one iteration per poll (the
+    benchmark's protocol): every plan, found or failed, returns within its
+    deadline plus 2 ms, several steps' worth of margin less than the loose
+    bound above."""
+    pool = scenes.scenes_256_sweep(8, base_seed=6060)
+    frames = [P.SearchFrame(sc.frame, sc.class_costs) for sc in pool]
+    b = Batch(max_plans=len(pool), max_cells=len(pool) * 65536)
+    for k, sc in enumerate(pool):
+        b.submit(PlanSpec(frames[k], sc.start, sc.goal, min_dist=sc.min_dist, ego_lb=sc.ego_lb, ego_ub=sc.ego_ub,
+                          max_iterations=0, optimize_rounds=20, deadline_ms=20.0, seed=1234, stream=8800 + k, tag=k))
+    res = []
+    while b.step(1) > 0:
+        res += b.poll()
+    res += b.poll()
+    assert sorted(r.tag for r in res) == list(range(len(pool)))
+    found = [r.latency_ms for r in res if r.ok]
+    failed = [r.latency_ms for r in res if not r.ok]
+    print(f"[batch] deadline 20 ms: {len(found)} found, max {max(found, default=0):.2f} ms; {len(failed)} failed, "
+          f"max {max(failed, default=0):.2f} ms", flush=True)
+    assert failed and all(r.status in (2, 3, 4) for r in res)
+    assert max(r.latency_ms for r in res) <= 20.0 + 2.0
+
+
 def test_batch_smart_regions_equals_oracle():
     """Smart expansion with 32 px density regions in the batch engine (the
     request's region side) gives the oracle's search and path."""
