@@ -22,6 +22,20 @@
 #define FR_BLOCK 256          /* threads per block of the tile kernels */
 #define FR_DIST_CAP 63        /* largest cap of the Chebyshev obstacle-distance plane (min(pad, 63) is kept) */
 #define FR_BIG_COST 99999999999LL
+#define FR_GROUP 7            /* footprint samples are certified in FR_GROUP x FR_GROUP groups */
+#define FR_GROUP_FREE 6       /* a group's middle sample at this distance certifies the group */
+#define FR_MAX_GROUPS 32      /* footprints of more groups are sampled row by row */
+
+/* The planes live in device global memory.  In the device pass their
+ * pointers carry the global address space, so every access compiles to a
+ * global_* instruction: a flat access also counts against the LDS/scalar
+ * wait counter, so each LDS or scalar wait would wait for it as well.  The
+ * layout is the same in both passes (64-bit pointers). */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FR_GLOBAL __attribute__((address_space(1)))
+#else
+#define FR_GLOBAL
+#endif
 
 enum FrStatus {
     FR_IDLE = 0,
@@ -34,7 +48,7 @@ enum FrStatus {
 struct FrPlanDev {
     int W, H, N;
     int pad, PW, PH;          /* padded distance plane (W + 2 pad) x (H + 2 pad) */
-    int TW;                   /* 8 x 8 tiles per tile row of the padded plane */
+    int TW;                   /* row pitch (bytes) of the padded distance plane */
     int mx, mz;               /* half footprint (min_dist / 2, truncated) */
     int lbx, lbz, ubx, ubz;   /* ego box, pixels inside are ignored */
     float og_x, og_y;         /* map-frame origin in grid pixels */
@@ -48,45 +62,47 @@ struct FrPlanDev {
     int smart;
     int max_iter;
     int exp_budget;           /* > 0: eligible nodes thinned to about this many rollouts per iteration */
+    int n_groups;             /* 7 x 7 sample groups of the footprint (0: more than FR_MAX_GROUPS) */
+    float probe_j[FR_MAX_GROUPS], probe_i[FR_MAX_GROUPS]; /* each group's middle sample (column, row) */
     uint64_t seed, stream;
     /* node grid */
-    uint8_t *type;
-    int32_t *derive, *px, *pz;
-    float *heading, *cost;
-    float *intrinsic;         /* may be null: all-zero intrinsic costs */
+    FR_GLOBAL uint8_t *type;
+    FR_GLOBAL int32_t *derive, *px, *pz;
+    FR_GLOBAL float *heading, *cost;
+    FR_GLOBAL float *intrinsic;         /* may be null: all-zero intrinsic costs */
     /* goal connection */
-    uint8_t *gc_flag;
-    float *gc_heading, *gc_cost;
-    uint32_t *gc_done;        /* bit per cell: goal-connection planes evaluated */
-    uint32_t *gc_bits;        /* bit per cell: gc_flag, for window scans by word */
+    FR_GLOBAL uint8_t *gc_flag;
+    FR_GLOBAL float *gc_heading, *gc_cost;
+    FR_GLOBAL uint32_t *gc_done;        /* bit per cell: goal-connection planes evaluated */
+    FR_GLOBAL uint32_t *gc_bits;        /* bit per cell: gc_flag, for window scans by word */
     int gc_lazy;              /* evaluate a cell when a node's window first covers it */
     /* scratch planes */
-    unsigned long long *claim;
-    int32_t *escr;            /* 4 * N ints: the erase's node list, cell -> list index, two jump buffers */
-    uint8_t *mark;
-    uint8_t *dist;            /* padded Chebyshev distance to the nearest obstacle pixel, in 8 x 8
-                                 tiles of 64 bytes (fr_dist_at) */
-    uint8_t *dist_tmp;
-    uint8_t *trav;            /* bit0: feasible for all headings; bit1: safe-zone bit 0x100 */
-    uint32_t *region;
+    FR_GLOBAL unsigned long long *claim;
+    FR_GLOBAL int32_t *escr;            /* 4 * N ints: the erase's node list, cell -> list index, two jump buffers */
+    FR_GLOBAL uint8_t *mark;
+    FR_GLOBAL uint8_t *dist;            /* padded Chebyshev distance to the nearest obstacle pixel, row-major
+                                 (fr_dist_at) */
+    FR_GLOBAL uint8_t *dist_tmp;
+    FR_GLOBAL uint8_t *trav;            /* bit0: feasible for all headings; bit1: safe-zone bit 0x100 */
+    FR_GLOBAL uint32_t *region;
     /* input frame (float3 per pixel) and class-cost table */
-    const float *frame;
-    const float *class_costs;
+    const FR_GLOBAL float *frame;
+    const FR_GLOBAL float *class_costs;
     int n_classes;
     /* path buffers */
-    float4 *path;
-    float4 *pd;
+    FR_GLOBAL float4 *path;
+    FR_GLOBAL float4 *pd;
     int path_cap;
     int opt_limit;            /* path_optimize rounds of batch finalisation */
     /* planes a new plan starts from zero (batch admission resets them) */
-    uint4 *zero_base;
+    FR_GLOBAL uint4 *zero_base;
     long zero_words;
 };
 
-/* Byte offset of padded cell (xp, zp) in the tiled distance plane: tiles of
- * 8 x 8 cells (64 bytes) in row-major tile order, cells row-major inside a
- * tile, so a footprint's samples and a curve's consecutive points fall in a
- * few cache lines whatever their direction. */
+/* Byte offset of padded cell (xp, zp) in the distance plane: row-major with
+ * a pitch of TW bytes, one multiply-add per lookup (an 8 x 8-tiled layout
+ * measured 1 % slower per batched iteration: its address takes seven
+ * instructions, and the footprint's bytes hit L2 either way). */
 #if defined(__HIPCC__)
 __host__ __device__ __forceinline__
 #else
@@ -94,7 +110,11 @@ inline
 #endif
 unsigned fr_dist_at(int TW, int xp, int zp)
 {
-    return ((unsigned)((zp >> 3) * TW + (xp >> 3)) << 6) | ((unsigned)(zp & 7) << 3) | (unsigned)(xp & 7);
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul24((unsigned)zp, (unsigned)TW) + (unsigned)xp; /* full-rate 24-bit product */
+#else
+    return (unsigned)(zp * TW + xp);
+#endif
 }
 
 /* Mutable per-plan control block. */

@@ -3,6 +3,7 @@
  */
 #include "fr_engine.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <deque>
@@ -155,18 +156,18 @@ void FrSlab::release()
 /* The padded distance planes depend on the footprint; they are reallocated
  * alone so that the node grid survives a footprint change, as the reference's
  * grid survives set_plan_data. */
-/* Distance planes of a footprint: the 8 x 8-tiled plane and its untiled
- * scratch, over the grid padded by the footprint's reach. */
-static void fr_dist_geometry(int W, int H, int mx, int mz, int *pad, int *PW, int *PH, int *TW, size_t *tiled,
+/* Distance planes of a footprint: the plane and its scratch (the row pass),
+ * over the grid padded by the footprint's reach. */
+static void fr_dist_geometry(int W, int H, int mx, int mz, int *pad, int *PW, int *PH, int *TW, size_t *dbytes,
                              size_t *pwh)
 {
     const double hyp = std::sqrt((double)mx * mx + (double)mz * mz);
     *pad = (int)std::ceil(hyp) + 2;
     *PW = W + 2 * *pad;
     *PH = H + 2 * *pad;
-    *TW = (*PW + 7) / 8;
     *pwh = (size_t)*PW * *PH;
-    *tiled = (size_t)*TW * ((*PH + 7) / 8) * 64; /* whole 8 x 8 tiles */
+    *TW = *PW; /* row pitch */
+    *dbytes = *pwh;
 }
 
 /* The padded distance planes depend on the footprint; they are reallocated
@@ -181,12 +182,12 @@ void FrSlab::set_footprint(int mx_, int mz_, hipStream_t st)
     dist_base_ = nullptr;
     mx = mx_;
     mz = mz_;
-    size_t tiled = 0, PWH = 0;
-    fr_dist_geometry(W, H, mx, mz, &pad, &PW, &PH, &TW, &tiled, &PWH);
-    FR_CHECK(hipMalloc(&dist_base_, fr_align(tiled) + fr_align(PWH)));
+    size_t dbytes = 0, PWH = 0;
+    fr_dist_geometry(W, H, mx, mz, &pad, &PW, &PH, &TW, &dbytes, &PWH);
+    FR_CHECK(hipMalloc(&dist_base_, fr_align(dbytes) + fr_align(PWH)));
     dist = (uint8_t *)dist_base_;
-    dist_tmp = (uint8_t *)dist_base_ + fr_align(tiled);
-    FR_CHECK(hipMemsetAsync(dist_base_, 0, fr_align(tiled) + fr_align(PWH), st));
+    dist_tmp = (uint8_t *)dist_base_ + fr_align(dbytes);
+    FR_CHECK(hipMemsetAsync(dist_base_, 0, fr_align(dbytes) + fr_align(PWH), st));
 }
 
 /* Offsets of the planes in the slab's one allocation (a plane per field,
@@ -236,9 +237,9 @@ size_t FrSlab::bytes_for(int W, int H, int mx, int mz, int path_cap, bool with_i
 {
     const int cap = path_cap > 0 ? path_cap : W * H + 1;
     int pad, PW, PH, TW;
-    size_t tiled = 0, PWH = 0;
-    fr_dist_geometry(W, H, mx, mz, &pad, &PW, &PH, &TW, &tiled, &PWH);
-    return fr_slab_layout(W, H, cap, with_intrinsic).bytes + fr_align(tiled) + fr_align(PWH);
+    size_t dbytes = 0, PWH = 0;
+    fr_dist_geometry(W, H, mx, mz, &pad, &PW, &PH, &TW, &dbytes, &PWH);
+    return fr_slab_layout(W, H, cap, with_intrinsic).bytes + fr_align(dbytes) + fr_align(PWH);
 }
 
 void FrSlab::allocate(int W_, int H_, int mx_, int mz_, int path_cap_, bool with_intrinsic_, hipStream_t st)
@@ -450,6 +451,26 @@ void FrEngine::set_block(int threads)
     L_.block = threads;
 }
 
+/* The footprint's (2 mz + 1) x (2 mx + 1) samples in FR_GROUP x FR_GROUP
+ * groups (the last row and column of groups may be narrower); each group's
+ * middle sample, clamped into the group, is its probe. */
+static void fr_probe_table(FrPlanDev &d)
+{
+    const int gr = (2 * d.mz + FR_GROUP) / FR_GROUP, gc = (2 * d.mx + FR_GROUP) / FR_GROUP;
+    d.n_groups = (d.mx >= 0 && d.mz >= 0 && gr * gc <= FR_MAX_GROUPS) ? gr * gc : 0;
+    for (int g = 0; g < FR_MAX_GROUPS; ++g) { /* entries past the groups: the centre sample */
+        d.probe_i[g] = 0.0f;
+        d.probe_j[g] = 0.0f;
+    }
+    for (int g = 0; g < d.n_groups; ++g) {
+        const int a = g / gc, b = g % gc;
+        const int i0 = -d.mz + a * FR_GROUP, j0 = -d.mx + b * FR_GROUP;
+        const int i1 = std::min(i0 + FR_GROUP - 1, d.mz), j1 = std::min(j0 + FR_GROUP - 1, d.mx);
+        d.probe_i[g] = (float)std::min(i0 + FR_GROUP / 2, i1);
+        d.probe_j[g] = (float)std::min(j0 + FR_GROUP / 2, j1);
+    }
+}
+
 void FrEngine::bind(int slot, FrSlab &s, const FrPlanParams &p, const FrFrameView &frame, bool upload)
 {
     FrPlanDev d{};
@@ -462,6 +483,7 @@ void FrEngine::bind(int slot, FrSlab &s, const FrPlanParams &p, const FrFrameVie
     d.TW = s.TW;
     d.mx = s.mx;
     d.mz = s.mz;
+    fr_probe_table(d);
     d.lbx = p.lbx;
     d.lbz = p.lbz;
     d.ubx = p.ubx;

@@ -55,7 +55,7 @@ __device__ bool dev_feasible(const FrPlanDev &P, int x, int z, float h)
         return true;
     float s, c;
     fr_sincosf(h, &s, &c);
-    const uint8_t *D = P.dist;
+    const FR_GLOBAL uint8_t *D = P.dist;
     const int TW = P.TW, pad = P.pad;
     if (c != c || s != s) /* every sample truncates to pixel (0, 0) */
         return D[fr_dist_at(TW, pad, pad)] != 0;
@@ -270,20 +270,15 @@ __device__ __forceinline__ int dev_wave_incl_scan(int v)
  * within hypot(3, 3) of it before truncation, so within Chebyshev distance 5
  * after, and a probe at distance >= 6 from every obstacle certifies its
  * group.  A probe at distance 0 is itself an obstacle sample.  The probe
- * phase reads every probe; the group phase samples the groups their probes
- * left open, skipping the samples whose cell lies within the probe's own
- * distance of the probe's cell (those cells are free).  The sample
+ * phase reads every probe (offsets from the plan's table); the group phase
+ * samples all 49 slots of each group its probe left open.  The sample
  * coordinates are the reference's expressions, so the result is its boolean;
  * the two phases may run at different times since any collision rejects. */
-#define FR_GROUP 7
-#define FR_GROUP_FREE 6
-#define FR_MAX_GROUPS 32
 #define FR_PROBE_BATCH 24
 
 struct FrOpen {
     float s, c;              /* sin, cos of the heading */
     uint32_t open;           /* groups left open by their probes */
-    unsigned long long code; /* per open group: min(probe distance, 4) - 1, 2 bits */
 };
 
 __device__ __forceinline__ void dev_fp_cell(int j, int i, float s, float c, float fxc, float fzc, int *xl, int *zl)
@@ -305,16 +300,26 @@ __device__ __forceinline__ void dev_group_box(const FrPlanDev &P, int a, int b, 
     *jc = *j0 + FR_GROUP / 2 < *j1 ? *j0 + FR_GROUP / 2 : *j1;
 }
 
-/* Probe phase: -1 collision, 1 free, 0 open (groups to sample in o). */
+/* Cell of the sample in footprint column fj, row fi (floats of the
+ * reference's integer indices): the expressions of path_feasible.cu:48-60. */
+__device__ __forceinline__ void dev_fp_cell_f(float fj, float fi, float s, float c, float fxc, float fzc, int *xl,
+                                              int *zl)
+{
+    const float is = fi * s, ic = fi * c;
+    *xl = (int)(fj * c - is + fxc);
+    *zl = (int)(fj * s + ic + fzc);
+}
+
+/* Probe phase: -1 collision, 1 free, 0 open (groups to sample in o).  The
+ * probes' offsets come from the plan's table (P.probe_j / P.probe_i). */
 __device__ int dev_lane_probe(const FrPlanDev &P, int x, int z, float h, FrOpen *o)
 {
-    const uint8_t *D = P.dist;
+    const FR_GLOBAL uint8_t *D = P.dist;
     const int TW = P.TW, pad = P.pad;
     float s, c;
     fr_sincosf(h, &s, &c);
     const bool nan_h = c != c || s != s;
-    const int gr = (2 * P.mz + FR_GROUP) / FR_GROUP, gc = (2 * P.mx + FR_GROUP) / FR_GROUP, ng = gr * gc;
-    const bool grouped = ng <= FR_MAX_GROUPS;
+    const int ng = P.n_groups;
     const float fxc = (float)x, fzc = (float)z;
     /* Every byte the test may need first - the cell's bits, the centre and
      * origin distances, the first batch of probes - so that one round trip
@@ -325,20 +330,12 @@ __device__ int dev_lane_probe(const FrPlanDev &P, int x, int z, float h, FrOpen 
     const int dc = D[fr_dist_at(TW, x + pad, z + pad)];
     const int d00 = D[fr_dist_at(TW, pad, pad)];
     int dp[FR_PROBE_BATCH];
-    int a = 0, b = 0; /* group g = a * gc + b */
 #pragma unroll
     for (int k = 0; k < FR_PROBE_BATCH; ++k) {
-        dp[k] = 255;
-        if (grouped && k < ng) {
-            int i0, i1, j0, j1, ic, jc, px, pz;
-            dev_group_box(P, a, b, &i0, &i1, &j0, &j1, &ic, &jc);
-            dev_fp_cell(jc, ic, sp, cp, fxc, fzc, &px, &pz);
-            dp[k] = D[fr_dist_at(TW, px + pad, pz + pad)];
-        }
-        if (++b == gc) {
-            b = 0;
-            ++a;
-        }
+        /* table entries past the footprint's groups hold the centre sample */
+        int px, pz;
+        dev_fp_cell_f(P.probe_j[k], P.probe_i[k], sp, cp, fxc, fzc, &px, &pz);
+        dp[k] = D[fr_dist_at(TW, px + pad, pz + pad)];
     }
     if (tv & 1)
         return 1;
@@ -346,77 +343,93 @@ __device__ int dev_lane_probe(const FrPlanDev &P, int x, int z, float h, FrOpen 
         return d00 != 0 ? 1 : -1;
     if (dc >= pad)
         return 1;
-    if (!grouped)
+    if (ng == 0)
         return dev_feasible(P, x, z, h) ? 1 : -1;
     uint32_t open = 0;
-    unsigned long long code = 0;
-    for (int g0 = 0;;) {
 #pragma unroll
-        for (int k = 0; k < FR_PROBE_BATCH; ++k) {
-            if (dp[k] == 0)
-                return -1;
-            if (dp[k] < FR_GROUP_FREE) {
-                const int g = g0 + k;
-                open |= 1u << g;
-                code |= (unsigned long long)((dp[k] < 4 ? dp[k] : 4) - 1) << (2 * g);
-            }
-        }
-        g0 += FR_PROBE_BATCH;
-        if (g0 >= ng)
+    for (int k = 0; k < FR_PROBE_BATCH; ++k) {
+        if (k >= ng)
             break;
-        /* the next batch (footprints of more than FR_PROBE_BATCH groups) */
+        if (dp[k] == 0)
+            return -1;
+        if (dp[k] < FR_GROUP_FREE) {
+            open |= 1u << k;
+        }
+    }
+    if (ng > FR_PROBE_BATCH) {
+        /* the rest of a footprint of more than FR_PROBE_BATCH groups */
 #pragma unroll
-        for (int k = 0; k < FR_PROBE_BATCH; ++k) {
-            dp[k] = 255;
-            if (g0 + k < ng) {
-                int i0, i1, j0, j1, ic, jc, px, pz;
-                dev_group_box(P, a, b, &i0, &i1, &j0, &j1, &ic, &jc);
-                dev_fp_cell(jc, ic, s, c, fxc, fzc, &px, &pz);
-                dp[k] = D[fr_dist_at(TW, px + pad, pz + pad)];
-            }
-            if (++b == gc) {
-                b = 0;
-                ++a;
+        for (int k = FR_PROBE_BATCH; k < FR_MAX_GROUPS; ++k) {
+            int px, pz;
+            dev_fp_cell_f(P.probe_j[k], P.probe_i[k], s, c, fxc, fzc, &px, &pz);
+            dp[k - FR_PROBE_BATCH] = D[fr_dist_at(TW, px + pad, pz + pad)];
+        }
+#pragma unroll
+        for (int k = FR_PROBE_BATCH; k < FR_MAX_GROUPS; ++k) {
+            if (k >= ng)
+                break;
+            const int d = dp[k - FR_PROBE_BATCH];
+            if (d == 0)
+                return -1;
+            if (d < FR_GROUP_FREE) {
+                open |= 1u << k;
             }
         }
     }
     o->s = s;
     o->c = c;
     o->open = open;
-    o->code = code;
     return open ? 0 : 1;
 }
 
-/* Sampling of open group g of the footprint at (x, z): true when one of its
- * samples is an obstacle. */
-__device__ __forceinline__ bool dev_group_hit(const FrPlanDev &P, int x, int z, float s, float c,
-                                              unsigned long long code, int g)
+/* A pointer every lane of the wave holds (the callers' plan is uniform), in
+ * scalar registers. */
+template <typename T>
+__device__ __forceinline__ const FR_GLOBAL T *dev_uniform_ptr(const FR_GLOBAL T *p)
 {
-    const uint8_t *D = P.dist;
+    const unsigned long long u = (unsigned long long)p;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)u);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
+    return (const FR_GLOBAL T *)(((unsigned long long)hi << 32) | lo);
+}
+
+/* Sampling of open group g of the footprint at (x, z): true when one of its
+ * samples is an obstacle.  `D`: the plan's distance plane. */
+__device__ __forceinline__ bool dev_group_hit(const FrPlanDev &P, const FR_GLOBAL uint8_t *D, int x, int z, float s,
+                                              float c, int g)
+{
     const int TW = P.TW, pad = P.pad;
     const int gc = (2 * P.mx + FR_GROUP) / FR_GROUP;
     const float fxc = (float)x, fzc = (float)z;
-    int i0, i1, j0, j1, ic, jc, pcx, pcz;
+    int i0, i1, j0, j1, ic, jc;
     const int a = g / gc;
     dev_group_box(P, a, g - a * gc, &i0, &i1, &j0, &j1, &ic, &jc);
-    dev_fp_cell(jc, ic, s, c, fxc, fzc, &pcx, &pcz);
-    const int pd = (int)((code >> (2 * g)) & 3ull) + 1;
-    int hit = 0;
+    /* The row and column products of path_feasible.cu:48-60 once per row and
+     * column.  Rows and columns past the footprint (the last row or column of
+     * groups may be narrower) repeat the group's last one, so every slot of
+     * the 7 x 7 box tests a sample of the group (some twice) and the loads
+     * issue back to back, with no per-lane branch (a load under one is
+     * waited for before the branch joins). */
+    float rs[FR_GROUP], rc[FR_GROUP], cc[FR_GROUP], cs[FR_GROUP];
+#pragma unroll
+    for (int k = 0; k < FR_GROUP; ++k) {
+        const float fi = (float)(i0 + k < i1 ? i0 + k : i1), fj = (float)(j0 + k < j1 ? j0 + k : j1);
+        rs[k] = fi * s;
+        rc[k] = fi * c;
+        cc[k] = fj * c;
+        cs[k] = fj * s;
+    }
+    int lowest = 255;
 #pragma unroll
     for (int di = 0; di < FR_GROUP; ++di) {
 #pragma unroll
         for (int dj = 0; dj < FR_GROUP; ++dj) {
-            const int i = i0 + di, j = j0 + dj;
-            if (i <= i1 && j <= j1) {
-                int xl, zl;
-                dev_fp_cell(j, i, s, c, fxc, fzc, &xl, &zl);
-                const int ax = xl > pcx ? xl - pcx : pcx - xl, az = zl > pcz ? zl - pcz : pcz - zl;
-                if (ax >= pd || az >= pd)
-                    hit |= D[fr_dist_at(TW, xl + pad, zl + pad)] == 0;
-            }
+            const int xl = (int)(cc[dj] - rs[di] + fxc), zl = (int)(cs[dj] + rc[di] + fzc);
+            const int v = D[fr_dist_at(TW, xl + pad, zl + pad)];
+            lowest = v < lowest ? v : lowest;
         }
     }
-    return hit != 0;
+    return lowest == 0;
 }
 
 /* Per-wave record of up to 64 footprints waiting for their group phase. */
@@ -425,15 +438,18 @@ struct FrOpenSet {
     float s[64], c[64];
     uint32_t open[64];
     int pref[64];
-    unsigned long long code[64];
 };
 
 /* Group phase of footprints 0..m-1 of a per-wave set, the open groups of all
  * of them dealt to the lanes in rounds of 64 (a footprint near obstacles
  * with many open groups no longer holds the wave for one round per group).
  * False when any collides. */
-__device__ bool dev_groups_dealt(const FrPlanDev &P, FrOpenSet &b, int m)
+__device__ bool dev_groups_dealt(const FrPlanDev &P, FrOpenSet &bg, int m)
 {
+    /* the set is in LDS: addressed as such (ds_* instead of flat accesses;
+     * this function is not inlined, so its reference arrives generic) */
+    __attribute__((address_space(3))) FrOpenSet &b = *(__attribute__((address_space(3))) FrOpenSet *)&bg;
+    const FR_GLOBAL uint8_t *D = dev_uniform_ptr(P.dist);
     const int lane = dev_lane();
     const int cnt = lane < m ? __popc(b.open[lane]) : 0;
     const int incl = dev_wave_incl_scan(cnt);
@@ -458,7 +474,7 @@ __device__ bool dev_groups_dealt(const FrPlanDev &P, FrOpenSet &b, int m)
             uint32_t open = b.open[lo];
             for (int r = t - b.pref[lo]; r > 0; --r)
                 open &= open - 1;
-            hit = dev_group_hit(P, b.x[lo], b.z[lo], b.s[lo], b.c[lo], b.code[lo], __ffs((int)open) - 1);
+            hit = dev_group_hit(P, D, b.x[lo], b.z[lo], b.s[lo], b.c[lo], __ffs((int)open) - 1);
         }
         if (__ballot(hit))
             return false;
@@ -517,7 +533,6 @@ struct FrOpenBuf {
     int x[FR_OPEN_BUF], z[FR_OPEN_BUF];
     float s[FR_OPEN_BUF], c[FR_OPEN_BUF];
     uint32_t open[FR_OPEN_BUF];
-    unsigned long long code[FR_OPEN_BUF];
 };
 
 /* Group phase of the first m (<= 64) buffered footprints (their groups dealt
@@ -531,7 +546,6 @@ __device__ bool dev_open_flush(const FrPlanDev &P, FrOpenBuf &b, FrOpenSet &set,
         set.s[lane] = b.s[lane];
         set.c[lane] = b.c[lane];
         set.open[lane] = b.open[lane];
-        set.code[lane] = b.code[lane];
     }
     __builtin_amdgcn_wave_barrier();
     if (!dev_groups_dealt(P, set, m))
@@ -543,7 +557,6 @@ __device__ bool dev_open_flush(const FrPlanDev &P, FrOpenBuf &b, FrOpenSet &set,
         b.s[lane] = b.s[m + lane];
         b.c[lane] = b.c[m + lane];
         b.open[lane] = b.open[m + lane];
-        b.code[lane] = b.code[m + lane];
     }
     return true;
 }
@@ -699,6 +712,38 @@ __device__ __forceinline__ bool dev_curve_too_sharp(const FrHermite &hm, double 
     return k > max_curv;
 }
 
+/* True when no point of the curve can fail the curvature test: an upper
+ * bound of the curvature over all t in [0, 1] is at most 0.75 max_curv.
+ * With P'(t) = A t^2 + B t + C and P''(t) = 2 A t + B (A = 6 (p0 - p1) +
+ * 3 (m0 + m1), B = 6 (p1 - p0) - 4 m0 - 2 m1, C = m0), the cross product
+ * P' x P'' = -(A x B) t^2 + 2 (C x A) t + C x B is bounded on [0, 1] by the
+ * sum of its coefficients' magnitudes, and |P'| from below by its value at
+ * the middles of four quarters less the largest |P''| (at an end: P'' is
+ * linear) times the distance to a middle.  The 25 % margin dwarfs the
+ * rounding of the per-point evaluation (relative 1e-12 when |P'| >= 0.5),
+ * so a skipped test would have passed.  NaN set-ups give false. */
+__device__ __forceinline__ bool dev_curve_gentle(const FrHermite &hm, float max_curv)
+{
+    const double ax = 6.0 * (hm.dx0 - hm.dgx) + 3.0 * (hm.t1x + hm.t2x);
+    const double az = 6.0 * (hm.dz0 - hm.dgz) + 3.0 * (hm.t1y + hm.t2y);
+    const double bx = 6.0 * (hm.dgx - hm.dx0) - 4.0 * hm.t1x - 2.0 * hm.t2x;
+    const double bz = 6.0 * (hm.dgz - hm.dz0) - 4.0 * hm.t1y - 2.0 * hm.t2y;
+    const double cx = hm.t1x, cz = hm.t1y;
+    const double axb = ax * bz - az * bx, cxa = cx * az - cz * ax, cxb = cx * bz - cz * bx;
+    const double qmax = __builtin_fabs(axb) + 2.0 * __builtin_fabs(cxa) + __builtin_fabs(cxb);
+    const double ex = 2.0 * ax + bx, ez = 2.0 * az + bz;
+    const double m2 = __builtin_fmax(bx * bx + bz * bz, ex * ex + ez * ez);
+    double lo2 = 1e300;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const double t = 0.125 + 0.25 * j;
+        const double px = (ax * t + bx) * t + cx, pz = (az * t + bz) * t + cz;
+        lo2 = __builtin_fmin(lo2, px * px + pz * pz);
+    }
+    const double L = __builtin_sqrt(lo2) - 0.125 * __builtin_sqrt(m2);
+    return L >= 0.5 && qmax <= 0.75 * (double)max_curv * (L * L * L);
+}
+
 /* Pass 0, chunk by chunk of 64 points: each chunk's geometry and curvature,
  * then its centre cells' loads; the previous chunk's bytes are tested while
  * these are in flight.  Most rejected curves meet their first obstacle pixel
@@ -794,7 +839,6 @@ __device__ __forceinline__ float dev_wave_curve(const FrPlanDev &P, const FrHerm
                 ob.s[r] = o.s;
                 ob.c[r] = o.c;
                 ob.open[r] = o.open;
-                ob.code[r] = o.code;
             }
             nb += __popcll(om);
             if (nb >= 64 && !dev_open_flush(P, ob, os, 64, &nb))
@@ -1293,7 +1337,6 @@ __device__ bool dev_roll_flush(const FrPlanDev &P, const int *bx, const int *bz,
         set.s[lane] = o.s;
         set.c[lane] = o.c;
         set.open[lane] = pr == 0 ? o.open : 0u;
-        set.code[lane] = o.code;
     }
     if (__ballot(pr < 0))
         return false;
@@ -2048,10 +2091,16 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_demand(const FrPlanDev *plans, 
  * 64 consecutive queue entries, one block per wave (more cells per wave as
  * the queue grows): the lanes set up the block's goal curves together, one
  * cell per lane, then the wave walks the block's curves one by one. */
-__device__ __forceinline__ int dev_gc_block(long n, long nwaves)
+#ifndef FR_GC_EVAL_MIN
+#define FR_GC_EVAL_MIN 4
+#endif
+#ifndef FR_GC_FOOT_MIN
+#define FR_GC_FOOT_MIN 1
+#endif
+__device__ __forceinline__ int dev_gc_block(long n, long nwaves, int lo)
 {
     const long c = (n + nwaves - 1) / nwaves;
-    return c < 1 ? 1 : (c > 64 ? 64 : (int)c);
+    return c < lo ? lo : (c > 64 ? 64 : (int)c);
 }
 
 /* Per lane: set-up of the goal curve of queue entry d (base: the cost the
@@ -2078,7 +2127,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *__restric
         n = dq_cap;
     const int wpb = blockDim.x >> 6;
     const long nwaves = (long)gridDim.x * wpb;
-    const int cpw = dev_gc_block(n, nwaves);
+    const int cpw = dev_gc_block(n, nwaves, FR_GC_EVAL_MIN);
     const int lane = dev_lane();
     const long gw = (long)blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     for (long b0 = gw * cpw; b0 < n; b0 += nwaves * cpw) {
@@ -2087,7 +2136,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *__restric
         FrHermite hm = {};
         int np = 0;
         float lh = 0, base = 0;
-        bool ok = false;
+        bool ok = false, gentle = false;
         if (lane < cnt) {
             d = dq[b0 + lane];
             ok = dev_gc_lane_setup(plans, d, false, &hm, &np, &lh, &base);
@@ -2097,8 +2146,9 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *__restric
             const int z = d.y / P.W, x = d.y - z * P.W;
             if (ok && !(P.trav[d.y] & 1) && P.dist[fr_dist_at(P.TW, x + P.pad, z + P.pad)] == 0)
                 ok = false;
+            gentle = ok && dev_curve_gentle(hm, P.max_curv_goal);
         }
-        const unsigned long long okm = __ballot(ok);
+        const unsigned long long okm = __ballot(ok), gentlem = __ballot(gentle);
         unsigned long long kept = 0; /* cells of the block queued for k_gc_foot */
         for (int k = 0; k < cnt; ++k) {
             const int slot = dev_bcast_i(d.x, k), pos = dev_bcast_i(d.y, k);
@@ -2106,8 +2156,9 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *__restric
             bool keep = false;
             if ((okm >> k) & 1ull) {
                 const FrHermite hk = dev_bcast_hermite(hm, k);
-                keep = dev_wave_curve(P, hk, dev_bcast_i(np, k), P.max_curv_goal, P.gx, P.gz, false, false, 0.0f, 1) >=
-                       0;
+                /* a curve whose curvature bound clears the limit skips the per-point test */
+                const float mc = ((gentlem >> k) & 1ull) ? 0.0f : P.max_curv_goal;
+                keep = dev_wave_curve(P, hk, dev_bcast_i(np, k), mc, P.gx, P.gz, false, false, 0.0f, 1) >= 0;
             }
             if (keep)
                 kept |= 1ull << k;
@@ -2147,7 +2198,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_foot(const FrPlanDev *__restric
     const long n = counters[FR_CNT_DEMAND2];
     const int wpb = blockDim.x >> 6;
     const long nwaves = (long)gridDim.x * wpb;
-    const int cpw = dev_gc_block(n, nwaves);
+    const int cpw = dev_gc_block(n, nwaves, FR_GC_FOOT_MIN);
     const int lane = dev_lane();
     const long gw = (long)blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     for (long b0 = gw * cpw; b0 < n; b0 += nwaves * cpw) {
