@@ -913,13 +913,13 @@ class FrFastRRT {
         g.eng.read_ctl(0, &c);
         ctl_status = c.status;
         last_count = c.last_temp;
-        last_expansions = c.n_expansions;
-        last_new = c.n_temp;
-        last_fresh = c.n_fresh;
+        last_expansions = c.stat_expansions;
+        last_new = c.last_temp;
+        last_fresh = c.stat_fresh;
         iterations = c.iter;
         g.goal_valid = true;
         goal_flag = c.goal_reached != 0;
-        if (c.reseeded)
+        if (c.stat_reseeded)
             return true;
         return !goal_flag;
     }

@@ -547,14 +547,17 @@ class FrBatch {
         fr_prof_enable(false); /* no event nodes inside the graphs */
         for (int part = 0; part < (timed ? 3 : 1); ++part) {
             FR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            /* no k_begin: admission (k_batch_start) and each k_finish leave a
+             * plan ready for its next iteration, and the host does not edit a
+             * plan between iterations */
             if (!timed) {
-                fr_launch_iteration(L, st);
+                fr_launch_iteration(L, st, false);
             } else if (part == 0) {
-                fr_launch_iteration_pre(L, st);
+                fr_launch_iteration_pre(L, st, false);
             } else if (part == 1) {
                 fr_launch_iteration_expand(L, st);
             } else {
-                fr_launch_iteration_post(L, st);
+                fr_launch_iteration_post(L, st, false);
             }
             FR_CHECK(hipStreamEndCapture(st, &graph_[part]));
             FR_CHECK(hipGraphInstantiate(&exec_[part], graph_[part], nullptr, nullptr, 0));

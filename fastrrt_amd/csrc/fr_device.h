@@ -135,7 +135,8 @@ struct FrCtl {
     int n_expansions;         /* rollouts queued this iteration */
     int n_fresh;              /* nodes goal-checked this iteration */
     long long best_lcost;     /* goal direct connection: min floor(100 * total) */
-    unsigned long long best_node_key;
+    unsigned long long best_pick; /* min (lcost << 24 | node cell) over the connections (~0: none) */
+    int pick_slow;            /* a connection outside best_pick's fields: k_finish picks by scanning */
     int found;                /* direct connection added this iteration */
     /* path extraction / optimisation */
     long long bn_cost;
@@ -152,6 +153,9 @@ struct FrCtl {
     int opt_active;           /* batch finalisation: optimisation rounds pending */
     int n_eligible;           /* nodes the expansion gate passed this iteration (before any thinning) */
     int prev_eligible;        /* the same, previous iteration */
+    /* the last finished iteration's figures (the per-iteration fields above
+     * are reset for the next iteration when one finishes) */
+    int stat_expansions, stat_fresh, stat_reseeded;
 };
 
 /* One queued expansion (one rollout). */

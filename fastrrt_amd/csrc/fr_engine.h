@@ -17,9 +17,11 @@
 #include "fr_device.h"
 
 /* FR_CNT_CONV: the entries at the head of the fresh queue that the erase
- * converted (COLLISION -> NODE), queued before the accepted nodes */
+ * converted (COLLISION -> NODE), queued before the accepted nodes;
+ * FR_CNT_FRESH_SEEN: the fresh queue's length kept for k_finish (which resets
+ * the others for the next iteration) */
 enum { FR_CNT_EXP = 0, FR_CNT_FRESH = 1, FR_CNT_PAIRS = 2, FR_CNT_DEMAND = 3, FR_CNT_DEMAND2 = 4, FR_CNT_CONV = 5,
-       FR_CNT_N = 8 };
+       FR_CNT_FRESH_SEEN = 6, FR_CNT_N = 8 };
 
 /* goal-connection planes at setup: untouched, evaluated when first needed
  * (lazy), or evaluated for every cell (eager) */
@@ -63,11 +65,14 @@ void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, in
                      int goal_mode, hipStream_t st);
 /* evaluate the goal-connection planes of every cell not yet evaluated */
 void fr_launch_goal_complete(const FrLaunch &L, const int *d_slots, int n, int max_N, hipStream_t st);
-void fr_launch_iteration(const FrLaunch &L, hipStream_t st);
+/* entry: run k_begin first (the FastRRT handle, whose host may edit the plan
+ * between iterations); without it an iteration starts from the state the
+ * previous k_finish or the batch admission left (the batch engine) */
+void fr_launch_iteration(const FrLaunch &L, hipStream_t st, bool entry = true);
 /* the same iteration in three parts: before, k_expand, after */
-void fr_launch_iteration_pre(const FrLaunch &L, hipStream_t st);
+void fr_launch_iteration_pre(const FrLaunch &L, hipStream_t st, bool entry = true);
 void fr_launch_iteration_expand(const FrLaunch &L, hipStream_t st);
-void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st);
+void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st, bool entry = true);
 void fr_launch_clear(const FrLaunch &L, int slot, int N, hipStream_t st);
 void fr_launch_count(const FrLaunch &L, int slot, int N, int *d_out, hipStream_t st);
 void fr_launch_goal_scan(const FrLaunch &L, int slot, int N, int gx, int gz, float gh, float tol, float herr,

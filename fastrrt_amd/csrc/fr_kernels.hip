@@ -1100,31 +1100,17 @@ __global__ void __launch_bounds__(FR_BLOCK) k_setup_goal_connection(const FrPlan
 /* loop iteration                                                           */
 /* ------------------------------------------------------------------------ */
 
-/* One wave per plan: lane 0 resets the plan's per-iteration control fields,
- * the lanes together clear the density counts the iteration will write. */
-__global__ void __launch_bounds__(FR_BLOCK) k_begin(const FrPlanDev *plans, FrCtl *ctl, const int *slots, const int *n_slots_p, int *counters)
+/* The state a loop iteration starts from, for one plan (one wave: the lanes
+ * clear the density counts the iteration will write, lane 0 the control
+ * fields; `iter`: the iteration about to run).  Run by k_finish for the
+ * plan's next iteration; k_begin runs it again at the FastRRT handle's
+ * entries (a repeat only advances the claim epoch once more). */
+__device__ void dev_ctl_ready(const FrPlanDev &P, FrCtl &C, int iter)
 {
-    const int i = blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
     const int lane = dev_lane();
-    const int n_slots = *n_slots_p;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        counters[FR_CNT_EXP] = 0;
-        counters[FR_CNT_FRESH] = 0;
-        counters[FR_CNT_PAIRS] = 0;
-        counters[FR_CNT_DEMAND] = 0;
-        counters[FR_CNT_DEMAND2] = 0;
-        counters[FR_CNT_CONV] = 0;
-    }
-    if (i >= n_slots)
-        return;
-    const int s = slots[i];
-    FrCtl &C = ctl[s];
-    const FrPlanDev &P = plans[s];
-    if (C.status != FR_SEARCHING)
-        return;
     if (P.smart) {
         const int nr = P.dens_w * P.dens_h;
-        uint32_t *wr = P.region + ((C.iter + 1) & 1) * nr;
+        FR_GLOBAL uint32_t *wr = P.region + ((iter + 1) & 1) * nr;
         for (int r = lane; r < nr; r += 64)
             wr[r] = 0;
     }
@@ -1132,8 +1118,6 @@ __global__ void __launch_bounds__(FR_BLOCK) k_begin(const FrPlanDev *plans, FrCt
         return;
     C.frontier = C.last_temp >= 100;
     C.force = C.last_temp == 0;
-    C.prev_eligible = C.n_eligible;
-    C.n_eligible = 0;
     C.collision = 0;
     C.n_node_pre = 0;
     C.n_conv = 0;
@@ -1144,9 +1128,37 @@ __global__ void __launch_bounds__(FR_BLOCK) k_begin(const FrPlanDev *plans, FrCt
     C.n_expansions = 0;
     C.n_fresh = 0;
     C.best_lcost = (long long)~0ull; /* holds fr_lcost_key() values */
-    C.best_node_key = ~0ull;
+    C.best_pick = ~0ull;
+    C.pick_slow = 0;
     C.epoch += 1;
     C.chain_only = 0;
+}
+
+__device__ __forceinline__ void dev_counters_reset(int *counters)
+{
+    counters[FR_CNT_EXP] = 0;
+    counters[FR_CNT_FRESH] = 0;
+    counters[FR_CNT_PAIRS] = 0;
+    counters[FR_CNT_DEMAND] = 0;
+    counters[FR_CNT_DEMAND2] = 0;
+    counters[FR_CNT_CONV] = 0;
+}
+
+/* Entry reset (one wave per plan) before a loop iteration of the FastRRT
+ * handle, whose host may have edited the plan since the last iteration. */
+__global__ void __launch_bounds__(FR_BLOCK) k_begin(const FrPlanDev *plans, FrCtl *ctl, const int *slots, const int *n_slots_p, int *counters)
+{
+    const int i = blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+    const int n_slots = *n_slots_p;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        dev_counters_reset(counters);
+    if (i >= n_slots)
+        return;
+    const int s = slots[i];
+    FrCtl &C = ctl[s];
+    if (C.status != FR_SEARCHING)
+        return;
+    dev_ctl_ready(plans[s], C, C.iter);
 }
 
 /* Tile scans: one wave per tile of FR_TILE = 1024 cells, lane l holding
@@ -2276,6 +2288,10 @@ __global__ void __launch_bounds__(FR_BLOCK) k_connect_emit(const FrPlanDev *plan
             atomicAdd(&C.n_fresh, 1);
             if (dev_goal_hit(P, x, z, P.heading[fpos]))
                 C.goal_reached = 1;
+            /* the node's queue entry, found from its cell by k_finish (the
+             * erase's scratch plane is free until the next erase; a node
+             * queued twice has identical entries) */
+            P.escr[fpos] = i;
         }
         FrWin w;
         if (!dev_win_make(P, x, z, &w))
@@ -2321,6 +2337,18 @@ __global__ void __launch_bounds__(FR_BLOCK) k_connect_emit(const FrPlanDev *plan
     }
 }
 
+/* The plan's pick among its connections (the least lcost, then the lowest
+ * node cell: the pick of graph_check_goal_reached.cu:115-139 with ties
+ * ordered), as one 64-bit minimum; connections outside its fields (lcost not
+ * in [0, 2^40), cell beyond 2^24) leave the pick to k_finish's scan. */
+__device__ __forceinline__ void dev_pick(FrCtl &C, long long lc, int pos)
+{
+    if (lc >= 0 && lc < (1LL << 40) && pos < (1 << 24))
+        atomicMin(&C.best_pick, ((unsigned long long)lc << 24) | (unsigned)pos);
+    else
+        C.pick_slow = 1;
+}
+
 /* Nodes whose pairs did not fit the pair queue scan their window in-thread
  * (thread `t` of `nt`); run by k_connect_eval after its pairs. */
 __device__ void dev_connect_inline(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, int n, int t, int nt)
@@ -2354,15 +2382,17 @@ __device__ void dev_connect_inline(const FrPlanDev *plans, FrCtl *ctl, FrFresh *
                 rank++;
             }
         f.key = best;
-        if (bl < FR_BIG_COST)
+        if (bl < FR_BIG_COST) {
             atomicMin((unsigned long long *)&ctl[f.slot].best_lcost, fr_lcost_key(bl));
+            dev_pick(ctl[f.slot], bl, f.pos);
+        }
     }
 }
 
 /* one wave per (node, flagged cell) pair, then the nodes whose pairs did
  * not fit (their flag keeps them out of the pair loop) */
 __global__ void __launch_bounds__(FR_BLOCK) k_connect_eval(const FrPlanDev *plans, FrCtl *ctl, FrFresh *fq, FrPair *pairs, long pair_cap,
-                               const int *counters)
+                               int *counters)
 {
     long n = counters[FR_CNT_PAIRS];
     if (n > pair_cap)
@@ -2387,23 +2417,12 @@ __global__ void __launch_bounds__(FR_BLOCK) k_connect_eval(const FrPlanDev *plan
         pairs[p].cost = c; /* read back by k_finish for the winning pair */
         atomicMin(&f.key, dev_node_key(lc, (int)(p - f.pair_base)));
         atomicMin((unsigned long long *)&ctl[f.slot].best_lcost, fr_lcost_key(lc));
+        dev_pick(ctl[f.slot], lc, f.pos);
     }
     dev_connect_inline(plans, ctl, fq, n_fresh, (int)(blockIdx.x * blockDim.x + threadIdx.x),
                        (int)(gridDim.x * blockDim.x));
-}
-
-__global__ void __launch_bounds__(FR_BLOCK) k_connect_pick(const FrPlanDev *plans, FrCtl *ctl, const FrFresh *fq, const int *counters)
-{
-    const int n = counters[FR_CNT_FRESH];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const FrFresh f = fq[i];
-        if (f.key == ~0ull)
-            continue;
-        FrCtl &C = ctl[f.slot];
-        if (fr_lcost_key(dev_key_lcost(f.key)) != (unsigned long long)C.best_lcost)
-            continue;
-        atomicMin(&C.best_node_key, ((unsigned long long)(unsigned)f.pos << 32) | (unsigned)i);
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        counters[FR_CNT_FRESH_SEEN] = n_fresh;
 }
 
 /* the rank-th goal-flagged cell of a node's window, in row-major order
@@ -2442,46 +2461,11 @@ __device__ int dev_wave_window_cell(const FrPlanDev &P, int node, int rank)
     return -1;
 }
 
-/* End of FastRRT::loop (fastrrt.cpp:130-180): reseed, goal-connection adds,
- * goal test, iteration bookkeeping.  One wave per plan: the lanes share the
- * winner's Hermite check, lane 0 writes. */
-__global__ void __launch_bounds__(FR_BLOCK) k_finish(const FrPlanDev *plans, FrCtl *ctl, const int *slots, const int *n_slots_p,
-                         const FrFresh *fq, const FrPair *pairs)
+/* k_finish's bookkeeping for one plan (lane 0): the reseed or the goal
+ * connection's two adds, the goal test, the status. */
+__device__ void dev_finish_plan(const FrPlanDev &P, FrCtl &C, const FrFresh *fq, unsigned long long bk, bool reseed,
+                                int cell, float c_cost, int npos, int nx, int nz)
 {
-    const int i = blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
-    const int n_slots = *n_slots_p;
-    if (i >= n_slots)
-        return;
-    const int s = slots[i];
-    FrCtl &C = ctl[s];
-    const FrPlanDev &P = plans[s];
-    if (C.status != FR_SEARCHING)
-        return;
-    const bool lead = dev_lane() == 0;
-    const bool reseed = C.n_temp == 0 && C.n_all == 0;
-    unsigned long long bk = C.best_node_key;
-    int nx = 0, nz = 0, cell = -1, npos = 0;
-    float c_cost = 0;
-    if (!reseed && bk != ~0ull) {
-        const FrFresh f = fq[(unsigned)(bk & 0xFFFFFFFFu)];
-        npos = f.pos;
-        nz = f.pos / P.W;
-        nx = f.pos - nz * P.W;
-        if (!(f.flags & FR_FRESH_INLINE)) {
-            /* the winning pair of k_connect_eval, its curve cost kept */
-            const FrPair pr = pairs[f.pair_base + dev_key_rank(f.key)];
-            cell = pr.cell;
-            c_cost = pr.cost;
-        } else {
-            cell = dev_wave_window_cell(P, f.pos, dev_key_rank(f.key));
-            if (cell >= 0)
-                (void)dev_pair_lcost(P, f.pos, cell, &c_cost, true);
-            else
-                bk = ~0ull; /* unreachable: the rank counts flags present since the pair pass */
-        }
-    }
-    if (!lead)
-        return;
     C.last_temp = C.n_temp;
     C.iter += 1;
     C.total_expansions += C.n_expansions;
@@ -2517,6 +2501,88 @@ __global__ void __launch_bounds__(FR_BLOCK) k_finish(const FrPlanDev *plans, FrC
     } else if (P.max_iter > 0 && C.iter >= P.max_iter) {
         C.status = FR_FAILED;
     }
+}
+
+/* End of FastRRT::loop (fastrrt.cpp:130-180): reseed, goal-connection adds,
+ * goal test, iteration bookkeeping.  One wave per plan: the lanes share the
+ * winner's Hermite check, lane 0 writes; then the plan's next iteration is
+ * made ready (dev_ctl_ready), the finished iteration's figures kept in the
+ * stat_* fields. */
+__global__ void __launch_bounds__(FR_BLOCK) k_finish(const FrPlanDev *plans, FrCtl *ctl, const int *slots, const int *n_slots_p,
+                         const FrFresh *fq, const FrPair *pairs, int *counters, int scan)
+{
+    const int i = blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+    const int n_slots = *n_slots_p;
+    /* the queues' counters for the next iteration (every reader has run) */
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        dev_counters_reset(counters);
+    if (i >= n_slots)
+        return;
+    const int s = slots[i];
+    FrCtl &C = ctl[s];
+    const FrPlanDev &P = plans[s];
+    if (C.status != FR_SEARCHING)
+        return;
+    const int next_iter = C.iter + 1;
+    const bool lead = dev_lane() == 0;
+    const bool reseed = C.n_temp == 0 && C.n_all == 0;
+    /* the winning node's queue entry: from the plan's pick, or (connections
+     * outside the pick's fields, or `scan`: the FastRRT handle, whose few
+     * entries make the scan cheap) the least (node cell, entry) among the
+     * entries whose best connection has the plan's least lcost */
+    int win = -1;
+    if (!C.pick_slow && !scan) {
+        if (C.best_pick != ~0ull)
+            win = P.escr[(int)(C.best_pick & 0xFFFFFFull)];
+    } else {
+        const int n = counters[FR_CNT_FRESH_SEEN];
+        unsigned long long key = ~0ull;
+        for (int k = dev_lane(); k < n; k += 64) {
+            const FrFresh f = fq[k];
+            if (f.slot == s && f.key != ~0ull &&
+                fr_lcost_key(dev_key_lcost(f.key)) == (unsigned long long)C.best_lcost) {
+                const unsigned long long kk = ((unsigned long long)(unsigned)f.pos << 32) | (unsigned)k;
+                key = kk < key ? kk : key;
+            }
+        }
+        for (int off = 32; off >= 1; off >>= 1) {
+            const unsigned long long o = __shfl_xor(key, off, 64);
+            key = o < key ? o : key;
+        }
+        if (key != ~0ull)
+            win = (int)(key & 0xFFFFFFFFull);
+    }
+    unsigned long long bk = win >= 0 ? 0ull : ~0ull;
+    int nx = 0, nz = 0, cell = -1, npos = 0;
+    float c_cost = 0;
+    if (!reseed && win >= 0) {
+        const FrFresh f = fq[win];
+        npos = f.pos;
+        nz = f.pos / P.W;
+        nx = f.pos - nz * P.W;
+        if (!(f.flags & FR_FRESH_INLINE)) {
+            /* the winning pair of k_connect_eval, its curve cost kept */
+            const FrPair pr = pairs[f.pair_base + dev_key_rank(f.key)];
+            cell = pr.cell;
+            c_cost = pr.cost;
+        } else {
+            cell = dev_wave_window_cell(P, f.pos, dev_key_rank(f.key));
+            if (cell >= 0)
+                (void)dev_pair_lcost(P, f.pos, cell, &c_cost, true);
+            else
+                bk = ~0ull; /* unreachable: the rank counts flags present since the pair pass */
+        }
+    }
+    if (lead) {
+        dev_finish_plan(P, C, fq, bk, reseed, cell, c_cost, npos, nx, nz);
+        C.stat_expansions = C.n_expansions;
+        C.stat_fresh = C.n_fresh;
+        C.stat_reseeded = C.reseeded;
+        C.prev_eligible = C.n_eligible;
+        C.n_eligible = 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    dev_ctl_ready(P, C, next_iter);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2928,11 +2994,12 @@ void fr_launch_goal_complete(const FrLaunch &L, const int *d_slots, int n, int m
 
 /* One loop iteration of every active plan, in three parts so that a caller
  * that captures graphs can time k_expand with events between them. */
-void fr_launch_iteration_pre(const FrLaunch &L, hipStream_t st)
+void fr_launch_iteration_pre(const FrLaunch &L, hipStream_t st, bool entry)
 {
     dim3 b(L.block);
-    hipLaunchKernelGGL(k_begin, dim3(fr_grid_for(L.max_slots, L.block / 64, 65535)), b, 0, st, L.plans, L.ctl, L.slots,
-                       L.n_slots, L.counters);
+    if (entry)
+        hipLaunchKernelGGL(k_begin, dim3(fr_grid_for(L.max_slots, L.block / 64, 65535)), b, 0, st, L.plans, L.ctl,
+                           L.slots, L.n_slots, L.counters);
     hipLaunchKernelGGL(k_select, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles, L.exp_q,
                        L.counters);
 }
@@ -2942,7 +3009,7 @@ void fr_launch_iteration_expand(const FrLaunch &L, hipStream_t st)
     hipLaunchKernelGGL(k_expand, dim3(L.grid_waves), dim3(L.block), 0, st, L.plans, L.ctl, L.exp_q, L.counters);
 }
 
-void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st)
+void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st, bool entry)
 {
     dim3 b(L.block);
     hipLaunchKernelGGL(k_commit, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.exp_q, L.counters);
@@ -2966,19 +3033,18 @@ void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st)
     hipLaunchKernelGGL(k_connect_eval, dim3(L.grid_waves), b, 0, st, L.plans, L.ctl, L.fresh_q, L.pairs,
                        L.pair_cap, L.counters);
     fr_prof_end(FR_PROF_CONNECT, st);
-    hipLaunchKernelGGL(k_connect_pick, dim3(L.grid_items), b, 0, st, L.plans, L.ctl, L.fresh_q, L.counters);
     hipLaunchKernelGGL(k_finish, dim3(fr_grid_for(L.max_slots, L.block / 64, 65535)), b, 0, st, L.plans, L.ctl,
-                       L.slots, L.n_slots, L.fresh_q, L.pairs);
+                       L.slots, L.n_slots, L.fresh_q, L.pairs, L.counters, entry ? 1 : 0);
 }
 
-void fr_launch_iteration(const FrLaunch &L, hipStream_t st)
+void fr_launch_iteration(const FrLaunch &L, hipStream_t st, bool entry)
 {
     fr_prof_begin(FR_PROF_ITERATION, st);
-    fr_launch_iteration_pre(L, st);
+    fr_launch_iteration_pre(L, st, entry);
     fr_prof_begin(FR_PROF_EXPAND, st);
     fr_launch_iteration_expand(L, st);
     fr_prof_end(FR_PROF_EXPAND, st);
-    fr_launch_iteration_post(L, st);
+    fr_launch_iteration_post(L, st, entry);
     fr_prof_end(FR_PROF_ITERATION, st);
 }
 
@@ -3142,9 +3208,16 @@ __global__ void __launch_bounds__(FR_BLOCK) k_batch_start(const FrPlanDev *plans
         return;
     const int s = slots[i];
     const FrPlanDev &P = plans[s];
+    /* the state k_finish leaves for a next iteration (dev_ctl_ready), as the
+     * first iteration of the plan starts from it (the density counts are
+     * zero after the slab reset) */
     FrCtl c = {};
     c.status = FR_SEARCHING;
     c.pending_start = 1;
+    c.force = 1;
+    c.best_lcost = (long long)~0ull;
+    c.best_pick = ~0ull;
+    c.epoch = 1;
     ctl[s] = c;
     dev_add(P, P.sx, P.sz, P.sh, -1, -1, 0.0f);
 }
