@@ -16,11 +16,9 @@
 
 #include "fr_device.h"
 
-/* FR_CNT_CONV: the entries at the head of the fresh queue that the erase
- * converted (COLLISION -> NODE), queued before the accepted nodes;
- * FR_CNT_FRESH_SEEN: the fresh queue's length kept for k_finish (which resets
- * the others for the next iteration) */
-enum { FR_CNT_EXP = 0, FR_CNT_FRESH = 1, FR_CNT_PAIRS = 2, FR_CNT_DEMAND = 3, FR_CNT_DEMAND2 = 4, FR_CNT_CONV = 5,
+/* FR_CNT_FRESH_SEEN: the fresh queue's length kept for k_finish (which
+ * resets the others for the next iteration) */
+enum { FR_CNT_EXP = 0, FR_CNT_FRESH = 1, FR_CNT_PAIRS = 2, FR_CNT_DEMAND = 3, FR_CNT_DEMAND2 = 4,
        FR_CNT_FRESH_SEEN = 6, FR_CNT_N = 8 };
 
 /* goal-connection planes at setup: untouched, evaluated when first needed

@@ -1141,7 +1141,6 @@ __device__ __forceinline__ void dev_counters_reset(int *counters)
     counters[FR_CNT_PAIRS] = 0;
     counters[FR_CNT_DEMAND] = 0;
     counters[FR_CNT_DEMAND2] = 0;
-    counters[FR_CNT_CONV] = 0;
 }
 
 /* Entry reset (one wave per plan) before a loop iteration of the FastRRT
@@ -1802,92 +1801,24 @@ __global__ void __launch_bounds__(FR_ERASE_BLOCK) k_erase_mark(const FrPlanDev *
         __syncthreads();
         cur ^= 1;
     }
-    /* marks are read for non-NULL cells only (k_erase_apply) */
+    /* marks are read for non-NULL cells only (k_accept) */
     for (int k = t; k < total; k += FR_ERASE_BLOCK)
         P.mark[s_pos[k]] = s_bad[cur][k];
 }
 
-/* Apply the erase marks and __CUDA_solveGraphCollision_set_nodes
- * (graph_collision.cu:85-104, first half: COLLISION -> NODE, count 0).  The
- * converted nodes are queued: they need their parents' counts incremented
- * and a goal test. */
-__global__ void __launch_bounds__(FR_BLOCK) k_erase_apply(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, const int *n_tiles_p,
-                              FrFresh *fq, int *counters)
-{
-    const int n_tiles = *n_tiles_p;
-    const int wpb = blockDim.x >> 6;
-    const int nw = gridDim.x * wpb;
-    const int lane = dev_lane();
-    for (int t = blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); t < n_tiles; t += nw) {
-        const int2 tl = tiles[t];
-        const int slot = __builtin_amdgcn_readfirstlane(tl.x);
-        const FrPlanDev &P = plans[slot];
-        const FrCtl &C = ctl[slot];
-        if (C.status != FR_SEARCHING || !dev_solve_runs(C))
-            continue;
-        const int base = __builtin_amdgcn_readfirstlane(tl.y) * FR_TILE;
-        const uint4 v = dev_tile_types(P, base);
-        /* k_erase_mark marks non-NULL cells only: a tile of NULL cells is unchanged */
-        const uint32_t nz = dev_bytes_nz(v);
-        if (!__ballot(nz != 0))
-            continue;
-        uint32_t conv = 0;
-        for (uint32_t m = nz; m; m &= m - 1) {
-            const int i = __ffs((int)m) - 1;
-            const int pos = base + 16 * lane + i;
-            if (P.mark[pos]) {
-                P.type[pos] = FR_TYPE_NULL;
-            } else if (dev_byte(v, i) == FR_TYPE_COLLISION) {
-                P.type[pos] = FR_TYPE_NODE;
-                P.derive[pos] = 0;
-                conv |= 1u << i;
-            }
-        }
-        int total;
-        int w = dev_wave_reserve(&counters[FR_CNT_FRESH], __popc(conv), &total);
-        if (lane == 0 && total)
-            atomicAdd(&counters[FR_CNT_CONV], total);
-        for (uint32_t m = conv; m; m &= m - 1) {
-            FrFresh f;
-            f.slot = slot;
-            f.pos = base + 16 * lane + __ffs((int)m) - 1;
-            f.flags = FR_FRESH_CONVERTED;
-            f.pair_base = 0;
-            f.pair_cnt = 0;
-            f.pad0 = 0;
-            f.key = ~0ull;
-            fq[w++] = f;
-        }
-    }
-}
-
-/* second half of set_nodes: the parents of converted nodes gain a child
- * (thread t of nt over the converted head of the fresh queue; run at the
- * start of k_accept, which neither reads the child counts nor writes that
- * head: its own entries go after it) */
-__device__ void dev_coll_parents(const FrPlanDev *plans, const FrFresh *fq, int n, int t, int nt)
-{
-    for (int i = t; i < n; i += nt) {
-        const FrFresh f = fq[i];
-        if (!(f.flags & FR_FRESH_CONVERTED))
-            continue;
-        const FrPlanDev &P = plans[f.slot];
-        const long pp = (long)P.pz[f.pos] * P.W + P.px[f.pos];
-        if (pp >= 0 && pp < P.N)
-            atomicAdd(&P.derive[pp], 1);
-    }
-}
-
-/* count(TEMP), countAll() and __CUDA_accept_derived_nodes
- * (fastrrt.cpp:130-142, graph_expansion.cu:36-65).  Accepted nodes, the
- * re-added start and (after external edits) every node are queued for the
- * goal test and the goal connection search.  First, the parents of the
- * nodes the erase converted gain their child (dev_coll_parents). */
+/* One sweep over the tiles: the erase's verdicts and
+ * __CUDA_solveGraphCollision_set_nodes (graph_collision.cu:85-104:
+ * COLLISION -> NODE with no children, the parent gaining one), then count(TEMP),
+ * countAll() and __CUDA_accept_derived_nodes (fastrrt.cpp:130-142,
+ * graph_expansion.cu:36-65) on the tile as the erase left it.  Converted
+ * nodes, accepted nodes, the re-added start and (after external edits)
+ * every node are queued for the goal test and the goal connection search.
+ * A converted node's parent is never converted itself (a COLLISION parent
+ * marks the child for erasing), so the parents' increments need no order
+ * against the converted nodes' resets. */
 __global__ void __launch_bounds__(FR_BLOCK) k_accept(const FrPlanDev *plans, FrCtl *ctl, const int2 *tiles, const int *n_tiles_p,
                          FrFresh *fq, int *counters)
 {
-    dev_coll_parents(plans, fq, counters[FR_CNT_CONV], (int)(blockIdx.x * blockDim.x + threadIdx.x),
-                     (int)(gridDim.x * blockDim.x));
     const int n_tiles = *n_tiles_p;
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
@@ -1901,33 +1832,60 @@ __global__ void __launch_bounds__(FR_BLOCK) k_accept(const FrPlanDev *plans, FrC
             continue;
         const int base = __builtin_amdgcn_readfirstlane(tl.y) * FR_TILE;
         const uint4 v = dev_tile_types(P, base);
-        const uint32_t nz = dev_bytes_nz(v);
+        uint32_t nz = dev_bytes_nz(v);
         if (!__ballot(nz != 0))
             continue;
+        uint32_t temps = dev_bytes_eq(v, FR_TYPE_TEMP);
+        uint32_t nodes = dev_bytes_eq(v, FR_TYPE_NODE);
+        uint32_t conv = 0;
+        if (dev_solve_runs(C)) {
+            /* k_erase_mark marked non-NULL cells only */
+            const uint32_t coll = dev_bytes_eq(v, FR_TYPE_COLLISION);
+            for (uint32_t m = nz; m; m &= m - 1) {
+                const int i = __ffs((int)m) - 1;
+                const int pos = base + 16 * lane + i;
+                if (P.mark[pos]) {
+                    P.type[pos] = FR_TYPE_NULL;
+                    nz &= ~(1u << i);
+                    temps &= ~(1u << i);
+                    nodes &= ~(1u << i);
+                } else if ((coll >> i) & 1u) {
+                    P.type[pos] = FR_TYPE_NODE;
+                    P.derive[pos] = 0;
+                    const long pp = (long)P.pz[pos] * P.W + P.px[pos];
+                    if (pp >= 0 && pp < P.N)
+                        atomicAdd(&P.derive[pp], 1);
+                    conv |= 1u << i;
+                    nodes |= 1u << i;
+                }
+            }
+        }
         const int start_pos = (P.sx >= 0 && P.sx < P.W && P.sz >= 0 && P.sz < P.H) ? P.sz * P.W + P.sx : -1;
-        const uint32_t temps = dev_bytes_eq(v, FR_TYPE_TEMP);
         uint32_t fresh = temps;
-        const uint32_t nodes = dev_bytes_eq(v, FR_TYPE_NODE);
         if (C.full_check)
             fresh |= nodes;
         else if (C.pending_start && start_pos >= base + 16 * lane && start_pos < base + 16 * lane + 16)
             fresh |= nodes & (1u << (start_pos - base - 16 * lane));
         for (uint32_t m = temps; m; m &= m - 1)
             P.type[base + 16 * lane + __ffs((int)m) - 1] = FR_TYPE_NODE;
+        /* a converted node is queued once, as converted (flagged), and once
+         * more when the accept queues every node */
         int total;
-        int w = dev_wave_reserve(&counters[FR_CNT_FRESH], __popc(fresh), &total);
+        int w = dev_wave_reserve(&counters[FR_CNT_FRESH], __popc(conv) + __popc(fresh), &total);
         const int ttot = dev_bcast_i(dev_wave_incl_scan(__popc(temps)), 63);
         const int atot = dev_bcast_i(dev_wave_incl_scan(__popc(nz)), 63);
-        for (uint32_t m = fresh; m; m &= m - 1) {
-            FrFresh f;
-            f.slot = slot;
-            f.pos = base + 16 * lane + __ffs((int)m) - 1;
-            f.flags = 0;
-            f.pair_base = 0;
-            f.pair_cnt = 0;
-            f.pad0 = 0;
-            f.key = ~0ull;
-            fq[w++] = f;
+        for (int pass = 0; pass < 2; ++pass) {
+            for (uint32_t m = pass ? fresh : conv; m; m &= m - 1) {
+                FrFresh f;
+                f.slot = slot;
+                f.pos = base + 16 * lane + __ffs((int)m) - 1;
+                f.flags = pass ? 0 : FR_FRESH_CONVERTED;
+                f.pair_base = 0;
+                f.pair_cnt = 0;
+                f.pad0 = 0;
+                f.key = ~0ull;
+                fq[w++] = f;
+            }
         }
         if (lane == 0) {
             if (ttot)
@@ -3016,8 +2974,6 @@ void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st, bool entry)
     const int dens_blocks = (L.grid_tiles + FR_ERASE_BLOCK / FR_BLOCK - 1) / (FR_ERASE_BLOCK / FR_BLOCK);
     hipLaunchKernelGGL(k_erase_mark, dim3(L.max_slots + dens_blocks), dim3(FR_ERASE_BLOCK), 0, st, L.plans, L.ctl,
                        L.slots, L.n_slots, L.tiles, L.n_tiles, L.max_slots);
-    hipLaunchKernelGGL(k_erase_apply, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles,
-                       L.fresh_q, L.counters);
     hipLaunchKernelGGL(k_accept, dim3(L.grid_tiles), b, 0, st, L.plans, L.ctl, L.tiles, L.n_tiles, L.fresh_q,
                        L.counters);
     fr_prof_begin(FR_PROF_CONNECT, st);
