@@ -22,7 +22,7 @@ Success is compared plan by plan on the plans both forms ran.  Frames are
 uploaded before the timed region (device-resident inputs).
 
 roofline: the batched form's unit of work, one launch of the captured
-iteration graph (one loop iteration of every running plan: 14 kernels, of
+iteration graph (one loop iteration of every running plan: 11 kernels, of
 which the two lazy goal-plane kernels and k_expand take about half the time).
 The timed region retires plans on wall-clock deadlines, so its work depends
 on timing (and a profiler's overhead changes it); the roofline is therefore
@@ -39,7 +39,14 @@ expansions (an upper bound; the term is < 0.2 % of the total at 256^2).
 FETCH_SIZE / WRITE_SIZE on that command, summed over the iteration's
 kernels and scaled by the same counters' reading of a copy kernel of known
 size (profiles/iteration_traffic.json).  The k_expand figures of the same
-pass are reported beside it.
+pass are reported beside it.  The path is instruction-bound more than
+HBM-bound, so `instruction_side` gives the same pass's work: footprint
+samples/s (the reference's footprint test reads every sample of every
+footprint it tests: at each cell a rollout enters and at each point of a
+goal curve whose footprints are tested), FP64 Hermite points/s of the goal
+curves, and the VALU issue rate against MI355X's peak (256 CUs x 4 SIMDs x
+2.4 GHz / 4 cycles per wave64 instruction), from the SQ_INSTS_VALU count of
+the same command (profiles/iteration_counters.json).
 
 cpu_baseline: the CPU oracle on the host cores, run before the GPU is
 touched: one process per core (`--cpu-workers`, at most the cores present)
@@ -63,6 +70,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 VALU instruction per SIMD
+# every 4 cycles (16 lanes a cycle) at 2.4 GHz
+VALU_PEAK_GWI = 256 * 4 * 2.4 / 4.0
 CELL_BYTES = 6.0     # per grid cell per plan-iteration: node type 4 B, obstacle 1 B, goal flag 1 B
 EXP_BYTES = 8.0      # per expansion: source heading + cost
 NEW_BYTES = 24.0     # per new node: parent, heading, cost, type, derive, region
@@ -299,7 +309,33 @@ def roofline_pass(L, batch, plans, args, frame_of):
             traffic = tr["hbm_bytes_per_iteration"]
     ex_avg = ex_ms / ex_n
     ex_bytes = ROLLOUT_BYTES * rollouts / ex_n
-    return {"bound": "hbm", "kernel": "iteration graph (14 kernels, one launch per loop iteration of every plan)",
+    # instruction side: the reference's footprint test reads every one of its
+    # (2 mz + 1)(2 mx + 1) samples, at each cell a rollout enters and at each
+    # point of a goal curve whose footprints are tested; FP64 Hermite points
+    # of the goal curves (centre pass and footprint pass)
+    md = plans[0].sc.min_dist
+    F = (2 * (md[1] // 2) + 1) * (2 * (md[0] // 2) + 1)
+    cells = sum(r.work[0] for r in res)
+    gpts = sum(r.work[1] for r in res)
+    fpts = sum(r.work[2] for r in res)
+    t_s = it_n * avg_ms * 1e-3
+    instr = {"footprint_samples_per_s": round((cells + fpts) * F / t_s, 1),
+             "footprint_tests_per_launch": round((cells + fpts) / it_n, 1), "samples_per_footprint": F,
+             "hermite_points_per_s": round((gpts + fpts) / t_s, 1),
+             "hermite_points_per_launch": round((gpts + fpts) / it_n, 1)}
+    cpath = os.path.join(ROOT, "profiles", "iteration_counters.json")
+    if os.path.exists(cpath):
+        with open(cpath) as fh:
+            cn = json.load(fh)
+        if cn.get("batch") == len(plans) and cn.get("roofline_iters") == args.roofline_iters:
+            wi = cn["valu_instructions_per_iteration"]
+            ach = wi / (avg_ms * 1e-3) / 1e9
+            instr["valu"] = {"bound": "valu", "achieved": round(ach, 2), "peak": VALU_PEAK_GWI,
+                             "unit": "G wave64 VALU instructions/s", "frac": round(ach / VALU_PEAK_GWI, 4),
+                             "instructions_per_launch": wi,
+                             "busiest_kernel": cn.get("busiest_kernel"),
+                             "source": "profiles/iteration_counters.json (SQ_INSTS_VALU of the same pass)"}
+    return {"bound": "hbm", "kernel": "iteration graph (11 kernels, one launch per loop iteration of every plan)",
             "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
             "avg_launch_ms": round(avg_ms, 5), "launches": it_n, "plans": len(plans),
@@ -309,6 +345,7 @@ def roofline_pass(L, batch, plans, args, frame_of):
             "k_expand": {"avg_launch_ms": round(ex_avg, 5), "launches": ex_n,
                          "algorithmic_bytes_per_launch": round(ex_bytes, 1),
                          "achieved_gbs": round(ex_bytes / (ex_avg * 1e-3) / 1e9, 4)},
+            "instruction_side": instr,
             "pass": f"{len(plans)} plans of the first timed step, batched, {args.roofline_iters} iterations, "
                     "no deadline, no optimisation rounds"}
 

@@ -27,6 +27,7 @@ SIGNATURES = {
     "fastrrt_frame_update": (c_bool, [c_void_p, fp]),
     "fastrrt_frame_read": (c_bool, [c_void_p, fp]),
     "fastrrt_frame_process_distance_to_goal": (c_bool, [c_void_p, c_int, c_int]),
+    "fastrrt_frame_process_safe_zone": (c_bool, [c_void_p, c_int, c_int]),
     "fastrrt_last_error": (c_char_p, []),
     # FastRRT handle (src/python_bind.cpp)
     "fastrrt_initialize": (c_void_p, [c_int, c_int, c_float, c_float, c_float, c_float, c_int, c_int, c_int, c_int,
@@ -130,7 +131,8 @@ class BatchHbm(ctypes.Structure):
 
 class PlanResult(ctypes.Structure):
     _fields_ = [("user_tag", c_i64), ("status", c_int), ("iterations", c_int), ("path_len", c_int),
-                ("expansions", c_int), ("submit_ms", c_double), ("done_ms", c_double), ("path_cost", c_float)]
+                ("expansions", c_int), ("submit_ms", c_double), ("done_ms", c_double), ("path_cost", c_float),
+                ("work_rollout_cells", c_i64), ("work_goal_points", c_i64), ("work_goal_foot_points", c_i64)]
 
 
 SIGNATURES.update({

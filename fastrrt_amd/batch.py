@@ -54,7 +54,8 @@ class PlanSpec:
 
 
 class Result:
-    __slots__ = ("tag", "status", "iterations", "expansions", "path", "submit_ms", "done_ms", "path_cost")
+    __slots__ = ("tag", "status", "iterations", "expansions", "path", "submit_ms", "done_ms", "path_cost",
+                 "work")
 
     @property
     def ok(self):
@@ -117,6 +118,7 @@ class Batch:
             r.path = self._buf[:3 * n].reshape(n, 3).copy() if n > 0 else np.zeros((0, 3), np.float32)
             r.submit_ms, r.done_ms = res.submit_ms, res.done_ms
             r.path_cost = float(res.path_cost)
+            r.work = (res.work_rollout_cells, res.work_goal_points, res.work_goal_foot_points)
             self._keep.pop(r.tag, None)
             out.append(r)
         _lib.check("batch_poll")

@@ -1128,6 +1128,29 @@ bool fastrrt_frame_process_distance_to_goal(void *frame, int goal_x, int goal_z)
     })
 }
 
+bool fastrrt_frame_process_safe_zone(void *frame, int min_dist_x, int min_dist_z)
+{
+    FR_TRY(false, {
+        FrFrameObj *f = static_cast<FrFrameObj *>(frame);
+        if (!f)
+            throw std::runtime_error("null frame");
+        if (min_dist_x < 0 || min_dist_z < 0)
+            throw std::runtime_error("min_dist must be >= 0");
+        uint8_t *near = nullptr;
+        FR_CHECK(hipDeviceSynchronize());
+        FR_CHECK(hipMalloc(&near, (size_t)f->W * f->H));
+        fr_launch_frame_safe_zone(f->d_xyz, f->d_costs, f->n_classes, f->W, f->H, min_dist_x / 2, min_dist_z / 2, near,
+                                  nullptr);
+        const hipError_t e = hipGetLastError();
+        const hipError_t e2 = hipDeviceSynchronize();
+        (void)hipFree(near);
+        FR_CHECK(e);
+        FR_CHECK(e2);
+        f->gen += 1;
+        return true;
+    })
+}
+
 /* ---- FastRRT handle ---- */
 
 void *fastrrt_initialize(int width, int height, float perception_width_m, float perception_height_m,

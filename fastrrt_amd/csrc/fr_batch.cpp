@@ -768,6 +768,9 @@ class FrBatch {
         f.res.status = status;
         f.res.iterations = c.iter;
         f.res.expansions = c.total_expansions;
+        f.res.work_rollout_cells = c.w_cells;
+        f.res.work_goal_points = c.w_goal_pts;
+        f.res.work_goal_foot_points = c.w_foot_pts;
         f.res.path_len = len;
         f.res.submit_ms = sl.p.t_submit;
         f.res.done_ms = now_ms() - t0_;

@@ -156,6 +156,11 @@ struct FrCtl {
     /* the last finished iteration's figures (the per-iteration fields above
      * are reset for the next iteration when one finishes) */
     int stat_expansions, stat_fresh, stat_reseeded;
+    /* work done for the plan since admission: cells its rollouts entered
+     * (each one footprint test), Hermite points of the goal-connection
+     * curves evaluated by the centre pass, and of those the footprint pass
+     * tested (each a curve's point count) */
+    int w_cells, w_goal_pts, w_foot_pts;
 };
 
 /* One queued expansion (one rollout). */

@@ -111,6 +111,10 @@ void fr_launch_region_count(const FrLaunch &L, int slot, int N, uint32_t *d_out,
 
 /* Distance-to-goal channel (pixel channel 1) of a frame (fr_frame.hip). */
 void fr_launch_frame_distance_to_goal(float *d_xyz, int W, int H, int gx, int gz, hipStream_t st);
+/* Safe-zone bit 0x100 of channel 2 for a (2 mz + 1) x (2 mx + 1) footprint;
+ * d_near: W * H bytes of scratch (fr_frame.hip). */
+void fr_launch_frame_safe_zone(float *d_xyz, const float *d_costs, int n_classes, int W, int H, int mx, int mz,
+                               uint8_t *d_near, hipStream_t st);
 
 /* Potential-field intrinsic costs of one grid (fr_apf.hip). */
 struct FrApfArgs {

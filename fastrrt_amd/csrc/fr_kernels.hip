@@ -1359,7 +1359,7 @@ __device__ bool dev_roll_flush(const FrPlanDev &P, const int *bx, const int *bz,
     return true;
 }
 
-__global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *__restrict__ plans, const FrCtl *__restrict__ ctl,
+__global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *__restrict__ plans, FrCtl *__restrict__ ctl,
                                                      FrExp *__restrict__ q, const int *__restrict__ counters)
 {
     __shared__ int s_bx[FR_BLOCK / 64][FR_ROLL_BUF];
@@ -1388,7 +1388,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *__restrict
         const FrExp e = q[i];
         const int slot = __builtin_amdgcn_readfirstlane(e.slot);
         const FrPlanDev &P = plans[slot];
-        const FrCtl &C = ctl[slot];
+        FrCtl &C = ctl[slot];
         const int src = __builtin_amdgcn_readfirstlane(e.src);
         const int W = P.W, H = P.H;
         const int sz = src / W, sx = src - sz * W;
@@ -1515,6 +1515,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *__restrict
         }
         if (lane != 0)
             continue;
+        atomicAdd(&C.w_cells, size);
         FrExp &o = q[i];
         o.end = -1;
         if (!ok || size < 1)
@@ -2088,8 +2089,8 @@ __device__ __forceinline__ bool dev_gc_lane_setup(const FrPlanDev *plans, int2 d
 /* Curvature and centre cells of the queued cells' goal curves; a rejection
  * is stored, the other cells are queued for k_gc_foot with their curves'
  * set-up (at most as many as k_gc_eval reads, so dq2 has dq's capacity). */
-__global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *__restrict__ plans, const int2 *__restrict__ dq,
-                                                      long dq_cap, int2 *__restrict__ dq2,
+__global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *__restrict__ plans, FrCtl *__restrict__ ctl,
+                                                      const int2 *__restrict__ dq, long dq_cap, int2 *__restrict__ dq2,
                                                       FrGcGeo *__restrict__ geo2, int *__restrict__ counters)
 {
     long n = counters[FR_CNT_DEMAND];
@@ -2137,6 +2138,8 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *__restric
          * bits stay clear, see dev_gc_store) */
         if (lane < cnt && !((kept >> lane) & 1ull))
             plans[d.x].gc_flag[d.y] = 0;
+        if (lane < cnt && np > 0)
+            atomicAdd(&ctl[d.x].w_goal_pts, np);
         /* one queue reservation per block */
         if (kept) {
             int at = 0;
@@ -2162,8 +2165,9 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *__restric
 /* Footprints and cost of the cells k_gc_eval kept, from the set-up it
  * handed over (the tangents are floats widened to double, so the curve is
  * the one k_gc_eval walked). */
-__global__ void __launch_bounds__(FR_BLOCK) k_gc_foot(const FrPlanDev *__restrict__ plans, const int2 *__restrict__ dq2,
-                                                      const FrGcGeo *__restrict__ geo2, const int *__restrict__ counters)
+__global__ void __launch_bounds__(FR_BLOCK) k_gc_foot(const FrPlanDev *__restrict__ plans, FrCtl *__restrict__ ctl,
+                                                      const int2 *__restrict__ dq2, const FrGcGeo *__restrict__ geo2,
+                                                      const int *__restrict__ counters)
 {
     const long n = counters[FR_CNT_DEMAND2];
     const int wpb = blockDim.x >> 6;
@@ -2209,6 +2213,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_foot(const FrPlanDev *__restric
         if (lane < cnt) {
             const FrPlanDev &P = plans[d.x];
             const int pos = d.y;
+            atomicAdd(&ctl[d.x].w_foot_pts, np);
             P.gc_flag[pos] = res < 0 ? 0 : 1;
             if (res >= 0) {
                 P.gc_heading[pos] = lh;
@@ -2980,9 +2985,10 @@ void fr_launch_iteration_post(const FrLaunch &L, hipStream_t st, bool entry)
     fr_prof_begin(FR_PROF_GOAL_CONNECTION, st);
     hipLaunchKernelGGL(k_gc_demand, dim3(L.grid_waves), b, 0, st, L.plans, L.fresh_q, L.demand, L.demand_cap,
                        L.counters);
-    hipLaunchKernelGGL(k_gc_eval, dim3(L.grid_waves), b, 0, st, L.plans, L.demand, L.demand_cap, L.demand2,
+    hipLaunchKernelGGL(k_gc_eval, dim3(L.grid_waves), b, 0, st, L.plans, L.ctl, L.demand, L.demand_cap, L.demand2,
                        L.demand2_geo, L.counters);
-    hipLaunchKernelGGL(k_gc_foot, dim3(L.grid_waves), b, 0, st, L.plans, L.demand2, L.demand2_geo, L.counters);
+    hipLaunchKernelGGL(k_gc_foot, dim3(L.grid_waves), b, 0, st, L.plans, L.ctl, L.demand2, L.demand2_geo,
+                       L.counters);
     fr_prof_end(FR_PROF_GOAL_CONNECTION, st);
     hipLaunchKernelGGL(k_connect_emit, dim3(L.grid_waves), b, 0, st, L.plans, L.ctl, L.fresh_q, L.pairs,
                        L.pair_cap, L.counters);

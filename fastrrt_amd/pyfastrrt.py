@@ -116,6 +116,15 @@ class SearchFrame:
         if not _lib.load().fastrrt_frame_process_distance_to_goal(self._ptr, goal_x, goal_z):
             _lib.check("frame_process_distance_to_goal")
 
+    def process_safe_zone(self, min_dist_x: int, min_dist_z: int) -> None:
+        """Set the safe-zone bit 0x100 of channel 2 where the footprint of
+        min_dist is free at every heading (no obstacle pixel within Chebyshev
+        distance ceil(hypot(min_dist_x // 2, min_dist_z // 2)) + 1); clear it
+        elsewhere.  libdriveless's producer of the bit is not in the
+        reference tree."""
+        if not _lib.load().fastrrt_frame_process_safe_zone(self._ptr, int(min_dist_x), int(min_dist_z)):
+            _lib.check("frame_process_safe_zone")
+
     def update(self, pixels):
         px = np.ascontiguousarray(pixels, dtype=np.float32)
         if px.shape != (self._h, self._w, 3):

@@ -45,6 +45,15 @@ bool fastrrt_frame_read(void *frame, float *xyz3);
  * squared pixel distance to (goal_x, goal_z), the curve length
  * can_connect_to_goal_using_hermite reads. */
 bool fastrrt_frame_process_distance_to_goal(void *frame, int goal_x, int goal_z);
+/* The safe-zone bit 0x100 of channel 2, which the goal connection reads at a
+ * curve's start cell (graph_direct_goal_connection.cu:124-135,
+ * isSafeZoneChecked = true at fastrrt.cpp:145) and which libdriveless (absent
+ * from the reference tree) sets: set where no obstacle pixel (class cost < 0
+ * or a class outside the table) lies within Chebyshev distance
+ * ceil(hypot(min_dist_x / 2, min_dist_z / 2)) + 1, so that the footprint is
+ * free at every heading; cleared elsewhere.  Pixels outside the frame count
+ * as free. */
+bool fastrrt_frame_process_safe_zone(void *frame, int min_dist_x, int min_dist_z);
 /* Message of the last failed call on this thread ("" if none). */
 const char *fastrrt_last_error(void);
 
@@ -283,6 +292,11 @@ typedef struct fastrrt_plan_result {
     double submit_ms, done_ms; /* batch clock */
     float path_cost;          /* cost of the path's last node (cells traversed plus their intrinsic
                                  costs, the grid's cost field; 0 when no path) */
+    /* work the search did (no counterpart in the reference): cells its
+     * rollouts entered, each one footprint test; points of the goal-connection
+     * curves evaluated (a curve's point count each); points of the curves
+     * whose footprints were tested */
+    int64_t work_rollout_cells, work_goal_points, work_goal_foot_points;
 } fastrrt_plan_result;
 
 /* A batch of up to max_plans concurrent plans whose grids total at most
