@@ -14,11 +14,10 @@ entry uses the 5/15/30 % rectangle sweep of the default benchmark, the others
 `scenes.scene_sized`), runs the same deterministic pass (a fixed number of
 loop iterations, no deadline, no optimisation rounds) once per candidate and
 times the iteration graph with HIP events (`fastrrt_profile_*`).  The search
-is coordinate descent: grid_waves, then grid_tiles, then grid_items, each
-over its candidate list with the others fixed at the best so far (a shape
-replaces the best only when it is at least 1 % faster); every candidate runs
-`repeats` times and keeps its median.  The search is coordinate descent:
-threads per block, grid_waves, grid_tiles, then grid_items.  Every pass's
+is coordinate descent: threads per block, grid_waves, grid_tiles, then
+grid_items, each over its candidate list with the others fixed at the best
+so far (a shape replaces the best only when it is at least 1 % faster);
+every candidate runs `repeats` times and keeps its median.  Every pass's
 per-plan results (status, iterations, expansions, path) must equal the first
 pass's; a difference raises, since it would mean a kernel depends on its
 launch shape.
