@@ -238,16 +238,23 @@ def validate_paths(items):
             "failed_plans": bad}
 
 
-def run_batched(b, plans, args, frame_of, max_iter=None, opt_rounds=None, deadline=None):
+def plan_spec(pl, args, frame_of, max_iter=None, opt_rounds=None, deadline=None):
     from fastrrt_amd.batch import PlanSpec
+    sc = pl.sc
+    return PlanSpec(frame_of(sc), sc.start, sc.goal, min_dist=sc.min_dist, ego_lb=sc.ego_lb, ego_ub=sc.ego_ub,
+                    max_iterations=args.max_iter if max_iter is None else max_iter,
+                    optimize_rounds=args.opt_rounds if opt_rounds is None else opt_rounds,
+                    deadline_ms=args.budget_ms if deadline is None else deadline, seed=SEED,
+                    stream=pl.stream, tag=pl.tag)
+
+
+def run_batched(b, plans, args, frame_of, max_iter=None, opt_rounds=None, deadline=None):
+    """The requests are built, then submitted together (each plan's deadline
+    counts from its own submission); the wall time covers both."""
     t0 = time.perf_counter()
-    for pl in plans:
-        sc = pl.sc
-        b.submit(PlanSpec(frame_of(sc), sc.start, sc.goal, min_dist=sc.min_dist, ego_lb=sc.ego_lb, ego_ub=sc.ego_ub,
-                          max_iterations=args.max_iter if max_iter is None else max_iter,
-                          optimize_rounds=args.opt_rounds if opt_rounds is None else opt_rounds,
-                          deadline_ms=args.budget_ms if deadline is None else deadline, seed=SEED,
-                          stream=pl.stream, tag=pl.tag))
+    specs = [plan_spec(pl, args, frame_of, max_iter, opt_rounds, deadline) for pl in plans]
+    for spec in specs:
+        b.submit(spec)
     res = []
     while b.step(args.iters_per_poll) > 0:
         res += b.poll()
@@ -261,19 +268,15 @@ def run_served(b, plans, args, frame_of, capacity):
     least --admit-group slots are free (or the queue's last plans remain),
     the next plans are submitted.  Each plan's deadline counts from its own
     submission."""
-    from fastrrt_amd.batch import PlanSpec
     t0 = time.perf_counter()
-    queue = list(plans)
+    queue = [plan_spec(pl, args, frame_of) for pl in plans]
+    queue.reverse()
     res, inflight = [], 0
     group = max(1, min(args.admit_group, capacity))
     while queue or inflight:
         if queue and (capacity - inflight >= group or len(queue) <= capacity - inflight):
             while queue and inflight < capacity:
-                pl = queue.pop(0)
-                sc = pl.sc
-                b.submit(PlanSpec(frame_of(sc), sc.start, sc.goal, min_dist=sc.min_dist, ego_lb=sc.ego_lb,
-                                  ego_ub=sc.ego_ub, max_iterations=args.max_iter, optimize_rounds=args.opt_rounds,
-                                  deadline_ms=args.budget_ms, seed=SEED, stream=pl.stream, tag=pl.tag))
+                b.submit(queue.pop())
                 inflight += 1
         b.step(args.iters_per_poll)
         got = b.poll()

@@ -143,9 +143,6 @@ class FrBatch {
             FR_CHECK(hipEventCreateWithFlags(&snap_ev_[b], hipEventDisableTiming));
         }
         FR_CHECK(hipHostMalloc((void **)&h_fin_list_, sizeof(int) * 2 * fin_max_, hipHostMallocDefault));
-        FR_CHECK(hipHostMalloc((void **)&h_idle_, sizeof(FrCtl), hipHostMallocDefault));
-        std::memset(h_idle_, 0, sizeof(FrCtl));
-        h_idle_->status = FR_IDLE;
         /* finished plans' records, written by the device (k_fin_pack) */
         FR_CHECK(hipHostMalloc((void **)&h_pack_, (size_t)fin_max_ * kPackStride, hipHostMallocMapped));
         FR_CHECK(hipHostGetDevicePointer((void **)&d_pack_, h_pack_, 0));
@@ -176,8 +173,6 @@ class FrBatch {
         }
         if (h_fin_list_)
             (void)hipHostFree(h_fin_list_);
-        if (h_idle_)
-            (void)hipHostFree(h_idle_);
         if (h_pack_)
             (void)hipHostFree(h_pack_);
         if (d_list_)
@@ -191,7 +186,6 @@ class FrBatch {
         h_snap_[0] = h_snap_[1] = nullptr;
         snap_ev_[0] = snap_ev_[1] = nullptr;
         h_fin_list_ = nullptr;
-        h_idle_ = nullptr;
         h_pack_ = d_pack_ = nullptr;
         d_list_ = fin_list_ = nullptr;
     }
@@ -497,7 +491,7 @@ class FrBatch {
         eng_.upload_plans();
         eng_.upload_ints(d_list_, admitted.data(), (int)admitted.size());
         fr_launch_slab_reset(eng_.launch(), d_list_, (int)admitted.size(), eng_.stream());
-        eng_.setup(admitted, FR_GOAL_LAZY);
+        eng_.setup(admitted, FR_GOAL_LAZY, true);
         for (int sl : admitted)
             slots_[sl].admit_seq = snap_seq_;
         fr_launch_batch_start(eng_.launch(), d_list_, (int)admitted.size(), eng_.stream());
@@ -614,7 +608,11 @@ class FrBatch {
                 failed.push_back(i); /* with a margin for a slower next iteration */
         }
         for (int sl : failed)
-            retire(sl, FR_FAILED, nullptr, 0, snap && seq >= slots_[sl].admit_seq ? snap[sl] : none);
+            retire(sl, FR_FAILED, nullptr, 0, snap && seq >= slots_[sl].admit_seq ? snap[sl] : none, false);
+        if (!failed.empty()) {
+            fr_launch_ctl_idle(eng_.launch().ctl, failed.data(), (int)failed.size(), eng_.stream());
+            FR_CHECK(hipGetLastError());
+        }
         if (!found.empty() && !fin_busy_)
             launch_finalize(found);
         if (changed || !failed.empty())
@@ -778,8 +776,7 @@ class FrBatch {
             f.path.assign(path, path + 3 * (size_t)len);
         done_.push_back(std::move(f));
         if (reset_ctl)
-            FR_CHECK(hipMemcpyAsync(eng_.launch().ctl + s, h_idle_, sizeof(FrCtl), hipMemcpyHostToDevice,
-                                    eng_.stream()));
+            fr_launch_ctl_idle(eng_.launch().ctl, &s, 1, eng_.stream());
         give_back(std::move(sl.slab));
         cells_in_use_ -= sl.cells;
         sl.busy = false;
@@ -816,7 +813,6 @@ class FrBatch {
     std::vector<int> fin_slots_;
     int *fin_list_ = nullptr;
     int *h_fin_list_ = nullptr;
-    FrCtl *h_idle_ = nullptr;
     static constexpr int kPackPath = 2048;                         /* waypoints per pinned record */
     static constexpr long kPackStride = FR_PACK_PATH_OFF + 16L * kPackPath;
     uint8_t *h_pack_ = nullptr, *d_pack_ = nullptr;

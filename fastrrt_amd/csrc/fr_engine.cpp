@@ -630,17 +630,16 @@ void FrEngine::set_active(const std::vector<int> &slots)
     FR_CHECK(hipEventRecord(act_ev_[k], st_));
 }
 
-void FrEngine::setup(const std::vector<int> &slots, int goal_mode)
+void FrEngine::setup(const std::vector<int> &slots, int goal_mode, bool zeroed)
 {
     if (slots.empty())
         return;
-    int max_N = 0, max_PH = 0;
-    long max_PWH = 0;
+    int max_N = 0, max_PW = 0, max_PH = 0;
     for (int s : slots) {
         const FrPlanDev &d = h_plans_[s];
         max_N = std::max(max_N, d.N);
+        max_PW = std::max(max_PW, d.PW);
         max_PH = std::max(max_PH, d.PH);
-        max_PWH = std::max(max_PWH, (long)d.PW * d.PH);
     }
     upload_ints(d_setup_slots_, slots.data(), (int)slots.size());
     for (int s : slots) {
@@ -649,7 +648,7 @@ void FrEngine::setup(const std::vector<int> &slots, int goal_mode)
             upload_plan(s);
         }
     }
-    fr_launch_setup(L_, d_setup_slots_, (int)slots.size(), max_N, max_PH, max_PWH, goal_mode, st_);
+    fr_launch_setup(L_, d_setup_slots_, (int)slots.size(), max_N, max_PW, max_PH, goal_mode, zeroed, st_);
     FR_CHECK(hipGetLastError());
 }
 

@@ -59,8 +59,10 @@ struct FrBestNodeArgs {
 };
 
 /* kernel launchers (fr_kernels.hip) */
-void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, int max_PH, long max_PWH,
-                     int goal_mode, hipStream_t st);
+/* zeroed: the plans' slabs were just reset to zero (k_slab_reset), so the
+ * goal planes' reset is skipped */
+void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, int max_PW, int max_PH,
+                     int goal_mode, bool zeroed, hipStream_t st);
 /* evaluate the goal-connection planes of every cell not yet evaluated */
 void fr_launch_goal_complete(const FrLaunch &L, const int *d_slots, int n, int max_N, hipStream_t st);
 /* entry: run k_begin first (the FastRRT handle, whose host may edit the plan
@@ -92,6 +94,9 @@ void fr_launch_opt_round(const FrLaunch &L, const int *d_slots, int n, int max_p
 void fr_launch_goal_scan_multi(const FrLaunch &L, const int *d_slots, int n, int max_N, int gate, hipStream_t st);
 void fr_launch_batch_start(const FrLaunch &L, const int *d_slots, int n, hipStream_t st);
 void fr_launch_slab_reset(const FrLaunch &L, const int *d_slots, int n, hipStream_t st);
+/* control blocks of retired plans back to IDLE, one launch for any number of
+ * slots (the slot list travels in the kernel arguments) */
+void fr_launch_ctl_idle(FrCtl *ctl, const int *slots, int n, hipStream_t st);
 void fr_launch_graph_add(const FrLaunch &L, int slot, int x, int z, float h, int px, int pz, float cost, int type,
                          int count_parent, hipStream_t st);
 void fr_launch_set_collision(const FrLaunch &L, int slot, int x, int z, int npx, int npz, float h, float cost,
@@ -283,7 +288,8 @@ class FrEngine {
     int n_active() const { return (int)active_.size(); }
     const int *d_active() const { return L_.slots; }
 
-    void setup(const std::vector<int> &slots, int goal_mode); /* per-plan precompute */
+    /* per-plan precompute; zeroed: the slabs were just reset to zero */
+    void setup(const std::vector<int> &slots, int goal_mode, bool zeroed = false);
     void goal_complete(int slot);                  /* finish a lazy goal-connection pass */
     void iteration();                              /* one loop() for every active plan */
     void sync();

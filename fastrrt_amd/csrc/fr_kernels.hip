@@ -26,6 +26,8 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fr_device.h"
 #include "fr_engine.h"
 
@@ -932,19 +934,45 @@ __device__ int block_exclusive_scan(int v, int *out_prefix)
  * pixel whose class cost is negative (path_feasible.cu:472-475); pixels in
  * the ego box are skipped by the reference (:469-470) and count as free; a
  * class id outside the table counts as an obstacle. */
+__device__ __forceinline__ void dev_prep_px(const FrPlanDev &P, int pos, float c0, float c2, uint8_t *tv, uint8_t *mk)
+{
+    const int z = pos / P.W, x = pos - z * P.W;
+    const int trav = fr_trunc_f(c2);
+    *tv = (uint8_t)((trav & 1) | ((trav & 0x100) ? 2 : 0));
+    const int cls = fr_trunc_f(c0);
+    int obst = (cls < 0 || cls >= P.n_classes) ? 1 : (P.class_costs[cls] < 0);
+    if (x >= P.lbx && x <= P.ubx && z >= P.ubz && z <= P.lbz)
+        obst = 0;
+    *mk = (uint8_t)obst;
+}
+
+/* Four pixels per thread: their 48 bytes of frame in three 16-byte loads,
+ * both byte planes written a 32-bit word at a time (the planes and the
+ * frame start 256-byte aligned). */
 __global__ void __launch_bounds__(FR_BLOCK) k_setup_prep(const FrPlanDev *plans, const int *slots)
 {
     const FrPlanDev &P = plans[slots[blockIdx.y]];
-    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < P.N; pos += gridDim.x * blockDim.x) {
-        const int z = pos / P.W, x = pos - z * P.W;
+    const int quads = P.N >> 2;
+    const float4 *F = (const float4 *)P.frame;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
+        const float4 a = F[3 * q], b = F[3 * q + 1], c = F[3 * q + 2];
+        /* pixels: (a.x a.y a.z) (a.w b.x b.y) (b.z b.w c.x) (c.y c.z c.w) */
+        uint8_t tv[4], mk[4];
+        dev_prep_px(P, 4 * q, a.x, a.z, &tv[0], &mk[0]);
+        dev_prep_px(P, 4 * q + 1, a.w, b.y, &tv[1], &mk[1]);
+        dev_prep_px(P, 4 * q + 2, b.z, c.x, &tv[2], &mk[2]);
+        dev_prep_px(P, 4 * q + 3, c.y, c.w, &tv[3], &mk[3]);
+        ((uint32_t *)P.trav)[q] = (uint32_t)tv[0] | ((uint32_t)tv[1] << 8) | ((uint32_t)tv[2] << 16) |
+                                  ((uint32_t)tv[3] << 24);
+        ((uint32_t *)P.mark)[q] = (uint32_t)mk[0] | ((uint32_t)mk[1] << 8) | ((uint32_t)mk[2] << 16) |
+                                  ((uint32_t)mk[3] << 24);
+    }
+    /* the last N mod 4 pixels */
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int pos = 4 * quads + t;
+    if (t < (P.N & 3)) {
         const float *f = P.frame + 3 * (long)pos;
-        const int trav = fr_trunc_f(f[2]);
-        P.trav[pos] = (uint8_t)((trav & 1) | ((trav & 0x100) ? 2 : 0));
-        const int cls = fr_trunc_f(f[0]);
-        int obst = (cls < 0 || cls >= P.n_classes) ? 1 : (P.class_costs[cls] < 0);
-        if (x >= P.lbx && x <= P.ubx && z >= P.ubz && z <= P.lbz)
-            obst = 0;
-        P.mark[pos] = (uint8_t)obst;
+        dev_prep_px(P, pos, f[0], f[2], &P.trav[pos], &P.mark[pos]);
     }
 }
 
@@ -1010,30 +1038,51 @@ __global__ void __launch_bounds__(FR_BLOCK) k_setup_rowdist(const FrPlanDev *pla
 }
 
 /* Column pass: D(x, z) = min_dz max(|dz|, rowdist(x, z + dz)), capped.  A
- * dz at or beyond the running minimum cannot lower it; rows are read eight
- * offsets at a time so the loads overlap. */
-__global__ void __launch_bounds__(FR_BLOCK) k_setup_coldist(const FrPlanDev *plans, const int *slots)
+ * dz at or beyond the running minimum cannot lower it.  One block per 64 x
+ * 64 tile of the padded plane: the tile's columns, cap rows either side
+ * included, are staged in LDS (rows beyond the plane read 255), and each
+ * lane scans its column there for 16 rows of the tile. */
+#define FR_COL_TILE 64
+__global__ void __launch_bounds__(256) k_setup_coldist(const FrPlanDev *plans, const int *slots)
 {
+    __shared__ uint8_t s_t[(FR_COL_TILE + 2 * FR_DIST_CAP) * FR_COL_TILE];
     const FrPlanDev &P = plans[slots[blockIdx.y]];
-    const long total = (long)P.PW * P.PH;
+    const int cap = dev_dist_cap(P);
+    const int ntx = (P.PW + FR_COL_TILE - 1) / FR_COL_TILE;
+    const int nty = (P.PH + FR_COL_TILE - 1) / FR_COL_TILE;
     const uint8_t *T = P.dist_tmp;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-        const int zp = (int)(i / P.PW), xp = (int)(i - (long)zp * P.PW);
-        int best = T[i];
-        for (int dz0 = 1; dz0 < best; dz0 += 8) {
-            int cand = best;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int dz = dz0 + k;
-                const int up = zp - dz >= 0 ? T[(long)(zp - dz) * P.PW + xp] : 255;
-                const int dn = zp + dz < P.PH ? T[(long)(zp + dz) * P.PW + xp] : 255;
-                int v = up < dn ? up : dn;
-                v = v > dz ? v : dz;
-                cand = v < cand ? v : cand;
-            }
-            best = cand;
+    const int c = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
+        const int ty = tile / ntx, tx = tile - ty * ntx;
+        const int x0 = tx * FR_COL_TILE, z0 = ty * FR_COL_TILE;
+        const int rows = FR_COL_TILE + 2 * cap; /* staged rows z0 - cap .. z0 + 63 + cap */
+        const int xp = x0 + c;
+        __syncthreads(); /* the previous tile's reads are done */
+        for (int r = wv; r < rows; r += 4) {
+            const int zp = z0 - cap + r;
+            uint8_t v = 255;
+            if (zp >= 0 && zp < P.PH && xp < P.PW)
+                v = T[(long)zp * P.PW + xp];
+            s_t[r * FR_COL_TILE + c] = v;
         }
-        P.dist[fr_dist_at(P.TW, xp, zp)] = (uint8_t)best;
+        __syncthreads();
+        if (xp < P.PW) {
+            for (int k = 0; k < FR_COL_TILE / 4; ++k) {
+                const int r = wv * (FR_COL_TILE / 4) + k; /* row within the tile */
+                const int zp = z0 + r;
+                if (zp >= P.PH)
+                    break;
+                const uint8_t *col = s_t + (r + cap) * FR_COL_TILE + c;
+                int best = col[0];
+                for (int dz = 1; dz < best; ++dz) {
+                    const int up = col[-dz * FR_COL_TILE], dn = col[dz * FR_COL_TILE];
+                    int v = up < dn ? up : dn;
+                    v = v > dz ? v : dz;
+                    best = v < best ? v : best;
+                }
+                P.dist[fr_dist_at(P.TW, xp, zp)] = (uint8_t)best;
+            }
+        }
     }
 }
 
@@ -2924,21 +2973,26 @@ static int fr_grid_for(long work, int per_block, int cap)
     return (int)b;
 }
 
-void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, int max_PH, long max_PWH,
-                     int goal_mode, hipStream_t st)
+void fr_launch_setup(const FrLaunch &L, const int *d_slots, int n, int max_N, int max_PW, int max_PH,
+                     int goal_mode, bool zeroed, hipStream_t st)
 {
     if (n <= 0)
         return;
     dim3 b(FR_BLOCK);
-    hipLaunchKernelGGL(k_setup_prep, dim3(fr_grid_for(max_N, FR_BLOCK, 1024), n), b, 0, st, L.plans, d_slots);
+    hipLaunchKernelGGL(k_setup_prep, dim3(fr_grid_for((max_N + 3) / 4, FR_BLOCK, 1024), n), b, 0, st, L.plans,
+                       d_slots);
     hipLaunchKernelGGL(k_setup_rowdist, dim3(fr_grid_for(max_PH, FR_BLOCK / 64, 1024), n), b, 0, st, L.plans,
                        d_slots);
-    hipLaunchKernelGGL(k_setup_coldist, dim3(fr_grid_for(max_PWH, FR_BLOCK, 1024), n), b, 0, st, L.plans,
+    /* one block per 64 x 64 tile of the largest padded plane */
+    const long tiles = ((long)max_PW + FR_COL_TILE - 1) / FR_COL_TILE * (((long)max_PH + FR_COL_TILE - 1) / FR_COL_TILE);
+    hipLaunchKernelGGL(k_setup_coldist, dim3((unsigned)std::min(tiles, 4096L), n), dim3(256), 0, st, L.plans,
                        d_slots);
-    (void)max_PH;
     if (goal_mode == FR_GOAL_NONE)
         return;
-    hipLaunchKernelGGL(k_setup_gc_reset, dim3(fr_grid_for(max_N, FR_BLOCK, 1024), n), b, 0, st, L.plans, d_slots);
+    /* a slab reset to zero already holds the planes k_setup_gc_reset writes */
+    if (!zeroed)
+        hipLaunchKernelGGL(k_setup_gc_reset, dim3(fr_grid_for(max_N, FR_BLOCK, 1024), n), b, 0, st, L.plans,
+                           d_slots);
     if (goal_mode == FR_GOAL_EAGER) {
         fr_prof_begin(FR_PROF_GOAL_CONNECTION, st);
         hipLaunchKernelGGL(k_setup_goal_connection, dim3(fr_grid_for(max_N, FR_BLOCK / 64, 16384), n), b, 0, st,
@@ -3193,6 +3247,36 @@ void fr_launch_slab_reset(const FrLaunch &L, const int *d_slots, int n, hipStrea
 void fr_launch_batch_start(const FrLaunch &L, const int *d_slots, int n, hipStream_t st)
 {
     hipLaunchKernelGGL(k_batch_start, dim3(fr_grid_for(n, 64, 65535)), dim3(64), 0, st, L.plans, L.ctl, d_slots, n);
+}
+
+/* Retired plans' control blocks back to the IDLE state of an empty slot.
+ * The slots come in the kernel arguments, so retiring many plans at once
+ * costs one launch rather than one host-to-device copy per plan. */
+#define FR_IDLE_ARGS 240
+struct FrSlotArgs {
+    int n;
+    int s[FR_IDLE_ARGS];
+};
+
+__global__ void __launch_bounds__(256) k_ctl_idle(FrCtl *ctl, FrSlotArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n)
+        return;
+    FrCtl c = {};
+    c.status = FR_IDLE;
+    ctl[a.s[i]] = c;
+}
+
+void fr_launch_ctl_idle(FrCtl *ctl, const int *slots, int n, hipStream_t st)
+{
+    for (int base = 0; base < n; base += FR_IDLE_ARGS) {
+        FrSlotArgs a;
+        a.n = std::min(FR_IDLE_ARGS, n - base);
+        for (int k = 0; k < a.n; ++k)
+            a.s[k] = slots[base + k];
+        hipLaunchKernelGGL(k_ctl_idle, dim3(1), dim3(256), 0, st, ctl, a);
+    }
 }
 
 /* ------------------------------------------------------------------------ */
