@@ -253,6 +253,27 @@ __device__ __forceinline__ float dev_bcast_f(float v, int lane)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+/* Lane l gets lane l-1's v; lane 0 gets `first` (DPP wave_shr:1). */
+__device__ __forceinline__ float dev_wave_shr1(float v, float first)
+{
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(first), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+
+/* The serial float sum a_l = (((a0 + d_0) + d_1) + ... + d_l) of the lanes'
+ * increments, every a_l rounded as the serial loop rounds it: each round
+ * adds lane l's increment to lane l-1's partial sum (lane 0 to a0).  Lane l
+ * holds its exact sum from round l + 1 on, and it never changes after, since
+ * lane l - 1's does not; 64 rounds of two instructions each. */
+__device__ __forceinline__ float dev_wave_serial_sum(float a0, float d)
+{
+    float v = a0;
+#pragma unroll
+    for (int t = 0; t < 64; ++t)
+        v = dev_wave_shr1(v, a0) + d;
+    return v;
+}
+
 /* inclusive prefix sum over the wave */
 __device__ __forceinline__ int dev_wave_incl_scan(int v)
 {
@@ -746,6 +767,30 @@ __device__ __forceinline__ bool dev_curve_gentle(const FrHermite &hm, float max_
     return L >= 0.5 && qmax <= 0.75 * (double)max_curv * (L * L * L);
 }
 
+/* Exact early rejection in pass 0 of a set-up curve of more than 64 points,
+ * from 64 of its points spread over it (lane l takes point l * ceil(n / 64)).
+ * Pass 0 rejects a curve when an in-grid point's centre pixel is an obstacle
+ * pixel not feasible at every heading, and every in-grid point has the cell
+ * of a point that pass 0 tests (the first of its run of equal cells), so a
+ * sample there is conclusive.  Most rejected curves are then rejected in one
+ * chunk instead of the two or three that the walk takes to reach and test
+ * its first obstacle.  (The same look at the footprints of pass 1 cost more
+ * than it saved: the curves that reach pass 1 mostly pass it.) */
+#ifndef FR_SPARSE
+#define FR_SPARSE 1
+#endif
+__device__ __forceinline__ bool dev_curve_sparse_reject(const FrPlanDev &P, const FrHermite &hm, int num_points,
+                                                        bool host_nan)
+{
+    const int i = dev_lane() * ((num_points + 63) >> 6);
+    int cx = 0, cz = 0;
+    double t = 0;
+    const bool in = i < num_points && dev_hermite_point(P, hm, i, host_nan, &cx, &cz, &t);
+    const bool hit =
+        in && !(P.trav[cz * P.W + cx] & 1) && P.dist[fr_dist_at(P.TW, cx + P.pad, cz + P.pad)] == 0;
+    return __ballot(hit) != 0ull;
+}
+
 /* Pass 0, chunk by chunk of 64 points: each chunk's geometry and curvature,
  * then its centre cells' loads; the previous chunk's bytes are tested while
  * these are in flight.  Most rejected curves meet their first obstacle pixel
@@ -756,6 +801,8 @@ __device__ __forceinline__ bool dev_wave_curve_pass0(const FrPlanDev &P, const F
 {
     last_x = -1;
     last_z = -1;
+    if (FR_SPARSE && !skip_fp && num_points > 64 && dev_curve_sparse_reject(P, hm, num_points, host_nan))
+        return false;
     int tv_prev = 1, dv_prev = 1;
     for (int b = 0; b < num_points; b += 64) {
         double t;
@@ -1472,23 +1519,19 @@ __global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *__restrict
         int nb = 0; /* cells buffered, footprints not yet tested */
         for (int k0 = 0;; k0 += 64) {
             const int k = k0 + lane;
-            /* heading before (hk) and after (hk1) this lane's step */
-            float hk = h, hh = h;
-            for (int j = 0; j < 64; ++j) {
-                hk = j == lane ? hh : hk;
-                hh = hh + hif;
-            }
-            const float hk1 = hk + hif;
+            /* heading after (hk1) and before (hk) this lane's step: the
+             * serial chain h + hif + hif + ..., each lane's value rounded as
+             * the loop of kinematics.cu rounds it */
+            const float hk1 = dev_wave_serial_sum(h, hif);
+            const float hk = dev_wave_shr1(hk1, h);
+            const float hh = dev_bcast_f(hk1, 63);
             float sn, cs;
             fr_sincosf(hk + beta, &sn, &cs);
             const float dx = ds * cs, dy = ds * sn;
-            float xk = xf, yk = yf, xx = xf, yy = yf;
-            for (int j = 0; j < 64; ++j) {
-                xx = xx + dev_bcast_f(dx, j);
-                yy = yy + dev_bcast_f(dy, j);
-                xk = j == lane ? xx : xk;
-                yk = j == lane ? yy : yk;
-            }
+            /* position after this lane's step, the same serial sums */
+            const float xk = dev_wave_serial_sum(xf, dx);
+            const float yk = dev_wave_serial_sum(yf, dy);
+            const float xx = dev_bcast_f(xk, 63), yy = dev_bcast_f(yk, 63);
             const int cx = fr_trunc_d(ogx + (double)yk);
             const int cz = fr_trunc_d(ogy - (double)xk);
             int px = __shfl_up(cx, 1, 64), pz = __shfl_up(cz, 1, 64);
