@@ -138,11 +138,16 @@ class FrBatch {
     void allocate(int max_plans)
     {
         for (int b = 0; b < 2; ++b) {
-            FR_CHECK(hipHostMalloc((void **)&h_snap_[b], sizeof(FrCtl) * max_plans, hipHostMallocDefault));
+            /* written by a copy kernel on the stream (fr_launch_copy) */
+            FR_CHECK(hipHostMalloc((void **)&h_snap_[b], sizeof(FrCtl) * max_plans,
+                                   hipHostMallocMapped | hipHostMallocCoherent));
+            FR_CHECK(hipHostGetDevicePointer((void **)&d_snap_[b], h_snap_[b], 0));
             std::memset(h_snap_[b], 0, sizeof(FrCtl) * max_plans);
             FR_CHECK(hipEventCreateWithFlags(&snap_ev_[b], hipEventDisableTiming));
         }
-        FR_CHECK(hipHostMalloc((void **)&h_fin_list_, sizeof(int) * 2 * fin_max_, hipHostMallocDefault));
+        FR_CHECK(hipHostMalloc((void **)&h_fin_list_, sizeof(int) * 2 * fin_max_,
+                               hipHostMallocMapped | hipHostMallocCoherent));
+        FR_CHECK(hipHostGetDevicePointer((void **)&d_fin_list_, h_fin_list_, 0));
         /* finished plans' records, written by the device (k_fin_pack) */
         FR_CHECK(hipHostMalloc((void **)&h_pack_, (size_t)fin_max_ * kPackStride, hipHostMallocMapped));
         FR_CHECK(hipHostGetDevicePointer((void **)&d_pack_, h_pack_, 0));
@@ -184,8 +189,10 @@ class FrBatch {
         fin_ev_ = nullptr;
         st2_ = nullptr;
         h_snap_[0] = h_snap_[1] = nullptr;
+        d_snap_[0] = d_snap_[1] = nullptr;
         snap_ev_[0] = snap_ev_[1] = nullptr;
         h_fin_list_ = nullptr;
+        d_fin_list_ = nullptr;
         h_pack_ = d_pack_ = nullptr;
         d_list_ = fin_list_ = nullptr;
     }
@@ -274,8 +281,8 @@ class FrBatch {
                 for (int k = 0; k < iterations; ++k)
                     launch_iteration();
                 const int b = (int)(snap_seq_ & 1);
-                FR_CHECK(hipMemcpyAsync(h_snap_[b], eng_.launch().ctl, sizeof(FrCtl) * slots_.size(),
-                                        hipMemcpyDeviceToHost, eng_.stream()));
+                fr_launch_copy(d_snap_[b], eng_.launch().ctl, sizeof(FrCtl) * slots_.size(), eng_.stream());
+                FR_CHECK(hipGetLastError());
                 FR_CHECK(hipEventRecord(snap_ev_[b], eng_.stream()));
                 const long prev = snap_seq_ - 1;
                 ++snap_seq_;
@@ -672,7 +679,10 @@ class FrBatch {
                 fin_used_rounds_ = std::max(fin_used_rounds_, h_fin_list_[fin_max_ + q]);
         }
         fin_t_launch_ = t;
-        FR_CHECK(hipMemcpyAsync(fin_list_, h_fin_list_, sizeof(int) * 2 * fin_max_, hipMemcpyHostToDevice, st2_));
+        /* the list is read before the graph runs; the next launch waits for
+         * this finalisation to end, so the host buffer is free then */
+        fr_launch_copy(fin_list_, d_fin_list_, sizeof(int) * 2 * fin_max_, st2_);
+        FR_CHECK(hipGetLastError());
         FR_CHECK(hipGraphLaunch(fin_exec_, st2_));
         FR_CHECK(hipEventRecord(fin_ev_, st2_));
         for (int s : fin_slots_)
@@ -794,6 +804,7 @@ class FrBatch {
     std::deque<Finished> done_;
     std::map<std::tuple<int, int, int, int>, std::vector<std::unique_ptr<FrSlab>>> pool_;
     FrCtl *h_snap_[2] = {nullptr, nullptr};  /* pinned control-block copies, alternating */
+    FrCtl *d_snap_[2] = {nullptr, nullptr};  /* their device addresses */
     hipEvent_t snap_ev_[2] = {nullptr, nullptr};
     long snap_seq_ = 0;                      /* copies queued */
     long snap_done_ = 0;                     /* copies acted on */
@@ -812,7 +823,7 @@ class FrBatch {
     int finalizing_ = 0;
     std::vector<int> fin_slots_;
     int *fin_list_ = nullptr;
-    int *h_fin_list_ = nullptr;
+    int *h_fin_list_ = nullptr, *d_fin_list_ = nullptr;
     static constexpr int kPackPath = 2048;                         /* waypoints per pinned record */
     static constexpr long kPackStride = FR_PACK_PATH_OFF + 16L * kPackPath;
     uint8_t *h_pack_ = nullptr, *d_pack_ = nullptr;

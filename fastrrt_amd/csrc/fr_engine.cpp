@@ -349,10 +349,13 @@ void FrEngine::allocate()
         own_stream_ = true;
     }
     h_plans_.assign(max_slots, FrPlanDev{});
-    FR_CHECK(hipHostMalloc((void **)&h_plan_stage_, sizeof(FrPlanDev) * max_slots * FR_PLAN_RING,
-                           hipHostMallocDefault));
-    FR_CHECK(hipHostMalloc((void **)&h_int_stage_, sizeof(int) * (max_slots + 1) * FR_STAGE_RING,
-                           hipHostMallocDefault));
+    /* staging rings in mapped, coherent pinned memory: the iteration's
+     * uploads are read by a copy kernel on the stream (fr_launch_copy) */
+    const unsigned mapped = hipHostMallocMapped | hipHostMallocCoherent;
+    FR_CHECK(hipHostMalloc((void **)&h_plan_stage_, sizeof(FrPlanDev) * max_slots * FR_PLAN_RING, mapped));
+    FR_CHECK(hipHostGetDevicePointer((void **)&d_plan_stage_, h_plan_stage_, 0));
+    FR_CHECK(hipHostMalloc((void **)&h_int_stage_, sizeof(int) * (max_slots + 1) * FR_STAGE_RING, mapped));
+    FR_CHECK(hipHostGetDevicePointer((void **)&d_int_stage_, h_int_stage_, 0));
     for (int k = 0; k < FR_PLAN_RING; ++k)
         FR_CHECK(hipEventCreateWithFlags(&plan_ev_[k], hipEventDisableTiming));
     for (auto &e : int_ev_)
@@ -364,7 +367,8 @@ void FrEngine::allocate()
     act_tiles_off_ = z.act_tiles_off;
     act_bytes_ = z.act_bytes;
     FR_CHECK(hipMalloc(&d_act_, act_bytes_));
-    FR_CHECK(hipHostMalloc((void **)&h_act_stage_, act_bytes_ * FR_ACT_RING, hipHostMallocDefault));
+    FR_CHECK(hipHostMalloc((void **)&h_act_stage_, act_bytes_ * FR_ACT_RING, mapped));
+    FR_CHECK(hipHostGetDevicePointer((void **)&d_act_stage_, h_act_stage_, 0));
     for (auto &e : act_ev_)
         FR_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     L_.n_tiles = (int *)d_act_;
@@ -553,7 +557,8 @@ void FrEngine::upload_plans()
     FrPlanDev *stage = h_plan_stage_ + (size_t)k * max_slots_;
     FR_CHECK(hipEventSynchronize(plan_ev_[k])); /* that ring entry's copy has read it */
     std::memcpy(stage, h_plans_.data(), sizeof(FrPlanDev) * max_slots_);
-    FR_CHECK(hipMemcpyAsync(L_.plans, stage, sizeof(FrPlanDev) * max_slots_, hipMemcpyHostToDevice, st_));
+    fr_launch_copy(L_.plans, d_plan_stage_ + (size_t)k * max_slots_, sizeof(FrPlanDev) * max_slots_, st_);
+    FR_CHECK(hipGetLastError());
     FR_CHECK(hipEventRecord(plan_ev_[k], st_));
 }
 
@@ -566,7 +571,8 @@ void FrEngine::upload_ints(int *d_dst, const int *src, int n)
     int *stage = h_int_stage_ + (size_t)k * (max_slots_ + 1);
     FR_CHECK(hipEventSynchronize(int_ev_[k])); /* that ring entry's copy has read it */
     std::memcpy(stage, src, sizeof(int) * n);
-    FR_CHECK(hipMemcpyAsync(d_dst, stage, sizeof(int) * n, hipMemcpyHostToDevice, st_));
+    fr_launch_copy(d_dst, d_int_stage_ + (size_t)k * (max_slots_ + 1), sizeof(int) * n, st_);
+    FR_CHECK(hipGetLastError());
     FR_CHECK(hipEventRecord(int_ev_[k], st_));
 }
 
@@ -626,7 +632,8 @@ void FrEngine::set_active(const std::vector<int> &slots)
     if (nt)
         std::memcpy(stage + act_tiles_off_, h_tiles_.data(), sizeof(int2) * nt);
     const size_t bytes = nt ? act_tiles_off_ + sizeof(int2) * nt : act_tiles_off_;
-    FR_CHECK(hipMemcpyAsync(d_act_, stage, bytes, hipMemcpyHostToDevice, st_));
+    fr_launch_copy(d_act_, d_act_stage_ + (size_t)k * act_bytes_, bytes, st_);
+    FR_CHECK(hipGetLastError());
     FR_CHECK(hipEventRecord(act_ev_[k], st_));
 }
 

@@ -97,6 +97,11 @@ void fr_launch_slab_reset(const FrLaunch &L, const int *d_slots, int n, hipStrea
 /* control blocks of retired plans back to IDLE, one launch for any number of
  * slots (the slot list travels in the kernel arguments) */
 void fr_launch_ctl_idle(FrCtl *ctl, const int *slots, int n, hipStream_t st);
+/* bytes (a multiple of 4, both ends 4-byte aligned) copied by a kernel on
+ * the stream; used between device memory and mapped pinned host memory,
+ * where an asynchronous copy call has been seen to hold the calling thread
+ * for 6-7 ms now and then */
+void fr_launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
 void fr_launch_graph_add(const FrLaunch &L, int slot, int x, int z, float h, int px, int pz, float cost, int type,
                          int count_parent, hipStream_t st);
 void fr_launch_set_collision(const FrLaunch &L, int slot, int x, int z, int npx, int npz, float h, float cost,
@@ -312,12 +317,12 @@ class FrEngine {
      * of the device.  Under continuous admission (256 px, 168 slots, 20 ms
      * deadlines) four entries raised the p99 latency from 19.8 to 24 ms. */
     static const int FR_PLAN_RING = 1;
-    FrPlanDev *h_plan_stage_ = nullptr;
+    FrPlanDev *h_plan_stage_ = nullptr, *d_plan_stage_ = nullptr; /* mapped: host and device addresses */
     hipEvent_t plan_ev_[FR_PLAN_RING] = {};
     int plan_next_ = 0;
     /* pinned ring of FR_STAGE_RING int arrays of max_slots + 1 entries */
     static const int FR_STAGE_RING = 8;
-    int *h_int_stage_ = nullptr;
+    int *h_int_stage_ = nullptr, *d_int_stage_ = nullptr;
     hipEvent_t int_ev_[FR_STAGE_RING] = {};
     int int_next_ = 0;
     std::vector<int> active_;
@@ -326,7 +331,7 @@ class FrEngine {
      * uploaded in one copy from a pinned ring */
     static const int FR_ACT_RING = 4;
     void *d_act_ = nullptr;
-    uint8_t *h_act_stage_ = nullptr;
+    uint8_t *h_act_stage_ = nullptr, *d_act_stage_ = nullptr;
     size_t act_bytes_ = 0, act_tiles_off_ = 0;
     long act_max_tiles_ = 0;
     hipEvent_t act_ev_[FR_ACT_RING] = {};

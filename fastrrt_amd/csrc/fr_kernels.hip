@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <stdexcept>
 
 #include "fr_device.h"
 #include "fr_engine.h"
@@ -3320,6 +3321,35 @@ void fr_launch_ctl_idle(FrCtl *ctl, const int *slots, int n, hipStream_t st)
             a.s[k] = slots[base + k];
         hipLaunchKernelGGL(k_ctl_idle, dim3(1), dim3(256), 0, st, ctl, a);
     }
+}
+
+/* A copy by 4-byte words (16 at a time where both ends allow). */
+__global__ void __launch_bounds__(256) k_copy_words(uint32_t *dst, const uint32_t *src, long nwords)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if ((((uintptr_t)dst | (uintptr_t)src) & 15) == 0) {
+        const long n4 = nwords >> 2;
+        for (long i = t; i < n4; i += stride)
+            ((uint4 *)dst)[i] = ((const uint4 *)src)[i];
+        for (long i = 4 * n4 + t; i < nwords; i += stride)
+            dst[i] = src[i];
+    } else {
+        for (long i = t; i < nwords; i += stride)
+            dst[i] = src[i];
+    }
+}
+
+void fr_launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    if (bytes == 0)
+        return;
+    if ((bytes & 3) || (((uintptr_t)dst | (uintptr_t)src) & 3))
+        throw std::runtime_error("fr_launch_copy: sizes and addresses must be multiples of 4 bytes");
+    const long nw = (long)(bytes / 4);
+    const long blocks = std::min((nw / 4 + 255) / 256 + 1, 1024L);
+    hipLaunchKernelGGL(k_copy_words, dim3((unsigned)blocks), dim3(256), 0, st, (uint32_t *)dst,
+                       (const uint32_t *)src, nw);
 }
 
 /* ------------------------------------------------------------------------ */
