@@ -67,6 +67,26 @@ def footprint_free(obstacle, trav_all, x, z, heading, mx, mz, ego=None):
     return not bool(obstacle[zl[ok], xl[ok]].any())
 
 
+def border_rollout_end_ok(obstacle, trav_all, prev, cur, mx, mz, ego=None, samples=512):
+    """A rollout that leaves the grid stops in its last in-grid cell, a cell
+    on the border, and returns the heading after the step that left
+    (kinematics.cu:183-202: the heading is advanced before the grid test
+    breaks the loop), while that cell's footprint was tested with the
+    heading it was entered with, up to a cell's worth of 0.1 px steps
+    earlier on the same arc.  Along a rollout the heading moves
+    monotonically from the parent's to the end's, so the tested heading lies
+    in that range: True when `cur` is on the border and its footprint is
+    free at one of `samples` headings spread over the range."""
+    H, W = obstacle.shape
+    x, z = int(cur[0]), int(cur[1])
+    if not (x == 0 or z == 0 or x == W - 1 or z == H - 1):
+        return False
+    for h in np.linspace(float(prev[2]), float(cur[2]), samples, dtype=np.float32):
+        if footprint_free(obstacle, trav_all, x, z, np.float32(h), mx, mz, ego):
+            return True
+    return False
+
+
 def obstacle_mask(frame, class_costs):
     """classCost[TO_INT(frame.x)] < 0; a class outside the table is an obstacle."""
     cls = np.trunc(frame[:, :, 0]).astype(np.int64)
@@ -166,7 +186,7 @@ def check_path(frame, class_costs, path, start, goal, min_dist, ego=None, dist_t
         if footprint_free(obst, trav, int(xs[k]), int(zs[k]), np.float32(path[k, 2]), mx, mz, ego):
             continue
         hermite[k - 1] = hermite_edge_ok(obst, trav, path[k - 1], path[k], last, mx, mz, ego)
-        if not hermite[k - 1]:
+        if not hermite[k - 1] and not border_rollout_end_ok(obst, trav, path[k - 1], path[k], mx, mz, ego):
             fails.append("footprint")
             break
     dh = np.diff(path[:, 2])

@@ -2,6 +2,7 @@
 footprint verdict equals og_feasible (path_feasible.cu:13-78) on random
 poses of benchmark frames, oracle-planned paths pass every check, and
 corrupted paths fail the check they break."""
+import ctypes
 import math
 
 import numpy as np
@@ -112,3 +113,51 @@ def test_hermite_edge_equals_oracle_connection():
         assert got == (c >= 0), (x, z, h1, gx, gz, h2, mc, c)
         n_ok += got
     assert 20 < n_ok < 580
+
+
+def test_rollouts_leaving_the_grid_pass_the_border_rule():
+    """The oracle's rollouts (kinematics.cu:123-231) that leave the grid stop
+    in a border cell with the heading after the step that left: every such
+    end the oracle accepts passes border_rollout_end_ok, including ends whose
+    footprint at the returned heading collides; a border waypoint whose
+    footprint collides at every heading of its range is rejected."""
+    L = O.lib()
+    sc = scenes.scene_256(41, 0.10)
+    frame = np.ascontiguousarray(sc.frame, np.float32)
+    # free top rows with thin posts that only footprints turned away from
+    # the vertical reach
+    frame[0:24, :, 0] = scenes.FREE_CLASS
+    frame[12:20, :, 0] = np.where(np.arange(256) % 48 < 4, scenes.OBSTACLE_CLASS, scenes.FREE_CLASS)
+    ego = (sc.ego_lb, sc.ego_ub)
+    g = O.Graph(256, 256)
+    g.physical(256, 256, 40.0, 5.412658773)
+    g.costs(sc.class_costs)
+    g.search(22, 40, sc.ego_lb[0], sc.ego_lb[1], sc.ego_ub[0], sc.ego_ub[1])
+    obst = V.obstacle_mask(frame, sc.class_costs)
+    trav = (np.trunc(frame[:, :, 2]).astype(np.int64) & 1) != 0
+    rng = np.random.default_rng(12)
+    out = (ctypes.c_float * 4)()
+    max_steer = 40.0 * math.pi / 180.0
+    n_border = n_collide = 0
+    for _ in range(3000):
+        x, z = int(rng.integers(30, 226)), int(rng.integers(1, 6))
+        h0 = float(np.float32(rng.uniform(-0.6, 0.6)))
+        steer = float(np.float32(rng.uniform(-max_steer, max_steer)))
+        g.add(x, z, h0, -1, -1, 0)
+        L.og_kinematic_new_path(g.g, O.fptr(frame), x, z, steer, float(rng.integers(5, 41)), 1.0, out)
+        if out[0] < 0:
+            continue
+        ex, ez, eh = int(out[0]), int(out[1]), float(out[3])
+        if ez != 0:
+            continue
+        n_border += 1
+        n_collide += not V.footprint_free(obst, trav, ex, ez, np.float32(eh), 11, 20, ego)
+        assert V.border_rollout_end_ok(obst, trav, (x, z, h0), (ex, ez, eh), 11, 20, ego), (x, z, h0, steer)
+    assert n_border > 500 and n_collide > 20, (n_border, n_collide)
+    # a border cell inside an obstacle block collides at every heading
+    blocked = np.array(frame, copy=True)
+    blocked[0:6, 100:140, 0] = scenes.OBSTACLE_CLASS
+    ob2 = V.obstacle_mask(blocked, sc.class_costs)
+    assert not V.border_rollout_end_ok(ob2, trav, (120, 3, 0.0), (120, 0, 0.3), 11, 20, ego)
+    # an interior waypoint is not covered by the rule
+    assert not V.border_rollout_end_ok(obst, trav, (120, 30, 0.0), (120, 20, 0.3), 11, 20, ego)
