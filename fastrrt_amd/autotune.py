@@ -54,8 +54,9 @@ WAVES = (1024, 2048, 4096, 8192)
 TILES = (512, 1024, 2048, 4096)
 ITEMS = (256, 1024, 2048)
 BLOCKS = (64, 128, 256)
-# plans per batch by grid side: about 11 M cells, as the 168-plan 256 px batch
-PLANS = {128: 512, 256: 168, 384: 72, 512: 40}
+# plans per batch by grid side: the default benchmark's 120 plans at 256 px,
+# about 8-11 M cells at the other sides
+PLANS = {128: 512, 256: 120, 384: 72, 512: 40}
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "launch_mi355x.json")
 
 
