@@ -768,28 +768,58 @@ __device__ __forceinline__ bool dev_curve_gentle(const FrHermite &hm, float max_
     return L >= 0.5 && qmax <= 0.75 * (double)max_curv * (L * L * L);
 }
 
-/* Exact early rejection in pass 0 of a set-up curve of more than 64 points,
- * from 64 of its points spread over it (lane l takes point l * ceil(n / 64)).
- * Pass 0 rejects a curve when an in-grid point's centre pixel is an obstacle
- * pixel not feasible at every heading, and every in-grid point has the cell
- * of a point that pass 0 tests (the first of its run of equal cells), so a
- * sample there is conclusive.  Most rejected curves are then rejected in one
- * chunk instead of the two or three that the walk takes to reach and test
- * its first obstacle.  (The same look at the footprints of pass 1 cost more
- * than it saved: the curves that reach pass 1 mostly pass it.) */
+/* Exact first look of pass 0 at a set-up curve of more than 64 points, from
+ * 64 of its points spread over it (lane l takes point l * s, s = ceil(n / 64)).
+ * Rejection: pass 0 rejects a curve when an in-grid point's centre pixel is
+ * an obstacle pixel not feasible at every heading, and every in-grid point
+ * has the cell of a point that pass 0 tests (the first of its run of equal
+ * cells), so a sample there is conclusive.  Certification (`cert_ok`: no
+ * per-point curvature test, goal inside the grid): every point lies within
+ * g = max(s / 2 + 1, n - 1 - last sample) steps of t = 1 / (n - 1) of a
+ * sample, so within delta = M g / (n - 1) of it, M >= |P'| on [0, 1] being
+ * |A| + |B| + |C| for P'(t) = A t^2 + B t + C; its cell is then within
+ * Chebyshev distance ceil(delta) of the sample's, so samples all in the grid
+ * with distance planes above ceil(delta) + 1 leave no obstacle pixel on the
+ * curve, and the last in-grid point is the goal (t = 1 evaluates to it
+ * exactly).  Returns -1 (rejected), 1 (passes, last cell the goal) or 0
+ * (walk the curve).  Most rejected curves end here in one chunk instead of
+ * the two or three the walk takes to reach and test their first obstacle.
+ * (The same look at the footprints of pass 1 cost more than it saved: the
+ * curves that reach pass 1 mostly pass it.) */
 #ifndef FR_SPARSE
 #define FR_SPARSE 1
 #endif
-__device__ __forceinline__ bool dev_curve_sparse_reject(const FrPlanDev &P, const FrHermite &hm, int num_points,
-                                                        bool host_nan)
+__device__ __forceinline__ int dev_curve_sparse_pass0(const FrPlanDev &P, const FrHermite &hm, int num_points,
+                                                      bool host_nan, bool cert_ok)
 {
-    const int i = dev_lane() * ((num_points + 63) >> 6);
+    const int s = (num_points + 63) >> 6;
+    const int i = dev_lane() * s;
     int cx = 0, cz = 0;
     double t = 0;
     const bool in = i < num_points && dev_hermite_point(P, hm, i, host_nan, &cx, &cz, &t);
-    const bool hit =
-        in && !(P.trav[cz * P.W + cx] & 1) && P.dist[fr_dist_at(P.TW, cx + P.pad, cz + P.pad)] == 0;
-    return __ballot(hit) != 0ull;
+    int tv = 1, dv = 255;
+    if (in) {
+        tv = P.trav[cz * P.W + cx];
+        dv = P.dist[fr_dist_at(P.TW, cx + P.pad, cz + P.pad)];
+    }
+    if (__ballot(in && !(tv & 1) && dv == 0))
+        return -1;
+    if (!cert_ok)
+        return 0;
+    const double ax = 6.0 * (hm.dx0 - hm.dgx) + 3.0 * (hm.t1x + hm.t2x);
+    const double az = 6.0 * (hm.dz0 - hm.dgz) + 3.0 * (hm.t1y + hm.t2y);
+    const double bx = 6.0 * (hm.dgx - hm.dx0) - 4.0 * hm.t1x - 2.0 * hm.t2x;
+    const double bz = 6.0 * (hm.dgz - hm.dz0) - 4.0 * hm.t1y - 2.0 * hm.t2y;
+    const double M = __builtin_sqrt(ax * ax + az * az) + __builtin_sqrt(bx * bx + bz * bz) +
+                     __builtin_sqrt(hm.t1x * hm.t1x + hm.t1y * hm.t1y);
+    const int last = s * ((num_points - 1) / s < 63 ? (num_points - 1) / s : 63);
+    const int tail = num_points - 1 - last;
+    const int g = s / 2 + 1 > tail ? s / 2 + 1 : tail;
+    const double delta = M * (double)g / (double)(num_points - 1) * (1.0 + 1e-9) + 1e-6;
+    if (!(delta < 60.0)) /* NaN, or too coarse for the capped distance plane */
+        return 0;
+    const int thr = (int)__builtin_ceil(delta) + 1;
+    return __ballot(i < num_points && (!in || dv <= thr)) ? 0 : 1;
 }
 
 /* Pass 0, chunk by chunk of 64 points: each chunk's geometry and curvature,
@@ -802,8 +832,17 @@ __device__ __forceinline__ bool dev_wave_curve_pass0(const FrPlanDev &P, const F
 {
     last_x = -1;
     last_z = -1;
-    if (FR_SPARSE && !skip_fp && num_points > 64 && dev_curve_sparse_reject(P, hm, num_points, host_nan))
-        return false;
+    if (FR_SPARSE && !skip_fp && num_points > 64) {
+        const bool cert_ok = !(max_curv > 0) && hm.dgx >= 0 && hm.dgx < P.W && hm.dgz >= 0 && hm.dgz < P.H;
+        const int sp = dev_curve_sparse_pass0(P, hm, num_points, host_nan, cert_ok);
+        if (sp < 0)
+            return false;
+        if (sp > 0) {
+            last_x = (int)hm.dgx;
+            last_z = (int)hm.dgz;
+            return true;
+        }
+    }
     int tv_prev = 1, dv_prev = 1;
     for (int b = 0; b < num_points; b += 64) {
         double t;
@@ -2156,7 +2195,7 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_demand(const FrPlanDev *plans, 
  * the queue grows): the lanes set up the block's goal curves together, one
  * cell per lane, then the wave walks the block's curves one by one. */
 #ifndef FR_GC_EVAL_MIN
-#define FR_GC_EVAL_MIN 4
+#define FR_GC_EVAL_MIN 2
 #endif
 #ifndef FR_GC_FOOT_MIN
 #define FR_GC_FOOT_MIN 1

@@ -507,3 +507,42 @@ def test_feasible_connection_gtest():
     L.og_derivate_node(g.g, O.fptr(frame), float(np.float32(math.radians(20))), 50.0, 2.0, 128, 128, out)
     assert out[0] > 128 and out[1] < 128
     assert _feasible(g, frame, (128, 128), (out[0], out[1]), 2.0)[0] == 1
+
+
+def test_goal_curve_sample_certificate_bound():
+    """The centre pass's certificate from 64 samples of a goal curve of n > 64
+    points (sample l at point l * s, s = ceil(n / 64)) relies on every point
+    lying within delta = M g / (n - 1) of a sample, with M = |A| + |B| + |C|
+    bounding |P'| of the Hermite curve P' = A t^2 + B t + C and g = max(s / 2
+    + 1, n - 1 - last sample) the index gap.  Checked on the curves of random
+    cells to random goals (tangents of length `distance`, headings from the
+    cell and at the goal), points evaluated in FP64 from the Hermite basis."""
+    rng = np.random.default_rng(31)
+    worst = 0.0
+    for _ in range(400):
+        x, z = rng.integers(0, 1024, 2)
+        gx, gz = rng.integers(0, 1024, 2)
+        dist = math.sqrt(float((gx - x) ** 2 + (gz - z) ** 2))
+        n = int(dist)
+        if n <= 64:
+            continue
+        h1, h2 = rng.uniform(-math.pi, math.pi, 2)
+        p0, p1 = np.array([x, z], float), np.array([gx, gz], float)
+        m0 = dist * np.array([math.cos(h1), math.sin(h1)])
+        m1 = dist * np.array([math.cos(h2), math.sin(h2)])
+        t = np.arange(n) / (n - 1)
+        t2, t3 = t * t, t * t * t
+        pts = (np.outer(2 * t3 - 3 * t2 + 1, p0) + np.outer(t3 - 2 * t2 + t, m0) + np.outer(-2 * t3 + 3 * t2, p1)
+               + np.outer(t3 - t2, m1))
+        A = 6 * (p0 - p1) + 3 * (m0 + m1)
+        B = 6 * (p1 - p0) - 4 * m0 - 2 * m1
+        M = np.linalg.norm(A) + np.linalg.norm(B) + np.linalg.norm(m0)
+        s = (n + 63) // 64
+        last = s * min((n - 1) // s, 63)
+        g = max(s // 2 + 1, n - 1 - last)
+        delta = M * g / (n - 1)
+        samples = pts[np.arange(0, last + 1, s)]
+        d = np.min(np.linalg.norm(pts[:, None, :] - samples[None, :, :], axis=2), axis=1)
+        assert d.max() <= delta, (x, z, gx, gz, d.max(), delta)
+        worst = max(worst, float(d.max() / delta))
+    assert worst > 0.05  # the bound is exercised, not vacuous
