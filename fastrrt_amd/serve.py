@@ -57,17 +57,22 @@ class ScenePool:
 
 
 def validate_all(items, dist_tol=20.0):
-    """(scene, path) of every found plan -> {"paths", "valid", "failed"}:
-    each path checked by fastrrt_amd.validate (chain, goal tolerance, every
-    edge's footprints or Hermite connection, heading continuity)."""
+    """(scene, path[, tag]) of every found plan -> {"paths", "valid",
+    "failed"}: each path checked by fastrrt_amd.validate (chain, goal
+    tolerance, every edge's footprints or Hermite connection, heading
+    continuity); the first failures with the checks they fail, the plan's
+    tag, its scene's seed, size and coverage, and the path."""
     from . import validate
     failed, n_ok = [], 0
-    for sc, path in items:
+    for item in items:
+        sc, path = item[0], item[1]
         f, _ = validate.check_path(sc.frame, sc.class_costs, path, sc.start, sc.goal, sc.min_dist,
                                    (sc.ego_lb, sc.ego_ub), dist_tol, math.radians(10))
         n_ok += not f
         if f and len(failed) < 8:
-            failed.append(f)
+            failed.append({"checks": f, "tag": int(item[2]) if len(item) > 2 else None, "scene_seed": int(sc.seed),
+                           "size": [int(sc.frame.shape[1]), int(sc.frame.shape[0])],
+                           "coverage": float(sc.coverage), "path": np.asarray(path).tolist()})
     return {"paths": len(items), "valid": n_ok, "failed": failed}
 
 
@@ -125,7 +130,7 @@ def run_served(duration_s=10.0, rate=2000.0, sizes=SERVED_SIZES, deadlines=DEADL
         d[1] += int(r.ok)
         d[2] += int(miss[-1])
     lat = np.array(lat)
-    valid = validate_all([(reqs[t][2], r.path) for t, r in results.items() if r.ok])
+    valid = validate_all([(reqs[t][2], r.path, t) for t, r in results.items() if r.ok])
     return {"mode": "served", "requests": n, "returned": len(results), "validation": valid,
             "duration_s": duration_s, "offered_rate": rate,
             "completed_per_s": n / wall, "found_per_s": float(np.sum(ok)) / wall,
@@ -161,7 +166,7 @@ def run_batched(n_plans=512, sizes=BATCHED_SIZES, seed=12, max_iter=200, opt_rou
         d[0] += 1
         d[1] += int(r.ok)
         d[2] += r.iterations
-    valid = validate_all([(plans[r.tag][1], r.path) for r in res if r.ok])
+    valid = validate_all([(plans[r.tag][1], r.path, r.tag) for r in res if r.ok])
     return {"mode": "batched", "plans": n_plans, "returned": len(res), "validation": valid,
             "statuses": sorted(set(int(r.status) for r in res)),
             "cells": cells, "wall_ms": wall * 1e3, "plans_per_s": n_plans / wall,
@@ -241,7 +246,7 @@ def run_1024(n_plans=24, seed=13, timeout_ms=300.0, smart=True, region_side=32, 
         out["batched"] = {"plans_per_s": n_plans / wall, "p50_ms": float(np.percentile(lat, 50)),
                           "p99_ms": float(np.percentile(lat, 99)), "success": float(np.mean([r.ok for r in res])),
                           "mean_iterations": float(np.mean([r.iterations for r in res])),
-                          "validation": validate_all([(scs[r.tag], r.path) for r in res if r.ok])}
+                          "validation": validate_all([(scs[r.tag], r.path, r.tag) for r in res if r.ok])}
     return out
 
 

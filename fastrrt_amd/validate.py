@@ -87,6 +87,24 @@ def border_rollout_end_ok(obstacle, trav_all, prev, cur, mx, mz, ego=None, sampl
     return False
 
 
+def border_turn_ok(prev, cur, shape, bound, step_px=0.1):
+    """Heading continuity of a rollout that left the grid (kinematics.cu:
+    183-202): its end heading also carries the turn of the steps taken in
+    its last cell and of the step that left, at most sqrt(2) + 2 steps of
+    arc at the rollout's curvature, which the chord between the cells does
+    not see.  True when `cur` is on the border and the edge's turning rate,
+    that turn taken off, is within `bound` (step_px = v dt, 0.1 px at
+    1 m/s)."""
+    H, W = shape
+    x, z = int(cur[0]), int(cur[1])
+    if not (x == 0 or z == 0 or x == W - 1 or z == H - 1):
+        return False
+    dh = abs(float(cur[2]) - float(prev[2]))
+    chord = math.hypot(float(cur[0]) - float(prev[0]), float(cur[1]) - float(prev[1]))
+    dh = max(dh - bound * (math.sqrt(2.0) + 2.0 * step_px), 0.0)
+    return 2.0 * abs(math.sin(dh / 2.0)) / max(chord - math.sqrt(2.0), 1.0) <= bound + 1e-9
+
+
 def obstacle_mask(frame, class_costs):
     """classCost[TO_INT(frame.x)] < 0; a class outside the table is an obstacle."""
     cls = np.trunc(frame[:, :, 0]).astype(np.int64)
@@ -200,7 +218,7 @@ def check_path(frame, class_costs, path, start, goal, min_dist, ego=None, dist_t
         # its sample points, which bound its turning
         if not hermite[k]:
             hermite[k] = hermite_edge_ok(obst, trav, path[k], path[k + 1], bound[k], mx, mz, ego)
-        if not hermite[k]:
+        if not hermite[k] and not border_turn_ok(path[k], path[k + 1], (H, W), bound[k]):
             fails.append("heading_continuity")
             break
     return fails, rate

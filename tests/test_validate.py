@@ -161,3 +161,49 @@ def test_rollouts_leaving_the_grid_pass_the_border_rule():
     assert not V.border_rollout_end_ok(ob2, trav, (120, 3, 0.0), (120, 0, 0.3), 11, 20, ego)
     # an interior waypoint is not covered by the rule
     assert not V.border_rollout_end_ok(obst, trav, (120, 30, 0.0), (120, 20, 0.3), 11, 20, ego)
+
+
+def test_border_rollout_turns_pass_the_continuity_rule():
+    """Rollouts of the oracle that leave the grid through any side: the
+    turning rate of the edge to their end exceeds the inner bound for some
+    (the end heading carries the steps taken after the last cell was
+    entered), border_turn_ok accepts every one, no rollout that stays inside
+    exceeds the bound, and a short border edge turning 1.2 rad is rejected."""
+    L = O.lib()
+    inner, _ = V.max_turning_rates()
+    g = O.Graph(256, 256)
+    g.physical(256, 256, 40.0, 5.412658773)
+    g.costs([0, 1, 2, 3, 4, -1])
+    g.search(22, 40)
+    frame = O.empty_frame(256, 256, scenes.FREE_CLASS)
+    out = (ctypes.c_float * 4)()
+    rng = np.random.default_rng(100)
+    n_border = n_over = 0
+    for _ in range(12000):
+        side = int(rng.integers(0, 4))
+        x, z = int(rng.integers(1, 255)), int(rng.integers(1, 40))
+        if side == 1:
+            z = 255 - z
+        elif side == 2:
+            x, z = z, x
+        elif side == 3:
+            x, z = 255 - z, x
+        h0 = float(np.float32(rng.uniform(-math.pi, math.pi)))
+        steer = float(np.float32(rng.uniform(-0.698, 0.698)))
+        g.add(x, z, h0, -1, -1, 0)
+        L.og_kinematic_new_path(g.g, O.fptr(frame), x, z, steer, float(rng.integers(5, 41)), 1.0, out)
+        if out[0] < 0:
+            continue
+        a, b = (x, z, h0), (float(out[0]), float(out[1]), float(out[3]))
+        chord = math.hypot(b[0] - a[0], b[1] - a[1])
+        over = 2 * abs(math.sin((b[2] - a[2]) / 2)) / max(chord - math.sqrt(2), 1.0) > inner + 1e-9
+        if int(b[0]) in (0, 255) or int(b[1]) in (0, 255):
+            n_border += 1
+            n_over += over
+            assert V.border_turn_ok(a, b, (256, 256), inner), (a, b)
+        else:
+            assert not over, (a, b)
+    assert n_border > 1000 and n_over > 10, (n_border, n_over)
+    # a short border edge turning 1.2 rad, and the rule off the border
+    assert not V.border_turn_ok((120, 3, 0.0), (121, 0, 1.2), (256, 256), inner)
+    assert not V.border_turn_ok((120, 30, 0.0), (121, 27, 0.3), (256, 256), inner)
