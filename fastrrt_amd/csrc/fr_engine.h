@@ -314,8 +314,7 @@ class FrEngine {
     /* pinned ring of FR_PLAN_RING descriptor tables; an upload waits for
      * the copy FR_PLAN_RING uploads back.  One entry: each admission waits
      * for the previous one's copy, which keeps the host from admitting ahead
-     * of the device.  Under continuous admission (256 px, 168 slots, 20 ms
-     * deadlines) four entries raised the p99 latency from 19.8 to 24 ms. */
+     * of the device. */
     static const int FR_PLAN_RING = 1;
     FrPlanDev *h_plan_stage_ = nullptr, *d_plan_stage_ = nullptr; /* mapped: host and device addresses */
     hipEvent_t plan_ev_[FR_PLAN_RING] = {};
