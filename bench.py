@@ -19,8 +19,8 @@ key (seed, stream), so a plan's search is the same in both forms.
                     :110-114).
 
 Success is compared plan by plan on the plans both forms ran (by default all
-16 timed steps, 1920 plans: about 30 of them are solved by one form only,
-so the comparison's standard error is about 0.4 pp).  Frames are uploaded
+16 timed steps, 1920 plans: about 30-35 of them are solved by one form
+only, so the comparison's standard error is about 0.3 pp).  Frames are uploaded
 before the timed region (device-resident inputs).
 
 roofline: the batched form's unit of work, one launch of the captured
