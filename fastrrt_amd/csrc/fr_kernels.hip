@@ -1448,10 +1448,11 @@ __global__ void __launch_bounds__(FR_BLOCK) k_select(const FrPlanDev *plans, FrC
 
 /* expand_node (graph_expansion.cu:103-145) with check_kinematic_new_path
  * (kinematics.cu:123-231), one wave per rollout.  The step chain is walked
- * 64 steps at a time: the heading and position recurrences are float sums
- * whose rounding order matters, so they are accumulated step by step (every
- * lane runs the uniform chain and keeps its own step's value); the per-step
- * sincos and the cell of each step are spread over the lanes, and the new
+ * 128 steps at a time, two consecutive steps per lane: the heading and
+ * position recurrences are float sums whose rounding order matters, so they
+ * are accumulated step by step, as systolic sums along the wave (each round
+ * adds a lane's two increments to the previous lane's last value); the
+ * per-step sincos and the cell of each step are spread over the lanes, and the new
  * cells' footprints are tested 64 at a time once the walk's geometry is
  * known (dev_roll_flush).  A step's cell is new iff it differs from the
  * previous step's cell
@@ -1461,7 +1462,60 @@ __global__ void __launch_bounds__(FR_BLOCK) k_select(const FrPlanDev *plans, FrC
  * grid or that completes max_size cells; the end heading is the chain value
  * after the last step executed.  Every rollout reads the pre-expansion grid;
  * the end cell is claimed here and written by k_commit. */
-#define FR_ROLL_BUF 128
+#define FR_ROLL_BUF 192 /* < 64 cells left after a flush + at most one new cell per step of a 128-step chunk */
+
+/* 128 positions of a walk chunk in walk order (two steps per lane: position
+ * 2 l and 2 l + 1 on lane l), as two 64-bit words. */
+struct FrMask128 {
+    unsigned long long lo, hi;
+};
+
+/* bits 0..31 of x to the even positions */
+__device__ __forceinline__ unsigned long long dev_spread32(unsigned long long x)
+{
+    x &= 0xFFFFFFFFull;
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+
+/* a: ballot of the lanes' first steps, b: of their second steps */
+__device__ __forceinline__ FrMask128 dev_interleave(unsigned long long a, unsigned long long b)
+{
+    FrMask128 m;
+    m.lo = dev_spread32(a) | (dev_spread32(b) << 1);
+    m.hi = dev_spread32(a >> 32) | (dev_spread32(b >> 32) << 1);
+    return m;
+}
+
+__device__ __forceinline__ int dev_m128_first(const FrMask128 &m)
+{
+    return m.lo ? __ffsll((long long)m.lo) - 1 : (m.hi ? 64 + __ffsll((long long)m.hi) - 1 : 128);
+}
+
+/* position of the n-th (n >= 1) set position; n is at most the count */
+__device__ __forceinline__ int dev_m128_nth(const FrMask128 &m, int n)
+{
+    const int c = __popcll(m.lo);
+    unsigned long long w = n <= c ? m.lo : m.hi;
+    const int base = n <= c ? 0 : 64;
+    for (int r = 1; r < (n <= c ? n : n - c); ++r)
+        w &= w - 1;
+    return base + __ffsll((long long)w) - 1;
+}
+
+/* the positions below lim */
+__device__ __forceinline__ FrMask128 dev_m128_below(const FrMask128 &m, int lim)
+{
+    FrMask128 r;
+    r.lo = lim >= 64 ? m.lo : (m.lo & ((1ull << lim) - 1ull));
+    r.hi = lim >= 128 ? m.hi : (lim <= 64 ? 0ull : (m.hi & ((1ull << (lim - 64)) - 1ull)));
+    return r;
+}
+
 
 /* Footprint tests and cost of the first m (<= 64) buffered cells of a
  * rollout, one cell per lane, all lanes' loads in flight together; the cost
@@ -1508,7 +1562,6 @@ __global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *__restrict
      * fields are read through scalar loads */
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = dev_lane();
-    const unsigned long long below = (1ull << lane) - 1ull;
     int *bx = s_bx[wv], *bz = s_bz[wv];
     float *bh = s_bh[wv];
     /* Workgroups are dealt to the eight XCDs round robin (block b runs on
@@ -1557,77 +1610,102 @@ __global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *__restrict
         float end_h = h;
         bool ok = true;
         int nb = 0; /* cells buffered, footprints not yet tested */
-        for (int k0 = 0;; k0 += 64) {
-            const int k = k0 + lane;
-            /* heading after (hk1) and before (hk) this lane's step: the
-             * serial chain h + hif + hif + ..., each lane's value rounded as
-             * the loop of kinematics.cu rounds it */
-            const float hk1 = dev_wave_serial_sum(h, hif);
-            const float hk = dev_wave_shr1(hk1, h);
-            const float hh = dev_bcast_f(hk1, 63);
-            float sn, cs;
-            fr_sincosf(hk + beta, &sn, &cs);
-            const float dx = ds * cs, dy = ds * sn;
-            /* position after this lane's step, the same serial sums */
-            const float xk = dev_wave_serial_sum(xf, dx);
-            const float yk = dev_wave_serial_sum(yf, dy);
-            const float xx = dev_bcast_f(xk, 63), yy = dev_bcast_f(yk, 63);
-            const int cx = fr_trunc_d(ogx + (double)yk);
-            const int cz = fr_trunc_d(ogy - (double)xk);
-            int px = __shfl_up(cx, 1, 64), pz = __shfl_up(cz, 1, 64);
+        for (int k0 = 0;; k0 += 128) {
+            /* two steps per lane, a = k0 + 2 lane and b = a + 1: each round
+             * of the serial sums adds lane l's two increments after lane
+             * l - 1's second sum (the chunk's start on lane 0) */
+            const int ka = k0 + 2 * lane, kb = ka + 1;
+            float vA = h, vB = h;
+#pragma unroll
+            for (int t = 0; t < 64; ++t) {
+                vA = dev_wave_shr1(vB, h) + hif;
+                vB = vA + hif;
+            }
+            const float hA1 = vA, hB1 = vB, hA0 = dev_wave_shr1(vB, h), hB0 = vA;
+            const float hh = dev_bcast_f(vB, 63);
+            float snA, csA, snB, csB;
+            fr_sincosf(hA0 + beta, &snA, &csA);
+            fr_sincosf(hB0 + beta, &snB, &csB);
+            const float dxA = ds * csA, dyA = ds * snA, dxB = ds * csB, dyB = ds * snB;
+            float uA = xf, uB = xf, wA = yf, wB = yf;
+#pragma unroll
+            for (int t = 0; t < 64; ++t) {
+                uA = dev_wave_shr1(uB, xf) + dxA;
+                wA = dev_wave_shr1(wB, yf) + dyA;
+                uB = uA + dxB;
+                wB = wA + dyB;
+            }
+            const float xx = dev_bcast_f(uB, 63), yy = dev_bcast_f(wB, 63);
+            const int cxA = fr_trunc_d(ogx + (double)wA), czA = fr_trunc_d(ogy - (double)uA);
+            const int cxB = fr_trunc_d(ogx + (double)wB), czB = fr_trunc_d(ogy - (double)uB);
+            int px = __shfl_up(cxB, 1, 64), pz = __shfl_up(czB, 1, 64);
             if (lane == 0) {
                 px = prev_x;
                 pz = prev_z;
             }
-            const bool fresh = !(cx == px && cz == pz);
-            const bool out = cx < 0 || cx >= W || cz < 0 || cz >= H;
-            const unsigned long long m_stop = __ballot(k > loop_max);
-            const unsigned long long m_oob = __ballot(fresh && out);
-            const unsigned long long m_in = __ballot(fresh && !out);
-            /* first lane at which the walk ends: before its step (loop cap),
-             * or after it (out of the grid, or max_size reached) */
-            const int t_before = m_stop ? __ffsll((long long)m_stop) - 1 : 64;
-            int t_at = m_oob ? __ffsll((long long)m_oob) - 1 : 64;
+            const bool freshA = !(cxA == px && czA == pz);
+            const bool freshB = !(cxB == cxA && czB == czA);
+            const bool outA = cxA < 0 || cxA >= W || czA < 0 || czA >= H;
+            const bool outB = cxB < 0 || cxB >= W || czB < 0 || czB >= H;
+            const FrMask128 m_stop = dev_interleave(__ballot(ka > loop_max), __ballot(kb > loop_max));
+            const FrMask128 m_oob = dev_interleave(__ballot(freshA && outA), __ballot(freshB && outB));
+            const FrMask128 m_in = dev_interleave(__ballot(freshA && !outA), __ballot(freshB && !outB));
+            /* first position at which the walk ends: before its step (loop
+             * cap), or after it (out of the grid, or max_size reached) */
+            const int t_before = dev_m128_first(m_stop);
+            int t_at = dev_m128_first(m_oob);
             const int need = max_size - size;
-            if (__popcll(m_in) >= need) {
-                unsigned long long mm = m_in;
-                for (int r = 1; r < need; ++r)
-                    mm &= mm - 1;
-                const int t_reach = __ffsll((long long)mm) - 1;
+            if (__popcll(m_in.lo) + __popcll(m_in.hi) >= need) {
+                const int t_reach = dev_m128_nth(m_in, need);
                 t_at = t_reach < t_at ? t_reach : t_at;
             }
-            const bool ends_before = t_before <= t_at && t_before < 64;
-            const bool ends_at = !ends_before && t_at < 64;
-            const int lim = ends_before ? t_before : (ends_at ? t_at + 1 : 64);
-            const unsigned long long acc = m_in & (lim >= 64 ? ~0ull : ((1ull << lim) - 1ull));
-            /* the new cells join the buffer in walk order; their footprints
-             * and cost are taken 64 at a time (any collision rejects, so the
-             * tests can wait until the walk's geometry is known) */
-            if ((acc >> lane) & 1ull) {
-                const int r = nb + __popcll(acc & below);
-                bx[r] = cx;
-                bz[r] = cz;
-                bh[r] = hk1;
+            const bool ends_before = t_before <= t_at && t_before < 128;
+            const bool ends_at = !ends_before && t_at < 128;
+            const int lim = ends_before ? t_before : (ends_at ? t_at + 1 : 128);
+            const FrMask128 acc = dev_m128_below(m_in, lim);
+            /* the new cells join the buffer in walk order */
+            const unsigned long long aw = lane < 32 ? acc.lo : acc.hi;
+            const int sh = 2 * (lane & 31);
+            const bool accA = (aw >> sh) & 1ull, accB = (aw >> (sh + 1)) & 1ull;
+            const int rA = nb + (lane < 32 ? 0 : __popcll(acc.lo)) + __popcll(aw & ((1ull << sh) - 1ull));
+            if (accA) {
+                bx[rA] = cxA;
+                bz[rA] = czA;
+                bh[rA] = hA1;
             }
-            nb += __popcll(acc);
-            size += __popcll(acc);
-            if (acc) {
-                const int l = 63 - __clzll((long long)acc);
-                last_x = dev_bcast_i(cx, l);
-                last_z = dev_bcast_i(cz, l);
+            if (accB) {
+                const int rB = rA + (accA ? 1 : 0);
+                bx[rB] = cxB;
+                bz[rB] = czB;
+                bh[rB] = hB1;
+            }
+            const int n_acc = __popcll(acc.lo) + __popcll(acc.hi);
+            nb += n_acc;
+            size += n_acc;
+            if (n_acc) {
+                const int pl = acc.hi ? 127 - __clzll((long long)acc.hi) : 63 - __clzll((long long)acc.lo);
+                const int L = pl >> 1;
+                const int lxA = dev_bcast_i(cxA, L), lzA = dev_bcast_i(czA, L);
+                const int lxB = dev_bcast_i(cxB, L), lzB = dev_bcast_i(czB, L);
+                last_x = (pl & 1) ? lxB : lxA;
+                last_z = (pl & 1) ? lzB : lzA;
             }
             bool done = true;
             if (ends_before) {
-                end_h = dev_bcast_f(hk, t_before);
+                const int L = t_before >> 1;
+                const float a0 = dev_bcast_f(hA0, L), b0 = dev_bcast_f(hB0, L);
+                end_h = (t_before & 1) ? b0 : a0;
             } else if (ends_at) {
-                end_h = dev_bcast_f(hk1, t_at);
+                const int L = t_at >> 1;
+                const float a1 = dev_bcast_f(hA1, L), b1 = dev_bcast_f(hB1, L);
+                end_h = (t_at & 1) ? b1 : a1;
             } else {
                 done = false;
                 h = hh;
                 xf = xx;
                 yf = yy;
-                prev_x = dev_bcast_i(cx, 63);
-                prev_z = dev_bcast_i(cz, 63);
+                prev_x = dev_bcast_i(cxB, 63);
+                prev_z = dev_bcast_i(czB, 63);
             }
             while (nb >= 64 || (done && nb > 0)) {
                 const int m = nb < 64 ? nb : 64;
@@ -1636,10 +1714,12 @@ __global__ void __launch_bounds__(FR_BLOCK) k_expand(const FrPlanDev *__restrict
                     break;
                 }
                 nb -= m;
-                if (lane < nb) {
-                    bx[lane] = bx[64 + lane];
-                    bz[lane] = bz[64 + lane];
-                    bh[lane] = bh[64 + lane];
+                /* the rest to the front, 64 entries at a time (an entry is
+                 * read before any write reaches its place) */
+                for (int j = lane; j < nb; j += 64) {
+                    bx[j] = bx[64 + j];
+                    bz[j] = bz[64 + j];
+                    bh[j] = bh[64 + j];
                 }
             }
             if (done || !ok)
