@@ -19,17 +19,25 @@ key (seed, stream), so a plan's search is the same in both forms.
                     :110-114).
 
 Success is compared plan by plan on the plans both forms ran (by default all
-16 timed steps, 1920 plans: about 30-35 of them are solved by one form
-only, so the comparison's standard error is about 0.3 pp).  Frames are uploaded
-before the timed region (device-resident inputs).
+20 timed steps, 1920 plans: about 30 of them are solved by one form only, so
+the comparison's standard error is about 0.3 pp).  96 plans per step: the
+same plans solved in groups of 120, 96, 72 and 48 gave 628, 634, 636 and 643
+of 1920 (each smaller group gets more loop iterations per plan before the
+deadline and loses none that a larger one solved), and the plain loop solves
+about 633-645 of them, so 96 keeps the success gap within about 0.5 pp at
+about 50 x 96 plans/s.  Frames are uploaded before the timed region
+(device-resident inputs).
 
 roofline: the batched form's unit of work, one launch of the captured
 iteration graph (one loop iteration of every running plan: 11 kernels, of
 which the two lazy goal-plane kernels and k_expand take about half the time).
 The timed region retires plans on wall-clock deadlines, so its work depends
 on timing (and a profiler's overhead changes it); the roofline is therefore
-measured on a deterministic pass after it: the first timed step's plans,
-batched, `--roofline-iters` iterations, no deadline, no optimisation rounds.
+measured on a deterministic pass after it: the plans of step 1 of the
+sequence (the first timed step at the default single warm-up step, the same
+plans whatever `--warmup` is, so the committed profiles cover the very pass a
+run reports), batched, `--roofline-iters` iterations, no deadline, no
+optimisation rounds.
 The graph is timed with HIP events recorded on the engine's stream around
 each launch.  `python bench.py --roofline-only` runs that pass alone (the
 command the committed rocprofv3 summaries profile).  Algorithmic bytes per
@@ -85,9 +93,9 @@ SEED = 1234
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=120, help="plans per step")
+    ap.add_argument("--batch", type=int, default=96, help="plans per step")
     ap.add_argument("--budget-ms", type=float, default=20.0, help="per-plan latency budget (batched deadline)")
     ap.add_argument("--plain-timeout-ms", type=int, default=-1,
                     help="FastRRT handle timeout of the plain form (-1: budget - 2 ms, so that its p99 stays "
@@ -95,7 +103,7 @@ def parse():
     ap.add_argument("--max-iter", type=int, default=200)
     ap.add_argument("--opt-rounds", type=int, default=20)
     ap.add_argument("--plain-plans", type=int, default=-1, help="plans of the first timed step run plain (-1: all)")
-    ap.add_argument("--plain-steps", type=int, default=16,
+    ap.add_argument("--plain-steps", type=int, default=20,
                     help="timed steps whose plans also run plain (success compared plan by plan)")
     ap.add_argument("--cpu-sample", type=int, default=2, help="plans timed on the CPU oracle, one thread (0 = skip)")
     ap.add_argument("--cpu-workers", type=int, default=16, help="processes of the all-core CPU oracle run")
@@ -351,7 +359,7 @@ def roofline_pass(L, batch, plans, args, frame_of):
                          "algorithmic_bytes_per_launch": round(ex_bytes, 1),
                          "achieved_gbs": round(ex_bytes / (ex_avg * 1e-3) / 1e9, 4)},
             "instruction_side": instr,
-            "pass": f"{len(plans)} plans of the first timed step, batched, {args.roofline_iters} iterations, "
+            "pass": f"{len(plans)} plans of step 1 of the sequence, batched, {args.roofline_iters} iterations, "
                     "no deadline, no optimisation rounds"}
 
 
@@ -559,11 +567,11 @@ def main():
             frame_of(pl.sc)
     B = len(steps[0])
     batch = Batch(max_plans=B, max_cells=B * 256 * 256, launch="auto")
-    first = steps[args.warmup]
+    roof_plans = steps[min(1, len(steps) - 1)]
     if args.roofline_only:
         # warm-up on the same plans: both passes are the same deterministic work
-        run_batched(batch, first, args, frame_of, max_iter=args.roofline_iters, opt_rounds=0, deadline=0.0)
-        print(json.dumps({"roofline": roofline_pass(L, batch, first, args, frame_of)}), flush=True)
+        run_batched(batch, roof_plans, args, frame_of, max_iter=args.roofline_iters, opt_rounds=0, deadline=0.0)
+        print(json.dumps({"roofline": roofline_pass(L, batch, roof_plans, args, frame_of)}), flush=True)
         return
     results, walls = [], []
     if args.mode == "served":
@@ -632,7 +640,7 @@ def main():
     if cpu and cpu.get("c1"):
         c1 = c1_compare(P, L, Batch, cpu["c1"], args, frame_of)
         cpu["c1"].pop("per_plan", None)
-    roof = roofline_pass(L, batch, first, args, frame_of)
+    roof = roofline_pass(L, batch, roof_plans, args, frame_of)
     out = {
         "metric": "plans/s at p99<=20 ms (served/batched vs plain loop), 256^2 grid, success %",
         "value": round(batched["plans_per_s"], 3), "unit": "plans/s", "n_gpus": 1, "steps": args.steps,

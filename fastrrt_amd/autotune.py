@@ -27,10 +27,10 @@ a launch shape: it changes the search (the reference expands every eligible
 node each iteration), so it is off by default and `measure_budget` only
 reports its effect on success within a deadline, on the same plans as the
 default search.  The LDS buffers (the rollout cell buffer and the footprint
-group sets, 128 and 64 entries per wave; the erase's in-LDS ancestor
+group sets, 192 and 64 entries per wave; the erase's in-LDS ancestor
 capacity) are compile-time sizes fixed by the kernels' structure: a wave
-flushes its cell buffer 64 cells at a time, so 128 entries hold a flush and
-the next chunk of the walk.
+flushes its cell buffer 64 cells at a time and walks 128 steps per chunk, so
+192 entries hold what a flush leaves and the next chunk's new cells.
 
     python -m fastrrt_amd.autotune --sizes 128 256 384 512 --out table.json
 
@@ -54,9 +54,9 @@ WAVES = (1024, 2048, 4096, 8192)
 TILES = (512, 1024, 2048, 4096)
 ITEMS = (256, 1024, 2048)
 BLOCKS = (64, 128, 256)
-# plans per batch by grid side: the default benchmark's 120 plans at 256 px,
+# plans per batch by grid side: the default benchmark's 96 plans at 256 px,
 # about 8-11 M cells at the other sides
-PLANS = {128: 512, 256: 120, 384: 72, 512: 40}
+PLANS = {128: 512, 256: 96, 384: 72, 512: 40}
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "launch_mi355x.json")
 
 
@@ -162,7 +162,7 @@ def tune_side(side, n_plans=None, iters=24, repeats=3, seed=4242, log=print):
             "measured": [{"shape": list(k), "ms": round(v, 5)} for k, v in sorted(times.items())]}
 
 
-def measure_budget(budgets, n_plans=120, steps=10, deadline_ms=20.0, seed=4343, log=print):
+def measure_budget(budgets, n_plans=96, steps=10, deadline_ms=20.0, seed=4343, log=print):
     """The expansion budget's effect at the default benchmark's workload
     (256 px, 5/15/30 % rectangles, 22 x 40 footprint, `n_plans` plans per
     batch, each with a `deadline_ms` deadline): the same `steps` batches of
