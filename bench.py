@@ -39,7 +39,8 @@ plans whatever `--warmup` is, so the committed profiles cover the very pass a
 run reports), batched, `--roofline-iters` iterations, no deadline, no
 optimisation rounds.
 The graph is timed with HIP events recorded on the engine's stream around
-each launch.  `python bench.py --roofline-only` runs that pass alone (the
+each launch; k_expand on a second, identical pass with the iteration captured
+as three graphs and events between them.  `python bench.py --roofline-only` runs that pass alone (the
 command the committed rocprofv3 summaries profile).  Algorithmic bytes per
 plan-iteration (the path's per-unit figure): 6 B per grid cell (4 B node
 type, 1 B obstacle plane, 1 B goal-connection flag) + 8 B per expansion
@@ -297,14 +298,25 @@ def run_served(b, plans, args, frame_of, capacity):
 
 
 def roofline_pass(L, batch, plans, args, frame_of):
-    """The iteration graph (and k_expand) over a deterministic batched pass."""
-    L.fastrrt_profile_reset()
-    L.fastrrt_profile_enable(True)
-    res, _ = run_batched(batch, plans, args, frame_of, max_iter=args.roofline_iters, opt_rounds=0, deadline=0.0)
-    L.fastrrt_profile_enable(False)
-    prof = read_prof(L)
+    """The iteration graph (and k_expand) over a deterministic batched pass,
+    run twice: the graph timed whole as it runs unprofiled, then split in
+    three to time k_expand alone (the same work: the results must match)."""
+    def timed_pass(split):
+        L.fastrrt_profile_reset()
+        L.fastrrt_profile_split(split)
+        L.fastrrt_profile_enable(True)
+        r, _ = run_batched(batch, plans, args, frame_of, max_iter=args.roofline_iters, opt_rounds=0, deadline=0.0)
+        L.fastrrt_profile_enable(False)
+        L.fastrrt_profile_split(False)
+        return r, read_prof(L)
+
+    res, prof = timed_pass(False)
+    res2, prof2 = timed_pass(True)
+    key = lambda rs: sorted((r.tag, r.status, r.iterations, r.expansions) for r in rs)
+    if key(res) != key(res2):
+        raise RuntimeError("the roofline pass gave different results in its two runs")
     it_n, it_ms = prof["iteration"]
-    ex_n, ex_ms = prof["expand"]
+    ex_n, ex_ms = prof2["expand"]
     if not it_n or not ex_n:
         return None
     cells = sum(pl.sc.width * pl.sc.height for pl in plans) / len(plans)

@@ -56,6 +56,7 @@ SIGNATURES = {
     "fastrrt_export_state": (None, [c_void_p, i32p, i32p, i32p, i32p, fp, fp, u8p, fp, fp]),
     "fastrrt_profile_enable": (None, [c_bool]),
     "fastrrt_profile_reset": (None, []),
+    "fastrrt_profile_split": (None, [c_bool]),
     "fastrrt_profile_read": (c_bool, [c_int, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(c_double)]),
     # graph handle (src/python_cuda_graph_bind.cpp)
     "cudagraph_initialize": (c_void_p, [c_int, c_int, c_float, c_float, c_float, c_float, c_int, c_int, c_int, c_int,

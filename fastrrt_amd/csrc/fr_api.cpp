@@ -1553,6 +1553,7 @@ void cudagraph_export_state(void *ptr, int32_t *parent_x, int32_t *parent_z, int
 }
 
 void fastrrt_profile_enable(bool on) { FR_TRY_VOID({ fr_prof_enable(on); }) }
+void fastrrt_profile_split(bool on) { FR_TRY_VOID({ fr_prof_split(on); }) }
 
 void fastrrt_profile_reset(void) { FR_TRY_VOID({ fr_prof_reset(); }) }
 

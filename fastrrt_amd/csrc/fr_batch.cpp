@@ -517,14 +517,16 @@ class FrBatch {
     }
 
     /* One loop iteration of every running plan: one replay of the captured
-     * iteration graph.  With kernel timing on, the iteration is captured as
-     * three graphs (before k_expand, k_expand, after) and k_expand is timed
-     * with events recorded on the stream between their replays. */
+     * iteration graph, timed whole with kernel timing on.  With the split
+     * also on, the iteration is captured as three graphs (before k_expand,
+     * k_expand, after) and k_expand is timed with events recorded on the
+     * stream between their replays (the host then submits three graphs per
+     * iteration, which the whole iteration's time includes). */
     void launch_iteration()
     {
-        const bool prof = fr_prof_on();
-        if (!graph_[0] || prof != graph_timed_)
-            capture(prof);
+        const bool split = fr_prof_on() && fr_prof_split_on();
+        if (!graph_[0] || split != graph_timed_)
+            capture(split);
         hipStream_t st = eng_.stream();
         fr_prof_begin(FR_PROF_ITERATION, st);
         if (!graph_timed_) {

@@ -23,6 +23,7 @@ namespace {
 struct ProfState {
     std::mutex mu;
     bool on = false;
+    bool split = false;
     std::deque<std::pair<hipEvent_t, hipEvent_t>> done[FR_PROF_N];
     hipEvent_t open_ev[FR_PROF_N] = {};
     std::vector<hipEvent_t> pool;
@@ -84,6 +85,12 @@ void fr_prof_enable(bool on)
     prof().on = on;
 }
 bool fr_prof_on() { return prof().on; }
+void fr_prof_split(bool on)
+{
+    std::lock_guard<std::mutex> g(prof().mu);
+    prof().split = on;
+}
+bool fr_prof_split_on() { return prof().split; }
 void fr_prof_begin(int k, hipStream_t st)
 {
     ProfState &p = prof();

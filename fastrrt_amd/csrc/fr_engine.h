@@ -156,6 +156,10 @@ enum FrProfKernel { FR_PROF_GOAL_CONNECTION = 0, FR_PROF_EXPAND = 1, FR_PROF_CON
                     FR_PROF_N = 4 };
 void fr_prof_enable(bool on);
 bool fr_prof_on();
+/* with profiling on, capture the batch iteration in three parts so that
+ * k_expand is timed alone (otherwise the one graph is timed whole) */
+void fr_prof_split(bool on);
+bool fr_prof_split_on();
 void fr_prof_begin(int k, hipStream_t st);
 void fr_prof_end(int k, hipStream_t st);
 /* synchronises the recorded events; returns launches and total ms */

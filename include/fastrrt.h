@@ -124,6 +124,11 @@ void fastrrt_export_state(void *ptr, int32_t *parent_x, int32_t *parent_z, int32
 void fastrrt_profile_enable(bool on);
 void fastrrt_profile_reset(void);
 bool fastrrt_profile_read(int kernel, long *launches, double *total_ms);
+/* New: with profiling on, a batch's iteration graph is captured as three
+ * graphs (before k_expand, k_expand, after) so that kernel 1 times k_expand
+ * alone.  Off (the default), the iteration runs as the one graph it runs
+ * unprofiled and kernel 3 times exactly that. */
+void fastrrt_profile_split(bool on);
 
 /* ---------------------------------------------------------------------- */
 /* Graph handle (src/python_cuda_graph_bind.cpp)                           */
