@@ -203,3 +203,37 @@ def test_launch_shapes_leave_results_unchanged():
         with pytest.raises(RuntimeError):
             b.set_launch(*bad)
     assert b.launch() == (8, 1, 1) and b.block() == 64
+
+
+def test_profiled_capture_equals_the_graph():
+    """Kernel timing, with and without the three-part capture of the
+    iteration (fastrrt_profile_split), gives every plan the result of the
+    unprofiled iteration graph; the timers record the iteration (and, split,
+    k_expand) once per loop iteration."""
+    import ctypes
+    from fastrrt_amd import _lib
+    L = _lib.load()
+    scs = [_scene(i) for i in range(6)]
+    frames = [P.SearchFrame(sc.frame, sc.class_costs) for sc in scs]
+    b = Batch(max_plans=len(scs), max_cells=sum(s.width * s.height for s in scs))
+    out = {}
+    for mode in ("off", "whole", "split"):
+        print(f"[batch] profiling {mode}", flush=True)
+        L.fastrrt_profile_reset()
+        L.fastrrt_profile_split(mode == "split")
+        L.fastrrt_profile_enable(mode != "off")
+        for i, sc in enumerate(scs):
+            b.submit(_spec(sc, frames[i], i, max_iter=30, rounds=2))
+        res = b.drain()
+        L.fastrrt_profile_enable(False)
+        L.fastrrt_profile_split(False)
+        out[mode] = {r.tag: _key(r) for r in res}
+        n_it, n_ex, ms = ctypes.c_long(), ctypes.c_long(), ctypes.c_double()
+        L.fastrrt_profile_read(3, ctypes.byref(n_it), ctypes.byref(ms))
+        L.fastrrt_profile_read(1, ctypes.byref(n_ex), ctypes.byref(ms))
+        if mode == "off":
+            assert n_it.value == 0 and n_ex.value == 0
+        else:
+            assert n_it.value >= max(k[1] for k in out[mode].values()) - 1 > 0
+            assert n_ex.value == (n_it.value if mode == "split" else 0)
+    assert out["whole"] == out["off"] and out["split"] == out["off"]
