@@ -2280,6 +2280,17 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_demand(const FrPlanDev *plans, 
 #ifndef FR_GC_FOOT_MIN
 #define FR_GC_FOOT_MIN 1
 #endif
+/* Workgroups are dealt to the eight XCDs round robin (block b runs on XCD
+ * b % 8): each XCD takes one contiguous eighth of the queue, whose entries
+ * mostly belong to a few plans, so a plan's planes are read through one
+ * XCD's L2.  The grid is a multiple of 8 blocks.  Defines lo, hi (this XCD's
+ * range), nwaves (waves per XCD), cpw and gw (this wave's index on its XCD). */
+#define FR_GC_XCD_RANGE(n, lo_min)                                                                           \
+    const int xcd = (int)(blockIdx.x & 7u), jb = (int)(blockIdx.x >> 3);                                     \
+    const long nwaves = (long)(gridDim.x >> 3) * wpb;                                                          \
+    const long lo = (n) * xcd / 8, hi = (n) * (xcd + 1) / 8;                                                   \
+    const int cpw = dev_gc_block(hi - lo, nwaves, lo_min);                                                     \
+    const long gw = (long)jb * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 __device__ __forceinline__ int dev_gc_block(long n, long nwaves, int lo)
 {
     const long c = (n + nwaves - 1) / nwaves;
@@ -2309,12 +2320,10 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_eval(const FrPlanDev *__restric
     if (n > dq_cap)
         n = dq_cap;
     const int wpb = blockDim.x >> 6;
-    const long nwaves = (long)gridDim.x * wpb;
-    const int cpw = dev_gc_block(n, nwaves, FR_GC_EVAL_MIN);
+    FR_GC_XCD_RANGE(n, FR_GC_EVAL_MIN)
     const int lane = dev_lane();
-    const long gw = (long)blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    for (long b0 = gw * cpw; b0 < n; b0 += nwaves * cpw) {
-        const int cnt = (int)(n - b0 < cpw ? n - b0 : cpw);
+    for (long b0 = lo + gw * cpw; b0 < hi; b0 += nwaves * cpw) {
+        const int cnt = (int)(hi - b0 < cpw ? hi - b0 : cpw);
         int2 d = make_int2(0, 0);
         FrHermite hm = {};
         int np = 0;
@@ -2383,12 +2392,10 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_foot(const FrPlanDev *__restric
 {
     const long n = counters[FR_CNT_DEMAND2];
     const int wpb = blockDim.x >> 6;
-    const long nwaves = (long)gridDim.x * wpb;
-    const int cpw = dev_gc_block(n, nwaves, FR_GC_FOOT_MIN);
+    FR_GC_XCD_RANGE(n, FR_GC_FOOT_MIN)
     const int lane = dev_lane();
-    const long gw = (long)blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    for (long b0 = gw * cpw; b0 < n; b0 += nwaves * cpw) {
-        const int cnt = (int)(n - b0 < cpw ? n - b0 : cpw);
+    for (long b0 = lo + gw * cpw; b0 < hi; b0 += nwaves * cpw) {
+        const int cnt = (int)(hi - b0 < cpw ? hi - b0 : cpw);
         int2 d = make_int2(0, 0);
         FrHermite hm = {};
         int np = 0;
