@@ -620,6 +620,8 @@ def main():
                "wall_ms_per_step": float(np.mean(walls)),
                "found_p50_p99_max_ms": [round(float(np.percentile(lat_ok, q)), 3) for q in (50, 99, 100)] if lat_ok else None,
                "failed_p50_p99_max_ms": [round(float(np.percentile(lat_fail, q)), 3) for q in (50, 99, 100)] if lat_fail else None,
+               # plans whose result came back after their deadline (submit -> result > budget)
+               "deadline_miss_rate": round(float(np.mean(lat > args.budget_ms)), 5),
                "validation": validate_paths([(r.tag, scene_of[r.tag], r.path) for r in found_b]),
                "hbm": batch.memory()}
     # plain form on the plans of the timed steps (no kernel timing: the
