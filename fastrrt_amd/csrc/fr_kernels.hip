@@ -2223,6 +2223,15 @@ __device__ __forceinline__ long long dev_pair_lcost(const FrPlanDev &P, int node
     return (long long)floor((double)(100.0f * total));
 }
 
+/* First index and stride of a one-wave-per-item loop over [0, n) whose
+ * waves take the items of their XCD's contiguous eighth (block b runs on
+ * XCD b % 8; the grid is a multiple of 8 blocks). */
+#define FR_XCD_ITEMS(T, n, first, stride, end)                                                             \
+    const T end = (T)((long)(n) * (int)((blockIdx.x & 7u) + 1) / 8);                                        \
+    const T first = (T)((long)(n) * (int)(blockIdx.x & 7u) / 8) + (T)(blockIdx.x >> 3) * wpb +              \
+                    (T)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));                             \
+    const T stride = (T)(gridDim.x >> 3) * wpb;
+
 /* Cells of the new nodes' windows whose goal-connection planes are not yet
  * evaluated (lazy plans): claimed a word at a time with atomicOr on the
  * evaluated-bit plane, queued. */
@@ -2231,9 +2240,9 @@ __global__ void __launch_bounds__(FR_BLOCK) k_gc_demand(const FrPlanDev *plans, 
 {
     const int n = counters[FR_CNT_FRESH];
     const int wpb = blockDim.x >> 6;
-    const int nw = gridDim.x * wpb;
     const int lane = dev_lane();
-    for (int i = blockIdx.x * wpb + (int)(threadIdx.x >> 6); i < n; i += nw) {
+    FR_XCD_ITEMS(int, n, i0, nw, iend)
+    for (int i = i0; i < iend; i += nw) {
         const FrFresh f = fq[i];
         const FrPlanDev &P = plans[f.slot];
         if (!P.gc_lazy || P.type[f.pos] != FR_TYPE_NODE)
@@ -2451,9 +2460,9 @@ __global__ void __launch_bounds__(FR_BLOCK) k_connect_emit(const FrPlanDev *plan
 {
     const int n = counters[FR_CNT_FRESH];
     const int wpb = blockDim.x >> 6;
-    const int nw = gridDim.x * wpb;
     const int lane = dev_lane();
-    for (int i = blockIdx.x * wpb + (int)(threadIdx.x >> 6); i < n; i += nw) {
+    FR_XCD_ITEMS(int, n, i0, nw, iend)
+    for (int i = i0; i < iend; i += nw) {
         FrFresh &f = fq[i];
         const int slot = f.slot, fpos = f.pos;
         const FrPlanDev &P = plans[slot];
@@ -2581,8 +2590,8 @@ __global__ void __launch_bounds__(FR_BLOCK) k_connect_eval(const FrPlanDev *plan
         n = pair_cap;
     const int n_fresh = counters[FR_CNT_FRESH];
     const int wpb = blockDim.x >> 6;
-    const long nw = (long)gridDim.x * wpb;
-    for (long p = blockIdx.x * (long)wpb + (threadIdx.x >> 6); p < n; p += nw) {
+    FR_XCD_ITEMS(long, n, p0, nw, pend)
+    for (long p = p0; p < pend; p += nw) {
         const FrPair pr = pairs[p];
         if (pr.fresh < 0 || pr.fresh >= n_fresh)
             continue; /* slot reserved by a node whose pairs did not fit */
