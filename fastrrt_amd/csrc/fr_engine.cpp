@@ -445,7 +445,8 @@ void FrEngine::release()
 
 void FrEngine::set_grids(int waves, int tiles, int items)
 {
-    /* k_expand deals its queue to the eight XCDs in blocks of 8 */
+    /* k_expand, the goal-curve kernels and the fresh-node and pair kernels
+     * deal their queues to the eight XCDs in blocks of 8 */
     if (waves < 8 || waves > 65535 || waves % 8 != 0)
         throw std::runtime_error("grid_waves must be a multiple of 8 in [8, 65535]");
     if (tiles < 1 || tiles > 65535 || items < 1 || items > 65535)

@@ -1359,6 +1359,19 @@ __device__ __forceinline__ int dev_wave_reserve(int *counter, int cnt, int *tota
     return dev_bcast_i(at, 0) + incl - cnt;
 }
 
+/* First index and stride of a one-wave-per-item loop over [0, n) whose
+ * waves take the items of their XCD's contiguous eighth (block b runs on
+ * XCD b % 8) when the grid is a multiple of 8 blocks, and every
+ * gridDim-th wave's item otherwise. */
+#define FR_XCD_ITEMS(T, n, first, stride, end)                                                             \
+    const bool first##_x8 = (gridDim.x & 7u) == 0;                                                          \
+    const int first##_x = first##_x8 ? (int)(blockIdx.x & 7u) : 0;                                          \
+    const int first##_d = first##_x8 ? 8 : 1;                                                               \
+    const T end = (T)((long)(n) * (first##_x + 1) / first##_d);                                             \
+    const T first = (T)((long)(n) * first##_x / first##_d) + (T)(blockIdx.x / first##_d) * wpb +            \
+                    (T)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));                             \
+    const T stride = (T)(gridDim.x / first##_d) * wpb;
+
 /* Gate of __CUDA_random_node_expansion (graph_expansion.cu:157-164) and of
  * __CUDA_smart_node_expansion (graph_expansion_search_zones.cu:461-473):
  * queue one rollout per eligible NODE cell. */
@@ -2222,15 +2235,6 @@ __device__ __forceinline__ long long dev_pair_lcost(const FrPlanDev &P, int node
     const float total = c + P.gc_cost[cell];
     return (long long)floor((double)(100.0f * total));
 }
-
-/* First index and stride of a one-wave-per-item loop over [0, n) whose
- * waves take the items of their XCD's contiguous eighth (block b runs on
- * XCD b % 8; the grid is a multiple of 8 blocks). */
-#define FR_XCD_ITEMS(T, n, first, stride, end)                                                             \
-    const T end = (T)((long)(n) * (int)((blockIdx.x & 7u) + 1) / 8);                                        \
-    const T first = (T)((long)(n) * (int)(blockIdx.x & 7u) / 8) + (T)(blockIdx.x >> 3) * wpb +              \
-                    (T)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));                             \
-    const T stride = (T)(gridDim.x >> 3) * wpb;
 
 /* Cells of the new nodes' windows whose goal-connection planes are not yet
  * evaluated (lazy plans): claimed a word at a time with atomicOr on the
